@@ -1,0 +1,116 @@
+/*
+ * pbrtx_gpu.h -- C ABI of the MI355X (gfx950) ray-query and sampling path.
+ *
+ * Drop-in boundary for pbrt_rust's aggregate / renderer interfaces:
+ *   Intersectable::{intersect, intersect_p}   src/intersection.rs:61-66
+ *   BVHAccelerator::{new, intersect}          src/primitive/aggregates/bvh.rs:335-422
+ *   Aggregate dispatch                        src/primitive/aggregates/mod.rs:47-63
+ *   Renderer::render / SamplerRenderer        src/renderer.rs:8-26, src/sampler_renderer.rs:146-206
+ * Every function returns 0 (PBRTX_OK) or a negative PBRTX_ERR_* code and never
+ * aborts; pbrtx_last_error() gives the message of the calling thread's last
+ * failure.  Pointers named d_* are device (HBM) pointers; `stream` is a
+ * hipStream_t (NULL = the default stream); queries are asynchronous on it.
+ */
+#ifndef PBRTX_GPU_H
+#define PBRTX_GPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBRTX_OK 0
+#define PBRTX_ERR_ARG (-1)
+#define PBRTX_ERR_HIP (-2)
+#define PBRTX_ERR_OOM (-3)
+#define PBRTX_ERR_BUILD (-4)
+#define PBRTX_ERR_STATE (-5)
+
+/* traversal paths */
+#define PBRTX_PATH_PLAIN 0   /* one lane per ray, reference node layout */
+#define PBRTX_PATH_BATCHED 1 /* persistent waves, child-pair nodes */
+
+/* Ray record (32 B): Ray{o, d, mint, maxt} of src/ray.rs:7-61 without the
+   time / depth fields the queries do not read. */
+typedef struct pbrtx_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} pbrtx_ray;
+
+/* Hit record (16 B): the closest hit's primitive, t and barycentrics
+   (Intersection of src/intersection.rs:14-23 reduced to what shading needs;
+   dg is rebuilt from these).  prim = input prim id (mesh faces in face order,
+   then spheres) or 0xFFFFFFFF on a miss. */
+typedef struct pbrtx_hit {
+    uint32_t prim;
+    float t, b1, b2;
+} pbrtx_hit;
+
+/* Sphere primitive (Shape::sphere, src/shape/sphere.rs:28-44). */
+typedef struct pbrtx_sphere {
+    float o2w[16];     /* object-to-world Transform m, row-major */
+    float o2w_inv[16]; /* its m_inv */
+    float radius, z0, z1, phimax_deg;
+    int32_t reverse_orientation;
+} pbrtx_sphere;
+
+typedef struct pbrtx_bvh pbrtx_bvh;     /* host-side build result */
+typedef struct pbrtx_scene pbrtx_scene; /* device-resident BVH + primitives */
+
+const char* pbrtx_last_error(void);
+
+/* ---- BVH build: BVHAccelerator::new (bvh.rs:335-362) --------------------- */
+/* Builds the reference's tree on the host over mesh faces (world-space
+   vertices, index triples) followed by spheres.  split: 0 "sah", 1 "middle",
+   2 "equal" (bvh.rs:341-349).  Faces become primitives in face order
+   (Primitive::fully_refine + Mesh::refine, primitive/mod.rs:47-62,
+   mesh.rs:324-338). */
+int pbrtx_bvh_build(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
+                    const pbrtx_sphere* spheres, uint32_t nspheres, int32_t split,
+                    uint32_t max_prims, pbrtx_bvh** out);
+/* node count, deepest level, prim count, host build time */
+int pbrtx_bvh_info(const pbrtx_bvh* bvh, uint32_t* n_nodes, uint32_t* depth, uint32_t* n_prims,
+                   double* build_ms);
+/* the flattened PackedBVHNode array (bvh.rs:260-326): bounds[6n], off[n]
+   (second child or prim offset), meta[n] (axis or prim count), leaf[n], and
+   the BVH-ordered prim ids order[n_prims] */
+int pbrtx_bvh_export(const pbrtx_bvh* bvh, float* bounds, uint32_t* off, uint32_t* meta,
+                     uint8_t* leaf, uint32_t* order);
+int pbrtx_bvh_free(pbrtx_bvh* bvh);
+
+/* ---- device scene -------------------------------------------------------- */
+/* Copies both node layouts and the primitive records to HBM.  The handle is
+   immutable afterwards and may be shared by host threads. */
+int pbrtx_scene_create(const pbrtx_bvh* bvh, pbrtx_scene** out);
+int pbrtx_scene_destroy(pbrtx_scene* scene);
+int pbrtx_scene_device_bytes(const pbrtx_scene* scene, uint64_t* bytes);
+
+/* ---- queries ------------------------------------------------------------- */
+/* Intersectable::intersect over n rays (BVHAccelerator::intersect,
+   bvh.rs:377-421): d_hits[i] = closest hit; on a hit d_rays[i].tmax = t, the
+   reference's maxt side effect (geometric.rs:64). */
+int pbrtx_intersect(pbrtx_scene* scene, pbrtx_ray* d_rays, uint32_t n, pbrtx_hit* d_hits,
+                    int32_t path, void* stream);
+/* Intersectable::intersect_p (intersection.rs:63-65): bit i of
+   d_occluded[ceil(n/32)] = 1 iff ray i hits anything in [tmin, tmax]. */
+int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
+                      uint32_t* d_occluded, int32_t path, void* stream);
+
+/* Launch shape of a path: block in {64,128,256}; blocks_per_cu 0 = occupancy
+   maximum (batched path); refill_pct = idle-lane refill threshold. */
+int pbrtx_set_tuning(int32_t path, int32_t block, int32_t blocks_per_cu, int32_t refill_pct);
+
+/* ---- device memory and timing plumbing ----------------------------------- */
+int pbrtx_device_malloc(void** d_ptr, uint64_t bytes);
+int pbrtx_device_free(void* d_ptr);
+int pbrtx_memcpy_htod(void* d_dst, const void* src, uint64_t bytes);
+int pbrtx_memcpy_dtoh(void* dst, const void* d_src, uint64_t bytes);
+int pbrtx_synchronize(void);
+/* HIP-event timer on a stream: start, then stop returns elapsed ms */
+int pbrtx_timer_start(void* stream);
+int pbrtx_timer_stop(void* stream, float* ms);
+int pbrtx_device_name(char* buf, uint32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,15 @@
+"""pbrt_rust_amd -- MI355X (gfx950) ray-query and sampling path of pbrt_rust.
+
+The compute runs in hand-written HIP kernels inside libpbrtx_gpu.so (C ABI:
+include/pbrtx_gpu.h); this package is the host-side mirror of the reference's
+aggregate / renderer interfaces.
+"""
+from ._lib import HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, device_name, lib, synchronize
+from .api import (Aggregate, BVHAccelerator, Primitive, RayBatch, Sphere, Transform, TriangleMesh,
+                  set_tuning)
+
+__all__ = [
+    "Aggregate", "BVHAccelerator", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "Primitive",
+    "RAY_DTYPE", "RayBatch", "Sphere", "Transform", "TriangleMesh", "device_name", "lib", "set_tuning",
+    "synchronize",
+]

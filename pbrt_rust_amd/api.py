@@ -1,0 +1,293 @@
+"""Host-side mirror of pbrt_rust's aggregate interface over the HIP path.
+
+Names and argument meanings follow the reference:
+  Shape::triangle_mesh / Shape::sphere      src/shape/mod.rs:170-190
+  Primitive::simple                          src/primitive/mod.rs:79-85
+  BVHAccelerator::new(p, max_prims, "sah"|"middle"|"equal")
+                                             src/primitive/aggregates/bvh.rs:335-362
+  Intersectable::{intersect, intersect_p}    src/intersection.rs:61-66
+  Aggregate::bvh                             src/primitive/aggregates/mod.rs:27-29
+Rays are batches (RayBatch) rather than single Ray objects; intersect sets
+each hit ray's tmax to the hit t, the reference's maxt side effect
+(src/primitive/geometric.rs:64).
+"""
+import ctypes
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, check, lib
+
+F32 = np.float32
+F32_MAX = float(np.finfo(np.float32).max)
+
+PATHS = {"plain": 0, "batched": 1}
+
+
+def _path_code(path):
+    if isinstance(path, str):
+        if path not in PATHS:
+            raise ValueError("unknown path %r" % path)
+        return PATHS[path]
+    return int(path)
+
+
+class Transform:
+    """Transform{m, m_inv} (src/transform/transform.rs:14-19) in float32."""
+
+    def __init__(self, m=None, m_inv=None):
+        self.m = np.eye(4, dtype=F32) if m is None else np.asarray(m, F32).reshape(4, 4)
+        self.m_inv = np.eye(4, dtype=F32) if m_inv is None else np.asarray(m_inv, F32).reshape(4, 4)
+
+    @staticmethod
+    def translate(v):
+        t = Transform()
+        t.m[:3, 3] = np.asarray(v, F32)
+        t.m_inv[:3, 3] = -np.asarray(v, F32)
+        return t
+
+    @staticmethod
+    def scale(x, y, z):
+        t = Transform()
+        t.m[0, 0], t.m[1, 1], t.m[2, 2] = F32(x), F32(y), F32(z)
+        t.m_inv[0, 0], t.m_inv[1, 1], t.m_inv[2, 2] = F32(1) / F32(x), F32(1) / F32(y), F32(1) / F32(z)
+        return t
+
+    def inverse(self):
+        return Transform(self.m_inv.copy(), self.m.copy())
+
+    @staticmethod
+    def _matmul(a, b):
+        # Matrix4x4 product summed left to right (matrix4x4.rs:167-182)
+        r = np.zeros((4, 4), F32)
+        for i in range(4):
+            for j in range(4):
+                s = F32(a[i, 0] * b[0, j])
+                s = F32(s + F32(a[i, 1] * b[1, j]))
+                s = F32(s + F32(a[i, 2] * b[2, j]))
+                r[i, j] = F32(s + F32(a[i, 3] * b[3, j]))
+        return r
+
+    def __mul__(self, o):
+        return Transform(self._matmul(self.m, o.m), self._matmul(o.m_inv, self.m_inv))
+
+    def xf_points(self, p):
+        """ApplyTransform<Point> (transform.rs:195-207) on an (n,3) array."""
+        p = np.asarray(p, F32).reshape(-1, 3)
+        m = self.m
+        x, y, z = p[:, 0], p[:, 1], p[:, 2]
+        rows = []
+        for i in range(4):
+            s = m[i, 0] * x
+            s = s + m[i, 1] * y
+            s = s + m[i, 2] * z
+            rows.append(s + m[i, 3])
+        xt, yt, zt, w = rows
+        one = w == F32(1)
+        out = np.stack([np.where(one, xt, xt / w), np.where(one, yt, yt / w), np.where(one, zt, zt / w)], 1)
+        return out.astype(F32)
+
+
+class TriangleMesh:
+    """Shape::triangle_mesh (shape/mod.rs:185-190): Mesh::new pre-transforms
+    the points to world space (mesh.rs:299-314)."""
+
+    def __init__(self, o2w, w2o, reverse_orientation, vi, p):
+        vi = np.asarray(vi, np.uint32).reshape(-1)
+        if vi.size % 3 != 0:
+            raise ValueError("vertex index count must be a multiple of 3 (mesh.rs:303)")
+        o2w = o2w or Transform()
+        self.p = o2w.xf_points(p)
+        self.vi = vi
+        self.reverse_orientation = bool(reverse_orientation)
+
+    @property
+    def nfaces(self):
+        return self.vi.size // 3
+
+
+class Sphere:
+    """Shape::sphere (shape/mod.rs:170-173, sphere.rs:28-44)."""
+
+    def __init__(self, o2w, w2o, reverse_orientation, radius, z0, z1, phimax):
+        if not radius > 0:
+            raise ValueError("sphere radius must be positive (sphere.rs:30)")
+        self.o2w = o2w
+        self.reverse_orientation = bool(reverse_orientation)
+        self.radius, self.z0, self.z1, self.phimax = float(radius), float(z0), float(z1), float(phimax)
+
+
+class Primitive:
+    """Primitive::simple (primitive/mod.rs:79-85): a shape in the aggregate."""
+
+    def __init__(self, shape):
+        self.shape = shape
+
+    @staticmethod
+    def simple(shape):
+        return Primitive(shape)
+
+
+class RayBatch:
+    """n Ray records {o, d, mint, maxt} (src/ray.rs:7-38); maxt defaults to
+    f32::MAX as Ray::new_with does."""
+
+    def __init__(self, o, d, mint=0.0, maxt=F32_MAX):
+        o = np.asarray(o, F32).reshape(-1, 3)
+        d = np.asarray(d, F32).reshape(-1, 3)
+        n = o.shape[0]
+        a = np.zeros(n, RAY_DTYPE)
+        a["ox"], a["oy"], a["oz"] = o[:, 0], o[:, 1], o[:, 2]
+        a["dx"], a["dy"], a["dz"] = d[:, 0], d[:, 1], d[:, 2]
+        a["tmin"] = np.broadcast_to(np.asarray(mint, F32), (n,))
+        a["tmax"] = np.broadcast_to(np.asarray(maxt, F32), (n,))
+        self.a = a
+
+    def __len__(self):
+        return self.a.shape[0]
+
+    @property
+    def maxt(self):
+        return self.a["tmax"]
+
+
+class BVHAccelerator:
+    """GPU aggregate with the reference's BVHAccelerator::new signature."""
+
+    def __init__(self, prims, max_prims=1, split_method="sah"):
+        split = {"sah": 0, "middle": 1, "equal": 2}.get(split_method)
+        if split is None:
+            warnings.warn('BVH split method %s unknown. Using "SAH"' % split_method)  # bvh.rs:345-348
+            split = 0
+        meshes = [p.shape for p in prims if isinstance(p.shape, TriangleMesh)]
+        spheres = [p.shape for p in prims if isinstance(p.shape, Sphere)]
+        if len(meshes) + len(spheres) != len(prims):
+            raise TypeError("only triangle meshes and spheres are supported")
+        kinds = [isinstance(p.shape, Sphere) for p in prims]
+        if kinds != sorted(kinds):
+            raise ValueError("meshes must precede spheres in the primitive list")
+        # one vertex / index array over all meshes, faces in input order
+        verts, idx, base = [], [], 0
+        for m in meshes:
+            verts.append(m.p)
+            idx.append(m.vi + base)
+            base += m.p.shape[0]
+        self.verts = np.ascontiguousarray(np.concatenate(verts) if verts else np.zeros((0, 3), F32))
+        self.idx = np.ascontiguousarray(np.concatenate(idx) if idx else np.zeros(0, np.uint32))
+        sph = (_lib.SphereDesc * max(1, len(spheres)))()
+        for k, s in enumerate(spheres):
+            for i, v in enumerate(s.o2w.m.reshape(16)):
+                sph[k].o2w[i] = float(v)
+            for i, v in enumerate(s.o2w.m_inv.reshape(16)):
+                sph[k].o2w_inv[i] = float(v)
+            sph[k].radius, sph[k].z0, sph[k].z1 = s.radius, s.z0, s.z1
+            sph[k].phimax_deg = s.phimax
+            sph[k].reverse_orientation = int(s.reverse_orientation)
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.pbrtx_bvh_build(self.verts.ctypes.data if self.verts.size else None,
+                                self.verts.shape[0], self.idx.ctypes.data if self.idx.size else None,
+                                self.idx.size // 3, ctypes.cast(sph, ctypes.c_void_p) if spheres else None,
+                                len(spheres), split, int(max_prims), ctypes.byref(h)))
+        self._bvh = h.value
+        self.nfaces = self.idx.size // 3
+        self.nspheres = len(spheres)
+        self._scene = None
+
+    # ---- host-side tree -------------------------------------------------------
+    def info(self):
+        n, d, p = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        ms = ctypes.c_double()
+        check(lib().pbrtx_bvh_info(self._bvh, ctypes.byref(n), ctypes.byref(d), ctypes.byref(p), ctypes.byref(ms)))
+        return dict(nodes=n.value, depth=d.value, prims=p.value, build_ms=ms.value)
+
+    def nodes(self):
+        info = self.info()
+        n, npr = info["nodes"], info["prims"]
+        b = np.zeros((n, 6), F32)
+        off = np.zeros(n, np.uint32)
+        meta = np.zeros(n, np.uint32)
+        leaf = np.zeros(n, np.uint8)
+        order = np.zeros(npr, np.uint32)
+        check(lib().pbrtx_bvh_export(self._bvh, b.ctypes.data, off.ctypes.data, meta.ctypes.data,
+                                     leaf.ctypes.data, order.ctypes.data))
+        return b, off, meta, leaf, order
+
+    def world_bound(self):
+        b = self.nodes()[0]
+        return b[0, :3].copy(), b[0, 3:].copy()
+
+    # ---- device scene ---------------------------------------------------------
+    @property
+    def scene(self):
+        if self._scene is None:
+            s = ctypes.c_void_p()
+            check(lib().pbrtx_scene_create(self._bvh, ctypes.byref(s)))
+            self._scene = s.value
+        return self._scene
+
+    def device_bytes(self):
+        b = ctypes.c_uint64()
+        check(lib().pbrtx_scene_device_bytes(self.scene, ctypes.byref(b)))
+        return b.value
+
+    def intersect_device(self, d_rays, n, d_hits, path="batched", stream=None):
+        check(lib().pbrtx_intersect(self.scene, d_rays, n, d_hits, _path_code(path), stream))
+
+    def intersect_p_device(self, d_rays, n, d_bits, path="batched", stream=None):
+        check(lib().pbrtx_intersect_p(self.scene, d_rays, n, d_bits, _path_code(path), stream))
+
+    def intersect(self, rays, path="batched"):
+        """Intersectable::intersect for a RayBatch: returns the hit records and
+        writes each hit t into rays.maxt."""
+        n = len(rays)
+        if n == 0:
+            return np.zeros(0, HIT_DTYPE)
+        dr = DeviceBuffer.from_array(rays.a)
+        dh = DeviceBuffer(n * HIT_DTYPE.itemsize)
+        self.intersect_device(dr.ptr, n, dh.ptr, path)
+        _lib.synchronize()
+        rays.a[:] = dr.download(RAY_DTYPE, n)
+        return dh.download(HIT_DTYPE, n)
+
+    def intersect_p(self, rays, path="batched"):
+        """Intersectable::intersect_p for a RayBatch -> bool array."""
+        n = len(rays)
+        if n == 0:
+            return np.zeros(0, bool)
+        nw = (n + 31) // 32
+        dr = DeviceBuffer.from_array(rays.a)
+        db = DeviceBuffer(nw * 4)
+        self.intersect_p_device(dr.ptr, n, db.ptr, path)
+        _lib.synchronize()
+        words = db.download(np.uint32, nw)
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n]
+        return bits.astype(bool)
+
+    def close(self):
+        L = lib()
+        if self._scene:
+            L.pbrtx_scene_destroy(self._scene)
+            self._scene = None
+        if self._bvh:
+            L.pbrtx_bvh_free(self._bvh)
+            self._bvh = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Aggregate:
+    """Aggregate::bvh (aggregates/mod.rs:27-29)."""
+
+    @staticmethod
+    def bvh(prims, max_prims, sm):
+        return BVHAccelerator(prims, max_prims, sm)
+
+
+def set_tuning(path, block=256, blocks_per_cu=0, refill_pct=50):
+    check(lib().pbrtx_set_tuning(_path_code(path), block, blocks_per_cu, refill_pct))

@@ -1,0 +1,402 @@
+// bvh_build.cpp -- host-side BVH construction (BVHAccelerator::new,
+// primitive/aggregates/bvh.rs:105-362) producing the reference's tree exactly,
+// written as an iterative pre-order build over one index permutation with
+// in-place partitions (no per-node allocation), then packed into the two
+// GPU layouts (32-byte reference nodes and 64-byte child-pair nodes).
+#include "bvh_build.hpp"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+namespace px {
+namespace {
+
+struct Box {
+    f3 lo, hi;
+};
+// f32::min / f32::max ignore a NaN operand (fmin / fmax); argument order as
+// in bbox.rs:155-183 so signed zeros resolve as they do there.
+inline float rmin(float a, float b) { return fminf(a, b); }
+inline float rmax(float a, float b) { return fmaxf(a, b); }
+inline float comp(const f3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// BBox::new: bbox.rs:36-41
+inline Box box_empty() { return Box{mk3(FLT_MAX, FLT_MAX, FLT_MAX), mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX)}; }
+// Union<Point>: bbox.rs:155-168
+inline Box box_union_p(const Box& b, const f3& p) {
+    return Box{mk3(rmin(b.lo.x, p.x), rmin(b.lo.y, p.y), rmin(b.lo.z, p.z)),
+               mk3(rmax(b.hi.x, p.x), rmax(b.hi.y, p.y), rmax(b.hi.z, p.z))};
+}
+// Union<BBox>: bbox.rs:170-183
+inline Box box_union(const Box& a, const Box& b) {
+    return Box{mk3(rmin(a.lo.x, b.lo.x), rmin(a.lo.y, b.lo.y), rmin(a.lo.z, b.lo.z)),
+               mk3(rmax(a.hi.x, b.hi.x), rmax(a.hi.y, b.hi.y), rmax(a.hi.z, b.hi.z))};
+}
+// surface_area: bbox.rs:73-78
+inline float surface_area(const Box& b) {
+    float dx = rmax(b.hi.x - b.lo.x, 0.0f);
+    float dy = rmax(b.hi.y - b.lo.y, 0.0f);
+    float dz = rmax(b.hi.z - b.lo.z, 0.0f);
+    return 2.0f * (dx * dy + dx * dz + dy * dz);
+}
+// max_extent: bbox.rs:87-96
+inline int max_extent(const Box& b) {
+    f3 d = sub3(b.hi, b.lo);
+    if (d.x > d.y && d.x > d.z) return 0;
+    if (d.y > d.z) return 1;
+    return 2;
+}
+
+struct M4 {
+    float m[4][4];
+};
+inline M4 load_m4(const float* a) {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = a[4 * i + j];
+    return r;
+}
+// ApplyTransform<Point>: transform.rs:195-207
+inline f3 xf_point(const M4& m, f3 p) {
+    float x = p.x, y = p.y, z = p.z;
+    float xt = m.m[0][0] * x + m.m[0][1] * y + m.m[0][2] * z + m.m[0][3];
+    float yt = m.m[1][0] * x + m.m[1][1] * y + m.m[1][2] * z + m.m[1][3];
+    float zt = m.m[2][0] * x + m.m[2][1] * y + m.m[2][2] * z + m.m[2][3];
+    float w = m.m[3][0] * x + m.m[3][1] * y + m.m[3][2] * z + m.m[3][3];
+    if (w != 1.0f) return mk3(xt / w, yt / w, zt / w);
+    return mk3(xt, yt, zt);
+}
+// ApplyTransform<Vector>: transform.rs:210-218
+inline f3 xf_vector(const M4& m, f3 v) {
+    return mk3(m.m[0][0] * v.x + m.m[0][1] * v.y + m.m[0][2] * v.z,
+               m.m[1][0] * v.x + m.m[1][1] * v.y + m.m[1][2] * v.z,
+               m.m[2][0] * v.x + m.m[2][1] * v.y + m.m[2][2] * v.z);
+}
+
+// partition_by: utils/mod.rs:112-169 over prim indices keyed by centroid[dim].
+// (The reference recurses on a sub-slice with that sub-slice's midpoint; the
+// loop keeps that.)
+void partition_by(uint32_t* v, size_t nv, const std::vector<f3>& cen, int dim) {
+    auto key = [&](uint32_t p) { return comp(cen[p], dim); };
+    while (true) {
+        if (nv < 3) {
+            if (nv == 2 && key(v[1]) < key(v[0])) std::swap(v[0], v[1]);
+            return;
+        }
+        float fst = key(v[0]), mid = key(v[nv / 2]), lst = key(v[nv - 1]);
+        float pivot;
+        if (fst < mid && mid < lst) pivot = mid;
+        else if (mid < fst && fst < lst) pivot = fst;
+        else pivot = lst;
+        size_t ls = 0, np = 0;
+        for (size_t i = 0; i < nv; ++i) {
+            float bv = key(v[i]);
+            if (bv < pivot) {
+                std::swap(v[ls + np], v[i]);
+                std::swap(v[ls + np], v[ls]);
+                ls += 1;
+            } else if (bv == pivot) {
+                std::swap(v[ls + np], v[i]);
+                np += 1;
+            }
+        }
+        size_t pidx = std::max<size_t>(ls, 1);
+        if (pidx + np <= nv / 2) {
+            v += pidx;
+            nv -= pidx;
+        } else if (pidx >= nv / 2) {
+            nv = pidx;
+        } else {
+            return;
+        }
+    }
+}
+
+template <typename Pred>
+size_t stable_partition(uint32_t* a, size_t n, Pred pred, uint32_t* tmp) {
+    size_t l = 0, r = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t x = a[i];
+        if (pred(x)) a[l++] = x;
+        else tmp[r++] = x;
+    }
+    if (r) memcpy(a + l, tmp, r * sizeof(uint32_t));
+    return l;
+}
+
+// bucket_for_prim: bvh.rs:118-130 (`as usize` saturates: NaN and negatives -> 0)
+inline int bucket_for(float c, float cmin, float cmax) {
+    float pdist = c - cmin;
+    float dist = cmax - cmin;
+    float f = 12.0f * (pdist / dist);
+    size_t b = (f != f || f <= 0.0f) ? 0 : (size_t)f;
+    if (b >= 12) b = 11;
+    return (int)b;
+}
+
+inline float bits_f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
+               const SphereIn* sph, uint32_t nsph, int split, uint32_t max_prims, HostBvh* out,
+               std::string* err) {
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t N64 = (uint64_t)nfaces + nsph;
+    if (N64 == 0) { *err = "scene has no primitives"; return false; }
+    if (N64 >= (1ull << 30)) { *err = "too many primitives"; return false; }
+    if (split < 0 || split > 2) { *err = "unknown split method"; return false; }
+    if (nfaces && (!verts || !idx)) { *err = "null mesh arrays"; return false; }
+    const uint32_t N = (uint32_t)N64;
+    for (uint64_t i = 0; i < 3ull * nfaces; ++i)
+        if (idx[i] >= nverts) { *err = "vertex index out of range"; return false; }
+
+    std::vector<Box> pb(N);
+    std::vector<f3> pc(N);
+    auto V = [&](uint32_t i) { return mk3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]); };
+    // Triangle::world_bound: mesh.rs:196-205 over the refined triple [c,b,a]
+    for (uint32_t f = 0; f < nfaces; ++f) {
+        Box b = box_empty();
+        b = box_union_p(b, V(idx[3 * f + 2]));
+        b = box_union_p(b, V(idx[3 * f + 1]));
+        b = box_union_p(b, V(idx[3 * f + 0]));
+        pb[f] = b;
+    }
+    out->spheres.resize(nsph);
+    for (uint32_t s = 0; s < nsph; ++s) {
+        const SphereIn& in = sph[s];
+        if (!(in.radius > 0.0f)) { *err = "sphere radius must be positive"; return false; }
+        SphereRec& rec = out->spheres[s];
+        float r = in.radius;
+        // Sphere::new: sphere.rs:28-44
+        rec.radius = r;
+        rec.zmin = std::min(std::max(fminf(in.z0, in.z1), -r), r);
+        rec.zmax = std::min(std::max(fmaxf(in.z0, in.z1), -r), r);
+        rec.phimax = std::min(std::max(in.phimax_deg, 0.0f), 360.0f) * PI_F / 180.0f;
+        memcpy(rec.w2o, in.o2w_inv, sizeof(rec.w2o));  // Transform::inverse swaps m, m_inv
+        // world_bound: sphere.rs:111-126 + transform.rs:248-264
+        M4 m = load_m4(in.o2w);
+        Box ob{mk3(-r, -r, rec.zmin), mk3(r, r, rec.zmax)};
+        f3 tx = xf_vector(m, mk3(ob.hi.x - ob.lo.x, 0.0f, 0.0f));
+        f3 ty = xf_vector(m, mk3(0.0f, ob.hi.y - ob.lo.y, 0.0f));
+        f3 tz = xf_vector(m, mk3(0.0f, 0.0f, ob.hi.z - ob.lo.z));
+        f3 tp = xf_point(m, ob.lo);
+        Box b{tp, tp};
+        b = box_union_p(b, add3(tp, tx));
+        b = box_union_p(b, add3(tp, ty));
+        b = box_union_p(b, add3(tp, tz));
+        b = box_union_p(b, add3(add3(tp, tx), ty));
+        b = box_union_p(b, add3(add3(tp, tx), tz));
+        b = box_union_p(b, add3(add3(tp, ty), tz));
+        b = box_union_p(b, add3(add3(add3(tp, tx), ty), tz));
+        pb[nfaces + s] = b;
+    }
+    for (uint32_t i = 0; i < N; ++i) pc[i] = mul3(add3(pb[i].lo, pb[i].hi), 0.5f);  // bvh.rs:77
+
+    std::vector<uint32_t> ord(N), tmp(N);
+    for (uint32_t i = 0; i < N; ++i) ord[i] = i;
+    std::vector<RNode>& nodes = out->rnodes;
+    nodes.clear();
+    nodes.reserve(2 * (size_t)N);
+    std::vector<Box> nb;
+    nb.reserve(2 * (size_t)N);
+
+    struct Task {
+        uint32_t b, e;
+        int64_t parent;
+        uint32_t depth;
+    };
+    std::vector<Task> stack;
+    stack.push_back(Task{0, N, -1, 0});
+    uint32_t maxdepth = 0;
+    std::vector<uint8_t> bk;  // bucket per position of the current range
+    while (!stack.empty()) {
+        Task T = stack.back();
+        stack.pop_back();
+        const uint32_t id = (uint32_t)nodes.size();
+        nodes.push_back(RNode{});
+        nb.push_back(Box{});
+        if (T.parent >= 0) nodes[(size_t)T.parent].off = id;
+        maxdepth = std::max(maxdepth, T.depth);
+        const uint32_t b = T.b, e = T.e, n = e - b;
+        // recursive_build bbox: bvh.rs:192-194 (fold in the range's order)
+        Box bb = box_empty();
+        for (uint32_t i = b; i < e; ++i) bb = box_union(bb, pb[ord[i]]);
+        auto make_leaf = [&]() {
+            nodes[id].off = b;
+            nodes[id].meta = 3u | (n << 2);
+            nb[id] = bb;
+        };
+        if (n == 1) { make_leaf(); continue; }
+        Box cb = box_empty();
+        for (uint32_t i = b; i < e; ++i) cb = box_union_p(cb, pc[ord[i]]);
+        const int dim = max_extent(cb);
+        const float cmin = comp(cb.lo, dim), cmax = comp(cb.hi, dim);
+        if (cmin == cmax) { make_leaf(); continue; }
+        uint32_t mid;
+        bool sah_leaf = false;
+        auto equal_counts = [&]() {
+            partition_by(ord.data() + b, n, pc, dim);
+            return b + n / 2;
+        };
+        if (split == 1) {
+            // split_middle: bvh.rs:86-90
+            float pmid = 0.5f * (cmin + cmax);
+            mid = b + (uint32_t)stable_partition(
+                          ord.data() + b, n, [&](uint32_t p) { return comp(pc[p], dim) < pmid; }, tmp.data());
+        } else if (split == 2 || n <= 4) {
+            mid = equal_counts();
+        } else {
+            // split_surface_area_heuristic: bvh.rs:105-186
+            uint32_t cnt[12] = {0};
+            Box bbk[12];
+            for (int k = 0; k < 12; ++k) bbk[k] = box_empty();
+            bk.resize(n);
+            for (uint32_t i = 0; i < n; ++i) {
+                uint32_t p = ord[b + i];
+                int k = bucket_for(comp(pc[p], dim), cmin, cmax);
+                bk[i] = (uint8_t)k;
+                cnt[k] += 1;
+                bbk[k] = box_union(bbk[k], pb[p]);
+            }
+            const float tsa = surface_area(bb);
+            int best = 0;
+            float mc = FLT_MAX;
+            Box b0 = box_empty();
+            uint32_t c0 = 0;
+            for (int i = 0; i < 11; ++i) {
+                c0 += cnt[i];
+                b0 = box_union(b0, bbk[i]);  // fold over buckets 0..i in order
+                Box b1 = box_empty();
+                uint32_t c1 = 0;
+                for (int k = i + 1; k < 12; ++k) { c1 += cnt[k]; b1 = box_union(b1, bbk[k]); }
+                float b0sa = (float)c0 * surface_area(b0);
+                float b1sa = (float)c1 * surface_area(b1);
+                float cost = 0.125f * (b0sa + b1sa) / tsa;
+                if (cost < mc) { best = i; mc = cost; }
+            }
+            size_t mcu = (mc != mc || mc <= 0.0f) ? 0 : (size_t)mc;
+            if (max_prims < n || mcu < n) {
+                // stable partition on the precomputed buckets
+                uint32_t* a = ord.data() + b;
+                size_t l = 0, r = 0;
+                for (uint32_t i = 0; i < n; ++i) {
+                    uint32_t x = a[i];
+                    if ((int)bk[i] <= best) a[l++] = x;
+                    else tmp[r++] = x;
+                }
+                if (r) memcpy(a + l, tmp.data(), r * sizeof(uint32_t));
+                mid = b + (uint32_t)l;
+            } else {
+                sah_leaf = true;
+                mid = b;
+            }
+        }
+        if (sah_leaf) { make_leaf(); continue; }
+        if (mid == b || mid == e) {
+            *err = "BVH split produced an empty side";  // bvh.rs:237-238 asserts
+            return false;
+        }
+        nodes[id].meta = (uint32_t)dim;
+        stack.push_back(Task{mid, e, (int64_t)id, T.depth + 1});
+        stack.push_back(Task{b, mid, -1, T.depth + 1});
+    }
+    // interior bounds = left ∪ right (bvh.rs:247), children follow parents
+    for (size_t i = nodes.size(); i-- > 0;) {
+        if ((nodes[i].meta & 3u) != 3u) nb[i] = box_union(nb[i + 1], nb[nodes[i].off]);
+    }
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        nodes[i].lo[0] = nb[i].lo.x; nodes[i].lo[1] = nb[i].lo.y; nodes[i].lo[2] = nb[i].lo.z;
+        nodes[i].hi[0] = nb[i].hi.x; nodes[i].hi[1] = nb[i].hi.y; nodes[i].hi[2] = nb[i].hi.z;
+    }
+    out->depth = maxdepth;
+    out->order = ord;
+    out->nfaces = nfaces;
+    out->nspheres = nsph;
+    out->root_lo[0] = nb[0].lo.x; out->root_lo[1] = nb[0].lo.y; out->root_lo[2] = nb[0].lo.z;
+    out->root_hi[0] = nb[0].hi.x; out->root_hi[1] = nb[0].hi.y; out->root_hi[2] = nb[0].hi.z;
+
+    // primitive records in leaf order
+    out->prims.resize(N);
+    for (uint32_t k = 0; k < N; ++k) {
+        uint32_t id = ord[k];
+        PrimRec& r = out->prims[k];
+        memset(&r, 0, sizeof(r));
+        r.w[10] = bits_f(id);
+        if (id < nfaces) {
+            f3 p1 = V(idx[3 * id + 2]), p2 = V(idx[3 * id + 1]), p3 = V(idx[3 * id + 0]);
+            f3 e1 = sub3(p2, p1), e2 = sub3(p3, p1);
+            r.w[0] = p1.x; r.w[1] = p1.y; r.w[2] = p1.z;
+            r.w[3] = e1.x; r.w[4] = e1.y; r.w[5] = e1.z;
+            r.w[6] = e2.x; r.w[7] = e2.y; r.w[8] = e2.z;
+            r.w[11] = bits_f(0u);
+        } else {
+            r.w[9] = bits_f(id - nfaces);
+            r.w[11] = bits_f(1u);
+        }
+    }
+
+    // the last record of every leaf carries bit 8 of the kind word
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if ((nodes[i].meta & 3u) != 3u) continue;
+        uint32_t cnt = nodes[i].meta >> 2;
+        PrimRec& r = out->prims[nodes[i].off + cnt - 1];
+        uint32_t kind;
+        memcpy(&kind, &r.w[11], 4);
+        r.w[11] = bits_f(kind | 0x100u);
+    }
+
+    // child-pair layout
+    std::vector<int32_t> widx(nodes.size(), -1);
+    int32_t nw = 0;
+    for (size_t i = 0; i < nodes.size(); ++i)
+        if ((nodes[i].meta & 3u) != 3u) widx[i] = nw++;
+    out->wnodes.assign((size_t)nw, WNode{});
+    auto child_ref = [&](size_t c, uint32_t* count) -> int32_t {
+        if ((nodes[c].meta & 3u) == 3u) {
+            *count = nodes[c].meta >> 2;
+            return ~(int32_t)nodes[c].off;
+        }
+        *count = 0;
+        return widx[c];
+    };
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if (widx[i] < 0) continue;
+        WNode& w = out->wnodes[(size_t)widx[i]];
+        size_t c0 = i + 1, c1 = nodes[i].off;
+        uint32_t n0, n1;
+        w.ref0 = child_ref(c0, &n0);
+        w.ref1 = child_ref(c1, &n1);
+        if (n0 > 0x7FFFu || n1 > 0x7FFFu) {
+            *err = "leaf with more than 32767 primitives";
+            return false;
+        }
+        w.meta = (nodes[i].meta & 3u) | (n0 << 2) | (n1 << 17);
+        w.pad = 0;
+        w.c0lx = nodes[c0].lo[0]; w.c0hx = nodes[c0].hi[0];
+        w.c0ly = nodes[c0].lo[1]; w.c0hy = nodes[c0].hi[1];
+        w.c0lz = nodes[c0].lo[2]; w.c0hz = nodes[c0].hi[2];
+        w.c1lx = nodes[c1].lo[0]; w.c1hx = nodes[c1].hi[0];
+        w.c1ly = nodes[c1].lo[1]; w.c1hy = nodes[c1].hi[1];
+        w.c1lz = nodes[c1].lo[2]; w.c1hz = nodes[c1].hi[2];
+    }
+    if ((nodes[0].meta & 3u) == 3u) {
+        out->wroot = ~(int32_t)nodes[0].off;
+        out->wroot_count = nodes[0].meta >> 2;
+    } else {
+        out->wroot = 0;
+        out->wroot_count = 0;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    out->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return true;
+}
+
+}  // namespace px

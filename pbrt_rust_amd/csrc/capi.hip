@@ -1,0 +1,315 @@
+// capi.hip -- extern "C" entry points of libpbrtx_gpu (include/pbrtx_gpu.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/pbrtx_gpu.h"
+#include "bvh_build.hpp"
+#include "capi_internal.hpp"
+#include "trace.hpp"
+
+using namespace px;
+
+struct pbrtx_bvh {
+    HostBvh h;
+};
+
+namespace px {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? PBRTX_ERR_OOM : PBRTX_ERR_HIP;
+}
+
+TraceTuning g_tuning[2] = {TraceTuning{256, 0, 50}, TraceTuning{256, 0, 50}};
+
+namespace {
+std::mutex g_scr_mu;
+std::map<hipStream_t, TraceScratch> g_scratch;
+}  // namespace
+
+// Scratch of a stream, grown to at least `spill` bytes.  Calls on one stream
+// are ordered, so one scratch per stream is race free.
+int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out) {
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    TraceScratch& s = g_scratch[st];
+    if (!s.counter) {
+        hipError_t e = hipMalloc((void**)&s.counter, 64);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(counter)");
+        e = fill_u32(s.counter, 0u, 16, st);
+        if (e != hipSuccess) return hip_fail(e, "fill");
+    }
+    if (spill > s.spill_bytes) {
+        if (s.spill) {
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            (void)hipFree(s.spill);
+            s.spill = nullptr;
+            s.spill_bytes = 0;
+        }
+        hipError_t e = hipMalloc(&s.spill, spill);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(spill)");
+        s.spill_bytes = spill;
+    }
+    *out = &s;
+    return PBRTX_OK;
+}
+
+int upload(void** d, const void* h, size_t bytes) {
+    *d = nullptr;
+    if (bytes == 0) return PBRTX_OK;
+    hipError_t e = hipMalloc(d, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(scene)");
+    e = hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(scene)");
+    return PBRTX_OK;
+}
+
+int scene_from_host(const HostBvh& h, DeviceScene* s) {
+    memset(s, 0, sizeof(*s));
+    int rc;
+    if ((rc = upload(&s->d_rnodes, h.rnodes.data(), h.rnodes.size() * sizeof(RNode)))) return rc;
+    if ((rc = upload(&s->d_wnodes, h.wnodes.data(), h.wnodes.size() * sizeof(WNode)))) return rc;
+    if ((rc = upload(&s->d_prims, h.prims.data(), h.prims.size() * sizeof(PrimRec)))) return rc;
+    if ((rc = upload(&s->d_spheres, h.spheres.data(), h.spheres.size() * sizeof(SphereRec)))) return rc;
+    s->bytes = h.rnodes.size() * sizeof(RNode) + h.wnodes.size() * sizeof(WNode) +
+               h.prims.size() * sizeof(PrimRec) + h.spheres.size() * sizeof(SphereRec);
+    SceneView& v = s->view;
+    v.rnodes = (const RNode*)s->d_rnodes;
+    v.wnodes = (const WNode*)s->d_wnodes;
+    v.prims = (const PrimRec*)s->d_prims;
+    v.spheres = (const SphereRec*)s->d_spheres;
+    v.n_rnodes = (uint32_t)h.rnodes.size();
+    v.n_wnodes = (uint32_t)h.wnodes.size();
+    v.n_prims = (uint32_t)h.prims.size();
+    v.n_spheres = (uint32_t)h.spheres.size();
+    v.wroot = h.wroot;
+    v.depth = h.depth;
+    for (int k = 0; k < 3; ++k) { v.root_lo[k] = h.root_lo[k]; v.root_hi[k] = h.root_hi[k]; }
+    return PBRTX_OK;
+}
+
+void scene_free(DeviceScene* s) {
+    if (s->d_rnodes) (void)hipFree(s->d_rnodes);
+    if (s->d_wnodes) (void)hipFree(s->d_wnodes);
+    if (s->d_prims) (void)hipFree(s->d_prims);
+    if (s->d_spheres) (void)hipFree(s->d_spheres);
+    memset(s, 0, sizeof(*s));
+}
+
+int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, hipStream_t st) {
+    if (path != PATH_PLAIN && path != PATH_BATCHED) return fail(PBRTX_ERR_ARG, "unknown path");
+    const TraceTuning& tun = g_tuning[path];
+    TraceScratch* scr;
+    int rc = stream_scratch(st, trace_spill_bytes(v, path, tun), &scr);
+    if (rc) return rc;
+    hipError_t e = trace_closest(v, rays, n, hits, path, tun, scr, st);
+    if (e != hipSuccess) return hip_fail(e, "trace_closest");
+    return PBRTX_OK;
+}
+
+int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int path, hipStream_t st) {
+    if (path != PATH_PLAIN && path != PATH_BATCHED) return fail(PBRTX_ERR_ARG, "unknown path");
+    const TraceTuning& tun = g_tuning[path];
+    TraceScratch* scr;
+    int rc = stream_scratch(st, trace_spill_bytes(v, path, tun), &scr);
+    if (rc) return rc;
+    hipError_t e = trace_any(v, rays, n, occ, path, tun, scr, st);
+    if (e != hipSuccess) return hip_fail(e, "trace_any");
+    return PBRTX_OK;
+}
+
+}  // namespace px
+
+struct pbrtx_scene {
+    DeviceScene s;
+};
+
+namespace {
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+std::mutex g_ev_mu;
+}  // namespace
+
+extern "C" {
+
+const char* pbrtx_last_error(void) { return g_err.c_str(); }
+
+int pbrtx_bvh_build(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
+                    const pbrtx_sphere* spheres, uint32_t nspheres, int32_t split, uint32_t max_prims,
+                    pbrtx_bvh** out) {
+    if (!out) return fail(PBRTX_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (nspheres && !spheres) return fail(PBRTX_ERR_ARG, "spheres is null");
+    static_assert(sizeof(pbrtx_sphere) == sizeof(SphereIn), "sphere layout");
+    pbrtx_bvh* b = new (std::nothrow) pbrtx_bvh();
+    if (!b) return fail(PBRTX_ERR_OOM, "host allocation");
+    std::string err;
+    if (!build_bvh(verts, nverts, idx, nfaces, reinterpret_cast<const SphereIn*>(spheres), nspheres,
+                   split, max_prims, &b->h, &err)) {
+        delete b;
+        return fail(PBRTX_ERR_BUILD, err);
+    }
+    *out = b;
+    return PBRTX_OK;
+}
+
+int pbrtx_bvh_info(const pbrtx_bvh* b, uint32_t* n_nodes, uint32_t* depth, uint32_t* n_prims,
+                   double* build_ms) {
+    if (!b) return fail(PBRTX_ERR_ARG, "bvh is null");
+    if (n_nodes) *n_nodes = (uint32_t)b->h.rnodes.size();
+    if (depth) *depth = b->h.depth;
+    if (n_prims) *n_prims = (uint32_t)b->h.prims.size();
+    if (build_ms) *build_ms = b->h.build_ms;
+    return PBRTX_OK;
+}
+
+int pbrtx_bvh_export(const pbrtx_bvh* b, float* bounds, uint32_t* off, uint32_t* meta, uint8_t* leaf,
+                     uint32_t* order) {
+    if (!b) return fail(PBRTX_ERR_ARG, "bvh is null");
+    const auto& n = b->h.rnodes;
+    for (size_t i = 0; i < n.size(); ++i) {
+        bool lf = (n[i].meta & 3u) == 3u;
+        if (bounds)
+            for (int k = 0; k < 3; ++k) { bounds[6 * i + k] = n[i].lo[k]; bounds[6 * i + 3 + k] = n[i].hi[k]; }
+        if (off) off[i] = n[i].off;
+        if (meta) meta[i] = lf ? (n[i].meta >> 2) : (n[i].meta & 3u);
+        if (leaf) leaf[i] = lf ? 1 : 0;
+    }
+    if (order) memcpy(order, b->h.order.data(), b->h.order.size() * sizeof(uint32_t));
+    return PBRTX_OK;
+}
+
+int pbrtx_bvh_free(pbrtx_bvh* b) {
+    delete b;
+    return PBRTX_OK;
+}
+
+int pbrtx_scene_create(const pbrtx_bvh* b, pbrtx_scene** out) {
+    if (!b || !out) return fail(PBRTX_ERR_ARG, "null argument");
+    *out = nullptr;
+    pbrtx_scene* s = new (std::nothrow) pbrtx_scene();
+    if (!s) return fail(PBRTX_ERR_OOM, "host allocation");
+    int rc = scene_from_host(b->h, &s->s);
+    if (rc) {
+        scene_free(&s->s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PBRTX_OK;
+}
+
+int pbrtx_scene_destroy(pbrtx_scene* s) {
+    if (!s) return PBRTX_OK;
+    scene_free(&s->s);
+    delete s;
+    return PBRTX_OK;
+}
+
+int pbrtx_scene_device_bytes(const pbrtx_scene* s, uint64_t* bytes) {
+    if (!s || !bytes) return fail(PBRTX_ERR_ARG, "null argument");
+    *bytes = s->s.bytes;
+    return PBRTX_OK;
+}
+
+int pbrtx_intersect(pbrtx_scene* s, pbrtx_ray* d_rays, uint32_t n, pbrtx_hit* d_hits, int32_t path,
+                    void* stream) {
+    if (!s) return fail(PBRTX_ERR_ARG, "scene is null");
+    if (n == 0) return PBRTX_OK;
+    if (!d_rays || !d_hits) return fail(PBRTX_ERR_ARG, "null ray or hit buffer");
+    return run_closest(s->s.view, reinterpret_cast<Ray*>(d_rays), n, reinterpret_cast<Hit*>(d_hits), path,
+                       (hipStream_t)stream);
+}
+
+int pbrtx_intersect_p(pbrtx_scene* s, const pbrtx_ray* d_rays, uint32_t n, uint32_t* d_occ, int32_t path,
+                      void* stream) {
+    if (!s) return fail(PBRTX_ERR_ARG, "scene is null");
+    if (n == 0) return PBRTX_OK;
+    if (!d_rays || !d_occ) return fail(PBRTX_ERR_ARG, "null ray or bit buffer");
+    return run_any(s->s.view, reinterpret_cast<const Ray*>(d_rays), n, d_occ, path, (hipStream_t)stream);
+}
+
+int pbrtx_set_tuning(int32_t path, int32_t block, int32_t blocks_per_cu, int32_t refill_pct) {
+    if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");
+    if (block != 64 && block != 128 && block != 256) return fail(PBRTX_ERR_ARG, "block must be 64, 128 or 256");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
+    if (refill_pct < 0 || refill_pct > 100) return fail(PBRTX_ERR_ARG, "refill_pct out of range");
+    g_tuning[path] = TraceTuning{block, blocks_per_cu, refill_pct};
+    return PBRTX_OK;
+}
+
+int pbrtx_device_malloc(void** p, uint64_t bytes) {
+    if (!p) return fail(PBRTX_ERR_ARG, "null pointer");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return PBRTX_OK;
+}
+int pbrtx_device_free(void* p) {
+    if (!p) return PBRTX_OK;
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return hip_fail(e, "hipFree");
+    return PBRTX_OK;
+}
+int pbrtx_memcpy_htod(void* d, const void* h, uint64_t bytes) {
+    hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
+    return PBRTX_OK;
+}
+int pbrtx_memcpy_dtoh(void* h, const void* d, uint64_t bytes) {
+    hipError_t e = hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy D2H");
+    return PBRTX_OK;
+}
+int pbrtx_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    // report a kernel-side stack overflow flag, if any
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    for (auto& kv : g_scratch) {
+        if (!kv.second.counter) continue;
+        uint32_t flag = 0;
+        e = hipMemcpy(&flag, kv.second.counter + 1, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy flag");
+        if (flag) return fail(PBRTX_ERR_STATE, "traversal stack deeper than the scene's recorded depth");
+    }
+    return PBRTX_OK;
+}
+
+int pbrtx_timer_start(void* stream) {
+    std::lock_guard<std::mutex> lk(g_ev_mu);
+    hipError_t e;
+    if (!g_ev0 && (e = hipEventCreate(&g_ev0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if (!g_ev1 && (e = hipEventCreate(&g_ev1)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventRecord(g_ev0, (hipStream_t)stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    return PBRTX_OK;
+}
+int pbrtx_timer_stop(void* stream, float* ms) {
+    std::lock_guard<std::mutex> lk(g_ev_mu);
+    if (!g_ev0 || !g_ev1 || !ms) return fail(PBRTX_ERR_STATE, "timer not started");
+    hipError_t e;
+    if ((e = hipEventRecord(g_ev1, (hipStream_t)stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(g_ev1)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    if ((e = hipEventElapsedTime(ms, g_ev0, g_ev1)) != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+    return PBRTX_OK;
+}
+int pbrtx_device_name(char* buf, uint32_t len) {
+    if (!buf || len == 0) return fail(PBRTX_ERR_ARG, "null buffer");
+    int dev = 0;
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return PBRTX_OK;
+}
+
+}  // extern "C"

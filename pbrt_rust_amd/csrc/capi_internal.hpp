@@ -1,0 +1,27 @@
+// capi_internal.hpp -- state shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "bvh_build.hpp"
+#include "trace.hpp"
+
+namespace px {
+
+struct DeviceScene {
+    SceneView view;
+    void *d_rnodes, *d_wnodes, *d_prims, *d_spheres;
+    uint64_t bytes;
+};
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out);
+int scene_from_host(const HostBvh& h, DeviceScene* s);
+void scene_free(DeviceScene* s);
+int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, hipStream_t st);
+int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int path, hipStream_t st);
+extern TraceTuning g_tuning[2];
+
+}  // namespace px

@@ -1,0 +1,488 @@
+// trace.hip -- BVH traversal kernels for gfx950 (closest hit and occlusion).
+//
+// Both paths reproduce BVHAccelerator::intersect (bvh.rs:375-422) exactly:
+// near-first order chosen by the sign of 1/d on the node's split axis, boxes
+// culled against the *current* maxt, leaf prims tested in order, a later
+// equal-t hit replacing an earlier one (mesh.rs:71).  Occlusion is the
+// reference's intersect(r).is_some() (intersection.rs:63-65) with an early
+// exit, which yields the same bit.
+//
+// PATH_PLAIN   one lane per ray, static schedule, the 32-byte reference node
+//              record, box test at pop time (exactly the reference loop), a
+//              per-lane LDS stack of node indices with a global spill.
+// PATH_BATCHED persistent waves pulling rays from a global counter and
+//              refilling idle lanes (wave ballot + prefix rank), 64-byte
+//              child-pair nodes whose two boxes are tested together; a child
+//              is pushed with its entry distance and re-checked against the
+//              current tmax when popped, which is the reference's pop-time test.
+#include "trace.hpp"
+
+namespace px {
+namespace {
+
+constexpr int STK_PLAIN = 24;   // LDS entries per lane (u32 node index)
+constexpr int STK_WIDE = 16;    // LDS entries per lane (ref + entry distance)
+constexpr int MAX_BLOCK = 256;
+
+__device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
+
+// Slab interval of one box (bbox.rs:185-209): returns entry tn and exit tf
+// over the three axes and [tmin, tmax].  The reference rejects as soon as
+// t0 > t1 after any axis; t0 only grows and t1 only shrinks, so testing once
+// after the three axes gives the same answer.  The swap keeps the
+// reference's NaN behaviour (a NaN slab bound is ignored by max / min).
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz,
+                                     float ox, float oy, float oz, float ix, float iy, float iz,
+                                     float tmin, float tmax, float* tn_out) {
+    float ax = (lx - ox) * ix, bx = (hx - ox) * ix;
+    float ay = (ly - oy) * iy, by = (hy - oy) * iy;
+    float az = (lz - oz) * iz, bz = (hz - oz) * iz;
+    float nx = ax > bx ? bx : ax, fx = ax > bx ? ax : bx;
+    float ny = ay > by ? by : ay, fy = ay > by ? ay : by;
+    float nz = az > bz ? bz : az, fz = az > bz ? az : bz;
+    float t0 = fmaxf(fmaxf(fmaxf(tmin, nx), ny), nz);
+    float t1 = fminf(fminf(fminf(tmax, fx), fy), fz);
+    *tn_out = t0;
+    return !(t0 > t1);
+}
+
+struct RayState {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax;
+};
+
+// Triangle::get_intersection_point (mesh.rs:41-72) with e1, e2 precomputed.
+__device__ __forceinline__ bool tri_test(const float4 a, const float4 b, const float4 c,
+                                         const RayState& r, float* t, float* b1o, float* b2o) {
+    const f3 p1 = mk3(a.x, a.y, a.z);
+    const f3 e1 = mk3(a.w, b.x, b.y);
+    const f3 e2 = mk3(b.z, b.w, c.x);
+    const f3 d = mk3(r.dx, r.dy, r.dz);
+    const f3 s1 = cross3(d, e2);
+    const float div = dot3(s1, e1);
+    if (div == 0.0f) return false;
+    const float inv = 1.0f / div;
+    const f3 s = sub3(mk3(r.ox, r.oy, r.oz), p1);
+    const float b1 = dot3(s1, s) * inv;
+    if (b1 < 0.0f || b1 > 1.0f) return false;
+    const f3 s2 = cross3(s, e1);
+    const float b2 = dot3(d, s2) * inv;
+    if (b2 < 0.0f || (b1 + b2) > 1.0f) return false;
+    const float tt = dot3(e2, s2) * inv;
+    if (tt < r.tmin || tt > r.tmax) return false;
+    *t = tt; *b1o = b1; *b2o = b2;
+    return true;
+}
+
+// Sphere::get_intersection_point (sphere.rs:46-107) on the object-space ray
+// (world2object.t(r), sphere.rs:131,137); utils::quadratic (utils/mod.rs:56-92).
+__device__ bool sphere_test(const SphereRec& s, const RayState& r, float* t_out) {
+    const float* m = s.w2o;
+    float x = r.ox, y = r.oy, z = r.oz;
+    float xt = m[0] * x + m[1] * y + m[2] * z + m[3];
+    float yt = m[4] * x + m[5] * y + m[6] * z + m[7];
+    float zt = m[8] * x + m[9] * y + m[10] * z + m[11];
+    float w = m[12] * x + m[13] * y + m[14] * z + m[15];
+    f3 o = (w != 1.0f) ? mk3(xt / w, yt / w, zt / w) : mk3(xt, yt, zt);
+    f3 d = mk3(m[0] * r.dx + m[1] * r.dy + m[2] * r.dz, m[4] * r.dx + m[5] * r.dy + m[6] * r.dz,
+               m[8] * r.dx + m[9] * r.dy + m[10] * r.dz);
+    float a = lensq3(d);
+    float b = 2.0f * dot3(d, o);
+    float c = lensq3(o) - s.radius * s.radius;
+    float disc = b * b - 4.0f * a * c;
+    float t0, t1;
+    if (disc < 0.0f) return false;
+    if (fabsf(disc) < 1e-6f) {
+        if (a == 0.0f) return false;
+        t0 = t1 = -b / (2.0f * a);
+    } else {
+        float root = sqrtf(disc);
+        float q = (b < 0.0f) ? -0.5f * (b - root) : -0.5f * (b + root);
+        if (a == 0.0f || q == 0.0f) return false;
+        float r0 = q / a, r1 = c / q;
+        t0 = r0 < r1 ? r0 : r1;
+        t1 = r0 < r1 ? r1 : r0;
+    }
+    if (t0 > r.tmax || t1 < r.tmin) return false;
+    float th = t0;
+    if (t0 < r.tmin) {
+        th = t1;
+        if (th > r.tmax) return false;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        f3 h = add3(o, mul3(d, th));
+        if (h.x == 0.0f && h.y == 0.0f) h.x = 1e-5f * s.radius;
+        float ang = atan2f(h.y, h.x);
+        if (ang < 0.0f) ang += 2.0f * PI_F;
+        bool bad = (h.z > -s.radius && h.z < s.zmin) || (h.z < s.radius && h.z > s.zmax) ||
+                   (ang > s.phimax);
+        if (!bad) { *t_out = th; return true; }
+        if (pass == 1 || th == t1 || t1 > r.tmax) return false;
+        th = t1;
+    }
+    return false;
+}
+
+// Test one primitive record; on a hit tmax becomes t (geometric.rs:64).
+__device__ __forceinline__ bool prim_test(const SceneView& sv, uint32_t pos, RayState& r,
+                                          uint32_t* prim, float* b1, float* b2, bool* last) {
+    const float4* p = reinterpret_cast<const float4*>(sv.prims + pos);
+    const float4 a = p[0], b = p[1], c = p[2];
+    const uint32_t kind = as_u(c.w);
+    *last = (kind & 0x100u) != 0u;
+    float t, u, v;
+    bool hit;
+    if ((kind & 0xFFu) == 0u) {
+        hit = tri_test(a, b, c, r, &t, &u, &v);
+    } else {
+        hit = sphere_test(sv.spheres[as_u(c.y)], r, &t);
+        u = 0.0f; v = 0.0f;
+    }
+    if (hit) {
+        r.tmax = t;
+        *prim = as_u(c.z);
+        *b1 = u; *b2 = v;
+    }
+    return hit;
+}
+
+__device__ __forceinline__ void load_ray(const Ray* rays, uint32_t i, RayState& r) {
+    const float4* p = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = p[0], b = p[1];
+    r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
+    r.dx = b.x; r.dy = b.y; r.dz = b.z; r.tmax = b.w;
+    // bvh.rs:382-383 (and the per-axis 1/d of bbox.rs:191: the same value)
+    r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
+}
+
+// ---------------------------------------------------------------------------
+// PATH_PLAIN
+// ---------------------------------------------------------------------------
+template <bool ANY>
+__global__ __launch_bounds__(MAX_BLOCK) void k_trace_plain(SceneView sv, Ray* rays, uint32_t n,
+                                                           Hit* hits, uint32_t* occ,
+                                                           uint32_t* spill, uint32_t spill_levels,
+                                                           uint32_t* err) {
+    __shared__ uint32_t stk[STK_PLAIN][MAX_BLOCK];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + tid;
+    const uint32_t total = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += total) {
+        const uint32_t i = base + tid;
+        bool hit = false;
+        if (i < n) {
+            RayState r;
+            load_ray(rays, i, r);
+            const bool neg0 = r.ix < 0.0f, neg1 = r.iy < 0.0f, neg2 = r.iz < 0.0f;
+            uint32_t hprim = NO_HIT;
+            float hb1 = 0.0f, hb2 = 0.0f;
+            int sp = 0;
+            auto push = [&](uint32_t v) {
+                if (sp < STK_PLAIN) stk[sp][tid] = v;
+                else if ((uint32_t)(sp - STK_PLAIN) < spill_levels)
+                    spill[(size_t)(sp - STK_PLAIN) * total + gtid] = v;
+                else { atomicOr(err, 1u); return; }
+                ++sp;
+            };
+            auto pop = [&]() -> uint32_t {
+                --sp;
+                if (sp < STK_PLAIN) return stk[sp][tid];
+                return spill[(size_t)(sp - STK_PLAIN) * total + gtid];
+            };
+            push(0u);
+            while (sp > 0) {
+                const uint32_t ni = pop();
+                const float4* q = reinterpret_cast<const float4*>(sv.rnodes + ni);
+                const float4 qa = q[0], qb = q[1];
+                float tn;
+                if (!slab(qa.x, qa.w, qa.y, qb.x, qa.z, qb.y, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz,
+                          r.tmin, r.tmax, &tn))
+                    continue;
+                const uint32_t off = as_u(qb.z), meta = as_u(qb.w);
+                if ((meta & 3u) == 3u) {
+                    const uint32_t cnt = meta >> 2;
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        bool last;
+                        if (prim_test(sv, off + k, r, &hprim, &hb1, &hb2, &last)) {
+                            hit = true;
+                            if (ANY) break;
+                        }
+                    }
+                    if (ANY && hit) break;
+                } else if (meta == 0 ? neg0 : (meta == 1 ? neg1 : neg2)) {
+                    push(ni + 1);
+                    push(off);
+                } else {
+                    push(off);
+                    push(ni + 1);
+                }
+            }
+            if (!ANY) {
+                hits[i] = Hit{hit ? hprim : NO_HIT, hit ? r.tmax : 0.0f, hb1, hb2};
+                if (hit) rays[i].tmax = r.tmax;
+            }
+        }
+        if (ANY) {
+            // 64 consecutive rays per wave: one ballot, two words
+            const unsigned long long m = __ballot(hit);
+            const uint32_t lane = __lane_id();
+            const uint32_t w0 = (base + (tid & ~63u)) >> 5;
+            if (lane == 0 && (size_t)w0 * 32 < n) occ[w0] = (uint32_t)(m & 0xFFFFFFFFull);
+            if (lane == 32 && (size_t)(w0 + 1) * 32 < n) occ[w0 + 1] = (uint32_t)(m >> 32);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PATH_BATCHED
+// ---------------------------------------------------------------------------
+template <bool ANY>
+__global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n,
+                                                          Hit* hits, uint32_t* occ,
+                                                          uint32_t* counter, uint2* spill,
+                                                          uint32_t spill_levels, uint32_t refill,
+                                                          uint32_t* err) {
+    __shared__ int32_t s_ref[STK_WIDE][MAX_BLOCK];
+    __shared__ float s_tn[STK_WIDE][MAX_BLOCK];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + tid;
+    const uint32_t total = gridDim.x * blockDim.x;
+    const uint32_t lane = __lane_id();
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int32_t POP = (int32_t)0x80000000;  // "take the next stack entry"
+
+    RayState r{};
+    bool neg0 = false, neg1 = false, neg2 = false;
+    uint32_t rid = 0, hprim = NO_HIT;
+    float hb1 = 0.0f, hb2 = 0.0f;
+    int32_t cur = POP;
+    int sp = 0;
+    bool has = false, drained = false;
+
+    while (true) {
+        // ---- refill idle lanes with the next rays (one atomic per wave)
+        const unsigned long long need = __ballot(!has && !drained);
+        if (need) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(counter, (uint32_t)__popcll(need));
+            b = __shfl(b, 0);
+            if (!has && !drained) {
+                const uint32_t id = b + (uint32_t)__popcll(need & lt_mask);
+                if (id < n) {
+                    rid = id;
+                    load_ray(rays, id, r);
+                    neg0 = r.ix < 0.0f; neg1 = r.iy < 0.0f; neg2 = r.iz < 0.0f;
+                    hprim = NO_HIT; hb1 = 0.0f; hb2 = 0.0f;
+                    sp = 0;
+                    float tn;
+                    // the root's own box (popped first by the reference)
+                    if (slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2],
+                             sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &tn))
+                        cur = sv.wroot;
+                    else
+                        cur = POP;
+                    has = true;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(has)) break;
+        // ---- traverse until too few lanes are busy
+        while (has) {
+            if (cur == POP) {
+                bool found = false;
+                while (sp > 0) {
+                    --sp;
+                    int32_t ref;
+                    float tn;
+                    if (sp < STK_WIDE) { ref = s_ref[sp][tid]; tn = s_tn[sp][tid]; }
+                    else {
+                        const uint2 e = spill[(size_t)(sp - STK_WIDE) * total + gtid];
+                        ref = (int32_t)e.x; tn = as_f(e.y);
+                    }
+                    if (tn <= r.tmax) { cur = ref; found = true; break; }
+                }
+                if (!found) {
+                    // ray finished
+                    if (ANY) {
+                        // no hit: bit stays 0
+                    } else {
+                        hits[rid] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
+                        if (hprim != NO_HIT) rays[rid].tmax = r.tmax;
+                    }
+                    has = false;
+                }
+            } else if (cur >= 0) {
+                const float4* q = reinterpret_cast<const float4*>(sv.wnodes + cur);
+                const float4 qa = q[0], qb = q[1], qc = q[2];
+                const int4 qd = reinterpret_cast<const int4*>(sv.wnodes + cur)[3];
+                float tn0, tn1;
+                const bool p0 = slab(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, r.ox, r.oy, r.oz, r.ix, r.iy,
+                                     r.iz, r.tmin, r.tmax, &tn0);
+                const bool p1 = slab(qb.z, qb.w, qc.x, qc.y, qc.z, qc.w, r.ox, r.oy, r.oz, r.ix, r.iy,
+                                     r.iz, r.tmin, r.tmax, &tn1);
+                const uint32_t axis = (uint32_t)qd.z & 3u;
+                const bool ng = axis == 0 ? neg0 : (axis == 1 ? neg1 : neg2);
+                // near child = second child when the direction is negative (bvh.rs:409-415)
+                const int32_t nref = ng ? qd.y : qd.x, fref = ng ? qd.x : qd.y;
+                const bool np = ng ? p1 : p0, fp = ng ? p0 : p1;
+                const float ftn = ng ? tn0 : tn1;
+                if (np && fp) {
+                    if (sp < STK_WIDE) {
+                        s_ref[sp][tid] = fref; s_tn[sp][tid] = ftn; ++sp;
+                    } else if ((uint32_t)(sp - STK_WIDE) < spill_levels) {
+                        spill[(size_t)(sp - STK_WIDE) * total + gtid] = make_uint2((uint32_t)fref, as_u(ftn));
+                        ++sp;
+                    } else {
+                        atomicOr(err, 1u);  // deeper than the scene's recorded depth: never expected
+                    }
+                    cur = nref;
+                } else if (np) {
+                    cur = nref;
+                } else if (fp) {
+                    cur = fref;
+                } else {
+                    cur = POP;
+                }
+            } else {
+                // leaf: prims in order until the record flagged last
+                uint32_t pos = (uint32_t)(~cur);
+                bool hitnow = false;
+                while (true) {
+                    bool last;
+                    if (prim_test(sv, pos, r, &hprim, &hb1, &hb2, &last)) {
+                        hitnow = true;
+                        if (ANY) break;
+                    }
+                    if (last) break;
+                    ++pos;
+                }
+                cur = POP;
+                if (ANY && hitnow) {
+                    atomicOr(occ + (rid >> 5), 1u << (rid & 31u));
+                    has = false;
+                }
+            }
+            // leave to refill when the wave has gone sparse
+            const unsigned long long busy = __ballot(has);
+            if ((uint32_t)__popcll(busy) * 100u < refill * 64u) break;
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int g_cu_count = 0;
+
+constexpr int PLAIN_BLOCKS_PER_CU = 16;
+
+int plain_grid(uint32_t n, int block) {
+    const int cus = device_cu_count();
+    long long want = ((long long)n + block - 1) / block;
+    long long cap = (long long)cus * PLAIN_BLOCKS_PER_CU;  // static grid-stride schedule
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (int)want;
+}
+
+template <typename K>
+int wide_grid(K kernel, const TraceTuning& tun) {
+    int per_cu = tun.blocks_per_cu;
+    if (per_cu <= 0) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, tun.block, 0) != hipSuccess || occ < 1)
+            occ = 2;
+        per_cu = occ;
+    }
+    return device_cu_count() * per_cu;
+}
+
+}  // namespace
+
+int device_cu_count() {
+    if (g_cu_count == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        g_cu_count = cus;
+    }
+    return g_cu_count;
+}
+
+hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, p, value, n);
+    return hipGetLastError();
+}
+
+static uint32_t plain_levels(const SceneView& s) {
+    // the reference-order stack holds at most depth + 1 entries
+    uint32_t need = s.depth + 2;
+    return need > (uint32_t)STK_PLAIN ? need - (uint32_t)STK_PLAIN : 0u;
+}
+static uint32_t wide_levels(const SceneView& s) {
+    uint32_t need = s.depth + 1;
+    return need > (uint32_t)STK_WIDE ? need - (uint32_t)STK_WIDE : 0u;
+}
+
+size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
+    const int block = tun.block;
+    if (path == PATH_PLAIN) {
+        size_t threads = (size_t)device_cu_count() * PLAIN_BLOCKS_PER_CU * block;
+        return (size_t)plain_levels(s) * threads * sizeof(uint32_t);
+    }
+    TraceTuning t = tun;
+    size_t threads = (size_t)wide_grid(k_trace_wide<false>, t) * block;
+    size_t threads2 = (size_t)wide_grid(k_trace_wide<true>, t) * block;
+    if (threads2 > threads) threads = threads2;
+    return (size_t)wide_levels(s) * threads * sizeof(uint2);
+}
+
+template <bool ANY>
+static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, uint32_t* occ, int path,
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int block = tun.block;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    if (ANY) {
+        hipError_t e = fill_u32(occ, 0u, ((size_t)n + 31) / 32, st);
+        if (e != hipSuccess) return e;
+    }
+    uint32_t* err = scr->counter + 1;
+    if (path == PATH_PLAIN) {
+        const int grid = plain_grid(n, block);
+        const uint32_t lv = plain_levels(s);
+        if ((size_t)lv * grid * block * sizeof(uint32_t) > scr->spill_bytes) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_trace_plain<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, hits, occ,
+                           (uint32_t*)scr->spill, lv, err);
+        return hipGetLastError();
+    }
+    const int grid = wide_grid(k_trace_wide<ANY>, tun);
+    const uint32_t lv = wide_levels(s);
+    if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
+    hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
+    hipLaunchKernelGGL(k_trace_wide<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, hits, occ, scr->counter,
+                       (uint2*)scr->spill, lv, refill, err);
+    return hipGetLastError();
+}
+
+hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, int path,
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
+    return launch<false>(s, rays, n, hits, nullptr, path, tun, scr, st);
+}
+hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
+                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
+    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st);
+}
+
+}  // namespace px

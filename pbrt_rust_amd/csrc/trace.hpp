@@ -1,0 +1,48 @@
+// trace.hpp -- device scene view and traversal launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pbrtx_common.hpp"
+
+namespace px {
+
+// Device-resident scene (all pointers in HBM).
+struct SceneView {
+    const RNode* rnodes;
+    const WNode* wnodes;
+    const PrimRec* prims;
+    const SphereRec* spheres;
+    uint32_t n_rnodes, n_wnodes, n_prims, n_spheres;
+    int32_t wroot;
+    uint32_t depth;
+    float root_lo[3], root_hi[3];
+};
+
+enum TracePath : int { PATH_PLAIN = 0, PATH_BATCHED = 1 };
+
+struct TraceTuning {
+    int block = 256;          // threads per block
+    int blocks_per_cu = 0;    // persistent grid: blocks per CU (0 = occupancy maximum)
+    int refill_pct = 50;      // refill when fewer than this % of a wave's lanes trace
+};
+
+// Scratch owned by the caller (per stream): spill stacks and a work counter.
+struct TraceScratch {
+    void* spill = nullptr;       // spill stack entries
+    size_t spill_bytes = 0;
+    uint32_t* counter = nullptr; // one u32, device
+};
+
+// Launch helpers; return hipSuccess or the launch error.
+// closest: hits[i] for each ray, and rays[i].tmax = t on a hit.
+// any: occluded bit i of occ (words zeroed by the launcher).
+hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, int path,
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st);
+hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
+                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st);
+// Number of spill bytes a launch needs for this scene and tuning.
+size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
+hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
+int device_cu_count();
+
+}  // namespace px

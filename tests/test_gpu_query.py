@@ -1,0 +1,178 @@
+"""GPU parity of Intersectable::intersect / intersect_p through the C ABI:
+both traversal paths against the CPU oracle, bit for bit (prim id, t, b1, b2,
+the maxt write-back, occlusion bits), on the reference's own fixtures,
+random scenes, edge cases and a deep tree that uses the spill stack."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pbrt_rust_amd as px
+from pbrt_rust_amd import scenes
+import xf32
+from gpu_util import FMAX, F32, check_any, check_closest, mesh_pair, random_rays
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+PATHS = ["plain", "batched"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("method", ["sah", "middle", "equal"])
+def test_eight_spheres_prim_ids(path, method):
+    g = GOLD["spheres8"]
+    prims = []
+    for c in g["centers"]:
+        t = px.Transform.translate(c)
+        prims.append(px.Primitive.simple(px.Sphere(t, t.inverse(), False, 1.0, -1.0, 1.0, 360.0)))
+    bvh = px.Aggregate.bvh(prims, 1, method)
+    o = [r["o"] for r in g["rays"]]
+    d = [r["d"] for r in g["rays"]]
+    hits = bvh.intersect(px.RayBatch(o, d), path)
+    assert hits["prim"].tolist() == [r["prim"] for r in g["rays"]]
+    assert bvh.intersect_p(px.RayBatch(o, d), path).all()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_tetrahedron_maxt_side_effect(path):
+    g = GOLD["tetrahedron"]
+    for method, mp in g["bvh_ray"]["builds"]:
+        bvh = px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, g["indices"],
+                                                                     g["points"]))], mp, method)
+        rays = px.RayBatch([g["bvh_ray"]["o"]], [g["bvh_ray"]["d"]], 0.0)
+        assert bvh.intersect_p(rays, path)[0]
+        bvh.intersect(rays, path)
+        assert rays.maxt[0] == g["bvh_ray"]["maxt_after"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("split", ["sah", "middle", "equal"])
+def test_random_scene_parity(path, split):
+    v, idx = scenes.random_triangles(3000, 11)
+    bvh, orc = mesh_pair(v, idx, split)
+    o, d = random_rays(20000, 12)
+    hits, ref = check_closest(bvh, orc, o, d, path=path)
+    assert (hits["prim"] != px.NO_HIT).mean() > 0.2
+    check_any(bvh, orc, o, d, path=path)
+    # segments: bounded maxt as VisibilityTester::segment produces
+    check_any(bvh, orc, o, d, mint=1e-3, maxt=0.5, path=path)
+    check_closest(bvh, orc, o, d, mint=0.1, maxt=0.7, path=path)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_ray_query_camera_parity(path):
+    # the plain ray-query workload at 128x128: camera rays, closest hit, then
+    # one shadow segment per hit to the point light (2,2,-2), eps = t*5e-4
+    L = ol.lib()
+    s = scenes.ray_query_scene(16384)
+    bvh, orc = mesh_pair(s["verts"], s["idx"], "sah")
+    cam = scenes.pinhole(128, 128)
+    r2c = np.zeros(16, F32)
+    L.orc_camera_raster_to_camera(128, 128, ol.f32(cam["screen"]), cam["fov"], r2c)
+    xy = np.zeros((128 * 128, 2), F32)
+    for y in range(128):
+        for x in range(128):
+            p = np.zeros(2, F32)
+            L.orc_pixel_samples(0xC0, 128, x, y, 1, 1, 1, p)
+            xy[y * 128 + x] = p
+    o = np.zeros(3 * xy.shape[0], F32); d = np.zeros_like(o)
+    L.orc_camera_rays(r2c, ol.f32(cam["cam_to_world"]).reshape(16), xy.shape[0], xy.reshape(-1), o, d)
+    o = o.reshape(-1, 3); d = d.reshape(-1, 3)
+    hits, ref = check_closest(bvh, orc, o, d, path=path)
+    h = hits["prim"] != px.NO_HIT
+    assert h.mean() > 0.1
+    t = hits["t"][h]
+    p = (o[h] + d[h] * t[:, None]).astype(F32)
+    eps = (t * F32(5e-4)).astype(F32)
+    lp = np.array([2, 2, -2], F32)
+    dist = np.sqrt(np.sum((p - lp) ** 2, axis=1)).astype(F32)
+    dirn = ((lp - p) * (F32(1) / dist)[:, None]).astype(F32)
+    check_any(bvh, orc, p, dirn, eps, dist, path=path)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_edge_cases(path):
+    v, idx = scenes.random_triangles(500, 21)
+    bvh, orc = mesh_pair(v, idx)
+    for n in (1, 2, 31, 33, 63, 64, 65, 257):
+        o, d = random_rays(n, 100 + n)
+        check_closest(bvh, orc, o, d, path=path)
+        check_any(bvh, orc, o, d, path=path)
+    # axis-aligned and zero-component directions (1/d = inf, NaN slabs)
+    rng = np.random.default_rng(5)
+    o = rng.uniform(-1.2, 1.2, (3000, 3)).astype(F32)
+    d = np.zeros((3000, 3), F32)
+    ax = rng.integers(0, 3, 3000)
+    d[np.arange(3000), ax] = rng.choice([-1.0, 1.0, -0.0], 3000)
+    d[:1000, (ax[:1000] + 1) % 3] = -0.0
+    o[::7] = np.round(o[::7], 1)  # origins on slab planes
+    check_closest(bvh, orc, o, d, path=path)
+    check_any(bvh, orc, o, d, path=path)
+    # empty intervals and starts beyond every hit
+    o, d = random_rays(1000, 7)
+    check_closest(bvh, orc, o, d, mint=5.0, maxt=1.0, path=path)
+    check_any(bvh, orc, o, d, mint=1e6, maxt=FMAX, path=path)
+    # zero rays is a no-op
+    assert bvh.intersect(px.RayBatch(np.zeros((0, 3)), np.zeros((0, 3))), path).shape == (0,)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_deep_tree_uses_spill_stack(path):
+    # triangles at exponentially growing x: SAH peels one off per level
+    n = 60
+    xs = 1.35 ** np.arange(n)
+    base = np.array([[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0.05]], np.float64)
+    v = np.concatenate([base + [x, 0, 0] for x in xs]).astype(F32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    bvh, orc = mesh_pair(v, idx)
+    assert bvh.info()["depth"] > 30
+    rng = np.random.default_rng(3)
+    o = np.stack([rng.uniform(-1, xs[-1], 4000), rng.uniform(0, 0.1, 4000), np.full(4000, -1.0)], 1).astype(F32)
+    d = np.tile(np.array([[0, 0, 1]], F32), (4000, 1))
+    d[::2] = np.array([0.001, 0.0001, 1], F32)
+    check_closest(bvh, orc, o, d, path=path)
+    check_any(bvh, orc, o, d, path=path)
+    # rays along the chain visit every level
+    o2 = np.stack([np.full(500, -5.0), rng.uniform(0.0, 0.09, 500), rng.uniform(0.0, 0.04, 500)], 1).astype(F32)
+    d2 = np.tile(np.array([[1, 0, 0.0001]], F32), (500, 1))
+    check_closest(bvh, orc, o2, d2, path=path)
+    px.synchronize()  # raises if a stack overflowed
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_mixed_triangles_and_spheres(path):
+    v, idx = scenes.random_triangles(300, 31)
+    prims = [px.Primitive.simple(px.TriangleMesh(None, None, False, idx, v))]
+    orc = ol.OracleScene()
+    orc.add_mesh(v, idx)
+    rng = np.random.default_rng(4)
+    for k in range(20):
+        c = rng.uniform(-1, 1, 3)
+        t = px.Transform.translate(c) * px.Transform.scale(0.1, 0.1, 0.1)
+        prims.append(px.Primitive.simple(px.Sphere(t, t.inverse(), False, 1.0, -1.0, 1.0, 360.0)))
+        orc.add_sphere(t.m, t.m_inv, 1.0, -1.0, 1.0, 360.0)
+    bvh = px.BVHAccelerator(prims, 1, "sah")
+    orc.build("sah", 1)
+    o, d = random_rays(5000, 8)
+    hits, ref = check_closest(bvh, orc, o, d, path=path)
+    assert (hits["prim"] >= 300).sum() > 10   # some sphere hits
+    check_any(bvh, orc, o, d, path=path)
+
+
+def test_paths_agree_at_full_size():
+    # size-independent property at the ray-query configuration's full size:
+    # 16,384 triangles, 1M rays -- plain and batched give identical records,
+    # and the occlusion bit equals "closest hit exists"
+    s = scenes.ray_query_scene(16384)
+    bvh = px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, s["idx"], s["verts"]))], 1, "sah")
+    o, d = random_rays(1 << 20, 77)
+    ra, rb = px.RayBatch(o, d), px.RayBatch(o, d)
+    ha = bvh.intersect(ra, "plain")
+    hb = bvh.intersect(rb, "batched")
+    assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32))
+    assert np.array_equal(ra.maxt.view(np.uint32), rb.maxt.view(np.uint32))
+    occ = bvh.intersect_p(px.RayBatch(o, d), "batched")
+    assert np.array_equal(occ, ha["prim"] != px.NO_HIT)
+    assert np.array_equal(bvh.intersect_p(px.RayBatch(o, d), "plain"), occ)
