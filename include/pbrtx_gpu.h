@@ -99,6 +99,54 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
    maximum (batched path); refill_pct = idle-lane refill threshold. */
 int pbrtx_set_tuning(int32_t path, int32_t block, int32_t blocks_per_cu, int32_t refill_pct);
 
+/* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */
+/* Scene + camera + sampler + integrator description.  Triangles (world-space
+   vertices, index triples) with a material per face: mats holds 4 floats per
+   material (kind 0 = matte / Lambertian, 1 = mirror; rgb reflectance).
+   face_light: area-light id of an emitting face or -1; area_le: radiance of
+   each area light (3 floats); point_lights: 6 floats each (position,
+   intensity).  Camera: Camera::perspective (camera/mod.rs:105-135) with a
+   static cam_to_world; StratifiedSampler nx x ny samples per pixel with
+   jitter (stratified.rs:60-127) driven by a counter-based generator keyed by
+   (seed, pixel, sample, dimension).  integrator 0 = direct lighting
+   (light_samples per area light), 1 = path (max_depth, roulette once the
+   bounce index exceeds rr_start).  split / max_prims as pbrtx_bvh_build. */
+typedef struct pbrtx_render_desc {
+    const float* verts; uint32_t nverts;
+    const uint32_t* idx; uint32_t nfaces;
+    const uint32_t* face_mat; const float* mats; uint32_t nmats;
+    const int32_t* face_light; const float* area_le; uint32_t n_area;
+    const float* point_lights; uint32_t n_point;
+    float cam_to_world[16]; float fov_deg; float screen[4];
+    uint32_t xres, yres, nx, ny, seed;
+    int32_t integrator;
+    uint32_t light_samples;
+    uint32_t max_depth;
+    uint32_t rr_start;
+    int32_t split; uint32_t max_prims;
+} pbrtx_render_desc;
+
+typedef struct pbrtx_render_stats {
+    uint64_t camera_rays, shadow_rays, extension_rays; /* rays traced, by kind */
+    double ms_total;                 /* whole frame, HIP events on the render stream */
+    double ms_trace_closest, ms_trace_any, ms_other; /* per stage, summed over launches */
+    double build_ms;                 /* host BVH build at create time */
+    uint32_t launches_closest, launches_any;
+} pbrtx_render_stats;
+
+typedef struct pbrtx_renderer pbrtx_renderer;
+/* builds the BVH, uploads scene, lights and camera */
+int pbrtx_renderer_create(const pbrtx_render_desc* desc, pbrtx_renderer** out);
+/* samples per work chunk (whole pixels; bounds the per-chunk buffers) */
+int pbrtx_renderer_set_chunk(pbrtx_renderer* r, uint32_t samples_per_chunk);
+/* renders one frame with the given traversal path; rgb_out (host, may be
+   NULL) receives xres*yres*3 floats: the film's final rgb */
+int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);
+/* radiance of the first `count` samples of the last frame (3 floats each;
+   sample index = pixel * spp + sample, pixel = y * xres + x) */
+int pbrtx_renderer_sample_radiance(pbrtx_renderer* r, float* out, uint64_t count);
+int pbrtx_renderer_destroy(pbrtx_renderer* r);
+
 /* ---- device memory and timing plumbing ----------------------------------- */
 int pbrtx_device_malloc(void** d_ptr, uint64_t bytes);
 int pbrtx_device_free(void* d_ptr);

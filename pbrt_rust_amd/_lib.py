@@ -50,7 +50,8 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("camera_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("extension_rays", ctypes.c_uint64), ("ms_total", ctypes.c_double),
                 ("ms_trace_closest", ctypes.c_double), ("ms_trace_any", ctypes.c_double),
-                ("ms_other", ctypes.c_double), ("build_ms", ctypes.c_double)]
+                ("ms_other", ctypes.c_double), ("build_ms", ctypes.c_double),
+                ("launches_closest", ctypes.c_uint32), ("launches_any", ctypes.c_uint32)]
 
 
 _lib = None
@@ -90,6 +91,7 @@ def lib():
         L.pbrtx_renderer_render.argtypes = [vp, i32, vp, c.POINTER(RenderStats)]
         L.pbrtx_renderer_destroy.argtypes = [vp]
         L.pbrtx_renderer_sample_radiance.argtypes = [vp, vp, u64]
+        L.pbrtx_renderer_set_chunk.argtypes = [vp, u32]
     _lib = L
     return L
 

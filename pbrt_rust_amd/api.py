@@ -291,3 +291,110 @@ class Aggregate:
 
 def set_tuning(path, block=256, blocks_per_cu=0, refill_pct=50):
     check(lib().pbrtx_set_tuning(_path_code(path), block, blocks_per_cu, refill_pct))
+
+
+# ---------------------------------------------------------------------------
+# Renderer / integrators (src/renderer.rs:8-26, src/sampler_renderer.rs,
+# src/integrator/mod.rs:148-180)
+# ---------------------------------------------------------------------------
+class StratifiedSampler:
+    """StratifiedSampler::new(x0, x1, y0, y1, xs, ys, jitter, ...) over the
+    whole image (sampler/stratified.rs:22-37); jitter values come from the
+    counter-based generator keyed by (seed, pixel, sample, dimension)."""
+
+    def __init__(self, xs, ys, seed=0):
+        self.xs, self.ys, self.seed = int(xs), int(ys), int(seed)
+
+    def maximum_sample_count(self):
+        return self.xs * self.ys
+
+
+class DirectLightingIntegrator:
+    """Direct lighting: emitted light plus, per light, the mean of
+    light_samples light samples (shadow rays) -- pbrt-v2 DirectLighting with
+    light sampling only; the reference has none (integrator/mod.rs:148-168)."""
+    kind = 0
+
+    def __init__(self, light_samples=4):
+        self.light_samples = int(light_samples)
+
+
+class PathIntegrator:
+    """Path tracing with one light sample per vertex, Lambertian / mirror BSDF
+    sampling and Russian roulette once the bounce index exceeds rr_start."""
+    kind = 1
+
+    def __init__(self, max_depth=5, rr_start=3):
+        self.max_depth, self.rr_start = int(max_depth), int(rr_start)
+
+
+class SamplerRenderer:
+    """Renderer::render for a generated scene dict (see scenes.py), a camera
+    dict (scenes.pinhole) and an integrator; the frame runs on the GPU."""
+
+    def __init__(self, scene, camera, sampler, integrator, split="sah", chunk=None):
+        from ._lib import RenderDesc
+        self._keep = dict(
+            verts=np.ascontiguousarray(scene["verts"], F32).reshape(-1),
+            idx=np.ascontiguousarray(scene["idx"], np.uint32).reshape(-1),
+            face_mat=np.ascontiguousarray(scene["face_mat"], np.uint32),
+            mats=np.ascontiguousarray(scene["mats"], F32).reshape(-1),
+            face_light=np.ascontiguousarray(scene["face_light"], np.int32),
+            area_le=np.ascontiguousarray(scene["area_le"], F32).reshape(-1),
+            point_lights=np.ascontiguousarray(scene["point_lights"], F32).reshape(-1),
+        )
+        k = self._keep
+        d = RenderDesc()
+        d.verts, d.nverts = k["verts"].ctypes.data, k["verts"].size // 3
+        d.idx, d.nfaces = k["idx"].ctypes.data, k["idx"].size // 3
+        d.face_mat, d.mats, d.nmats = k["face_mat"].ctypes.data, k["mats"].ctypes.data, k["mats"].size // 4
+        d.face_light = k["face_light"].ctypes.data
+        d.area_le, d.n_area = (k["area_le"].ctypes.data if k["area_le"].size else None), k["area_le"].size // 3
+        d.point_lights = k["point_lights"].ctypes.data if k["point_lights"].size else None
+        d.n_point = k["point_lights"].size // 6
+        for i, v in enumerate(np.asarray(camera["cam_to_world"], F32).reshape(16)):
+            d.cam_to_world[i] = float(v)
+        d.fov_deg = camera["fov"]
+        for i, v in enumerate(camera["screen"]):
+            d.screen[i] = float(v)
+        d.xres, d.yres = camera["xres"], camera["yres"]
+        d.nx, d.ny, d.seed = sampler.xs, sampler.ys, sampler.seed
+        d.integrator = integrator.kind
+        d.light_samples = getattr(integrator, "light_samples", 1)
+        d.max_depth = getattr(integrator, "max_depth", 5)
+        d.rr_start = getattr(integrator, "rr_start", 3)
+        d.split = {"sah": 0, "middle": 1, "equal": 2}[split]
+        d.max_prims = 1
+        self.desc = d
+        self.xres, self.yres, self.spp = d.xres, d.yres, d.nx * d.ny
+        h = ctypes.c_void_p()
+        check(lib().pbrtx_renderer_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h.value
+        if chunk:
+            check(lib().pbrtx_renderer_set_chunk(self._h, int(chunk)))
+
+    def render(self, path="batched", want_image=True):
+        from ._lib import RenderStats
+        st = RenderStats()
+        out = np.zeros((self.yres, self.xres, 3), F32) if want_image else None
+        check(lib().pbrtx_renderer_render(self._h, _path_code(path), out.ctypes.data if want_image else None,
+                                          ctypes.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in st._fields_}
+        return out, stats
+
+    def sample_radiance(self, count=None):
+        n = self.xres * self.yres * self.spp if count is None else int(count)
+        out = np.zeros(n * 3, F32)
+        check(lib().pbrtx_renderer_sample_radiance(self._h, out.ctypes.data, n))
+        return out.reshape(n, 3)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbrtx_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1071 @@
+// render.hip -- SamplerRenderer::render on gfx950 (sampler_renderer.rs:61-206).
+//
+// Wavefront organisation: every stage is a kernel over a batch of samples or
+// paths, ray queries go through the traversal kernels of trace.hip.
+//   direct lighting: camera rays -> closest hit -> shade (light samples,
+//     shadow segments, contributions) -> occlusion -> accumulate -> film
+//   path tracing: camera rays, then per bounce closest hit -> shade (one
+//     light sample + BSDF sample + roulette) -> occlusion -> accumulate.
+// PATH_PLAIN keeps one slot per sample / path and traces every slot each
+// stage (dead slots carry an empty interval); PATH_BATCHED compacts live
+// shadow and extension rays into dense queues with wave-aggregated atomics
+// (ballot + prefix rank) and traces them with the persistent kernel, whose
+// ray count is read from device memory.
+//
+// The arithmetic of every stage is that of the CPU restatement: the same
+// IEEE operations in the same order (no contraction), the same
+// counter-based sample values, so images match sample for sample.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "../../include/pbrtx_gpu.h"
+#include "capi_internal.hpp"
+
+namespace px {
+
+struct FaceShade {  // per face, precomputed with the hit-time formulas
+    float nn[3];    // dg.nn = normalize(dpdu x dpdv)  (mesh.rs:224-254, diff_geom.rs:52-60)
+    float tn[3];    // normalize(dpdu)                  (bsdf/mod.rs:70-73)
+    uint32_t mat;
+    int32_t light;
+};
+struct AreaTriRec {
+    float p1[3], p2[3], p3[3], nn[3];
+    float area, pad0, pad1, pad2;
+};
+struct MatRec {
+    uint32_t kind;
+    float r, g, b;
+};
+
+struct RenderParams {
+    float r2c[16], c2w[16];
+    uint32_t xres, yres, nx, ny, spp, seed;
+    uint32_t n_point, n_area, nlights, light_samples, max_depth, rr_start, S;
+    float dx, dy;
+    const FaceShade* faces;
+    const MatRec* mats;
+    const AreaTriRec* atris;
+    const uint32_t* a_first;
+    const uint32_t* a_count;
+    const float* a_total;
+    const float* a_le;     // 3 per area light
+    const float* points;   // 6 per point light
+};
+
+struct PathState {
+    float beta[3];
+    float L[3];
+    uint32_t gid;
+    uint32_t flags;  // 1 specular, 2 done after this bounce's shadow ray
+};
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int RB = 256;  // render kernel block
+
+__device__ __forceinline__ f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+// ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
+__device__ __forceinline__ f3 xf_point(const float* m, f3 p) {
+    float xt = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    float yt = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    float zt = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    if (w != 1.0f) return mk3(xt / w, yt / w, zt / w);
+    return mk3(xt, yt, zt);
+}
+__device__ __forceinline__ f3 xf_vector(const float* m, f3 v) {
+    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+               m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+
+// StratifiedSampler image sample of pixel (px,py), sample s (stratified.rs:60-83,
+// montecarlo.rs:119-128) with the counter-based jitter.
+__device__ __forceinline__ void sample_pos(const RenderParams& P, uint32_t px_, uint32_t py, uint32_t s,
+                                           float* ix, float* iy) {
+    float u[4];
+    rng4(P.seed, py * P.xres + px_, s, 0, u);
+    uint32_t sx = s % P.nx, sy = s / P.nx;
+    *ix = (((float)sx + u[0]) * P.dx) + (float)px_;
+    *iy = (((float)sy + u[1]) * P.dy) + (float)py;
+}
+
+// Camera::generate_ray_differential's main ray (camera/mod.rs:168-195,
+// 226-236, 269-270) with a static cam_to_world.
+__device__ __forceinline__ void camera_ray(const RenderParams& P, float ix, float iy, Ray* r) {
+    f3 pc = xf_point(P.r2c, mk3(ix, iy, 0.0f));
+    f3 d = normalize3(pc);
+    f3 o = xf_point(P.c2w, mk3(0.0f, 0.0f, 0.0f));
+    f3 dw = xf_vector(P.c2w, d);
+    *r = Ray{o.x, o.y, o.z, 0.0f, dw.x, dw.y, dw.z, FLT_MAX};
+}
+
+__device__ __forceinline__ bool black3(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ f3 mulc(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 divs3(f3 a, float f) { return mk3(a.x / f, a.y / f, a.z / f); }
+
+// One light sample at p (normal n, outgoing wo, material mat): the shadow
+// segment and its contribution f*Li*|wi.n|/pdf; false when nothing would be
+// traced.  PointLight::sample_l (point.rs:28-35; wi normalised), the quad area
+// light (uniform by area, pdf d^2/(cos*A)), VisibilityTester::segment
+// (visibility_tester.rs:16-24), BSDF::f with pbrt's flag test (bsdf/mod.rs:132-149).
+__device__ bool sample_light(const RenderParams& P, uint32_t li, f3 p, float eps, f3 n, f3 wo,
+                             const MatRec& mat, float u0, float u1, float u2, Ray* out, f3* c) {
+    f3 Li, wi, target;
+    float pdf, eps2;
+    if (li < P.n_point) {
+        const float* q = P.points + 6 * li;
+        f3 pos = ld3(q);
+        f3 w = sub3(pos, p);
+        float d2 = lensq3(w);
+        Li = divs3(ld3(q + 3), d2);
+        wi = normalize3(w);
+        pdf = 1.0f;
+        target = pos;
+        eps2 = 0.0f;
+    } else {
+        const uint32_t la = li - P.n_point;
+        const uint32_t first = P.a_first[la], cnt = P.a_count[la];
+        const float total = P.a_total[la];
+        float tgt = u0 * total;
+        uint32_t k = 0;
+        float acc = P.atris[first].area;
+        while (k + 1 < cnt && tgt >= acc) { ++k; acc = acc + P.atris[first + k].area; }
+        const AreaTriRec& t = P.atris[first + k];
+        float su1 = sqrtf(u1);
+        float b1 = 1.0f - su1;
+        float b2 = u2 * su1;
+        float b0 = (1.0f - b1) - b2;
+        f3 ps = add3(add3(mul3(ld3(t.p1), b1), mul3(ld3(t.p2), b2)), mul3(ld3(t.p3), b0));
+        f3 w = sub3(ps, p);
+        float d2 = lensq3(w);
+        float dist = sqrtf(d2);
+        wi = mul3(w, 1.0f / dist);
+        float cosl = dot3(ld3(t.nn), neg3(wi));
+        if (!(cosl > 0.0f)) return false;
+        Li = ld3(P.a_le + 3 * la);
+        pdf = d2 / (cosl * total);
+        target = ps;
+        eps2 = 1e-3f;
+    }
+    if (black3(Li) || pdf == 0.0f) return false;
+    f3 f = mk3(0.0f, 0.0f, 0.0f);
+    if (mat.kind == 0u && dot3(wi, n) * dot3(wo, n) > 0.0f) f = mul3(mk3(mat.r, mat.g, mat.b), 1.0f / PI_F);
+    if (black3(f)) return false;
+    float dist = sqrtf(lensq3(sub3(p, target)));
+    f3 dir = mul3(sub3(target, p), 1.0f / dist);
+    *out = Ray{p.x, p.y, p.z, eps, dir.x, dir.y, dir.z, (1.0f - eps2) * dist};
+    *c = divs3(mul3(mulc(f, Li), fabsf(dot3(wi, n))), pdf);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long lt_mask() {
+    uint32_t l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+// wave-aggregated slot allocation: every lane of the wave calls it
+__device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return NONE;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    return want ? base + (uint32_t)__popcll(m & lt_mask()) : NONE;
+}
+
+__device__ __forceinline__ void set_dead(Ray* r) { *r = Ray{0, 0, 0, 1.0f, 0, 0, 1.0f, 0.0f}; }
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(RB) void k_camera(RenderParams P, uint64_t g0, uint32_t n, Ray* rays) {
+    for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+        uint64_t g = g0 + i;
+        uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
+        float ix, iy;
+        sample_pos(P, pixel % P.xres, pixel / P.xres, s, &ix, &iy);
+        Ray r;
+        camera_ray(P, ix, iy, &r);
+        rays[i] = r;
+    }
+}
+
+// Direct lighting shade (deviation D1 of the build: emitted light plus, per
+// light, the mean over its samples of the unoccluded contributions).
+// Slots per sample: point lights first, then light_samples per area light.
+__global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
+                                                     const Hit* hits, Ray* sh, float4* contrib,
+                                                     uint32_t* qidx, float4* lbase, uint32_t* sh_count,
+                                                     int compact) {
+    const uint32_t S = P.S;
+    for (uint32_t base = blockIdx.x * RB; base < n; base += gridDim.x * RB) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < n;
+        Hit h = live ? hits[i] : Hit{NO_HIT, 0, 0, 0};
+        const bool hit = live && h.prim != NO_HIT;
+        f3 p = mk3(0, 0, 0), nn = mk3(0, 0, 0), wo = mk3(0, 0, 0);
+        float eps = 0.0f;
+        MatRec mat{1u, 0, 0, 0};
+        uint32_t pixel = 0, s = 0;
+        if (hit) {
+            const Ray r = rays[i];
+            p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
+            const FaceShade fs = P.faces[h.prim];
+            nn = ld3(fs.nn);
+            wo = neg3(mk3(r.dx, r.dy, r.dz));
+            eps = h.t * 5e-4f;  // mesh.rs:262
+            mat = P.mats[fs.mat];
+            f3 le = mk3(0, 0, 0);
+            if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+            lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
+            uint64_t g = g0 + i;
+            pixel = (uint32_t)(g / P.spp); s = (uint32_t)(g % P.spp);
+        } else if (live) {
+            lbase[i] = make_float4(0, 0, 0, 0);
+        }
+        for (uint32_t k = 0; k < S; ++k) {
+            bool valid = false;
+            Ray sr;
+            f3 c = mk3(0, 0, 0);
+            if (hit) {
+                uint32_t li, j;
+                if (k < P.n_point) { li = k; j = 0; }
+                else { li = P.n_point + (k - P.n_point) / P.light_samples; j = (k - P.n_point) % P.light_samples; }
+                float u[4] = {0, 0, 0, 0};
+                if (li >= P.n_point) rng4(P.seed, pixel, s, 1 + (li - P.n_point) * P.light_samples + j, u);
+                valid = sample_light(P, li, p, eps, nn, wo, mat, u[0], u[1], u[2], &sr, &c);
+            }
+            const size_t slot = (size_t)i * S + k;
+            if (compact) {
+                const uint32_t q = wave_alloc(valid, sh_count);
+                if (valid) sh[q] = sr;
+                if (live) qidx[slot] = q;
+            } else {
+                if (!valid) set_dead(&sr);
+                if (live) {
+                    sh[slot] = sr;
+                    qidx[slot] = valid ? (uint32_t)slot : NONE;
+                }
+                const unsigned long long m = __ballot(valid);
+                if (lane_id() == 0 && m) atomicAdd(sh_count, (uint32_t)__popcll(m));
+            }
+            if (live) contrib[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0, uint32_t n, const float4* lbase,
+                                                     const float4* contrib, const uint32_t* qidx,
+                                                     const uint32_t* occ, float4* sample_l) {
+    for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+        const float4 lb = lbase[i];
+        f3 L = mk3(lb.x, lb.y, lb.z);
+        uint32_t k = 0;
+        for (uint32_t li = 0; li < P.nlights; ++li) {
+            const uint32_t ns = li < P.n_point ? 1u : P.light_samples;
+            f3 Ld = mk3(0, 0, 0);
+            for (uint32_t j = 0; j < ns; ++j, ++k) {
+                const size_t slot = (size_t)i * P.S + k;
+                const uint32_t q = qidx[slot];
+                if (q == NONE) continue;
+                if ((occ[q >> 5] >> (q & 31u)) & 1u) continue;
+                const float4 c = contrib[slot];
+                Ld = add3(Ld, mk3(c.x, c.y, c.z));
+            }
+            L = add3(L, divs3(Ld, (float)ns));
+        }
+        sample_l[g0 + i] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+}
+
+// 64-bit running total (lo, hi words) += *add
+__global__ void k_add_counter(uint32_t* lo, const uint32_t* add, uint32_t* hi) {
+    uint64_t v = ((uint64_t)*hi << 32) | (uint64_t)*lo;
+    v += *add;
+    *lo = (uint32_t)v;
+    *hi = (uint32_t)(v >> 32);
+}
+
+// ---- path tracing ------------------------------------------------------------
+__global__ __launch_bounds__(RB) void k_path_init(RenderParams P, uint64_t g0, uint32_t n, Ray* rays,
+                                                  PathState* st, uint32_t* pid_q, uint32_t* q_count) {
+    for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+        uint64_t g = g0 + i;
+        uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
+        float ix, iy;
+        sample_pos(P, pixel % P.xres, pixel / P.xres, s, &ix, &iy);
+        Ray r;
+        camera_ray(P, ix, iy, &r);
+        rays[i] = r;
+        st[i] = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
+        if (pid_q) pid_q[i] = i;
+    }
+    if (q_count && blockIdx.x == 0 && threadIdx.x == 0) *q_count = n;
+}
+
+// One path vertex (PathIntegrator, deviation D1): emission on the first or a
+// specular bounce, one light sample (no MIS), BSDF sample, roulette.
+// Queue form: paths pid_q[0..*q_count) with their rays rays_q / hits_q;
+// plain form (pid_q == nullptr): every path i with rays[i] / hits[i].
+__global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, uint32_t bounce, uint32_t cap,
+                                                   const uint32_t* q_count, const uint32_t* pid_q,
+                                                   const Ray* rays_in, const Hit* hits, PathState* st,
+                                                   Ray* rays_out, uint32_t* pid_out, uint32_t* out_count,
+                                                   Ray* sh, uint32_t* sh_pid, float4* shc,
+                                                   uint32_t* sh_count, float4* sample_l) {
+    const uint32_t n = q_count ? *q_count : cap;
+    const bool queued = pid_q != nullptr;
+    for (uint32_t base = blockIdx.x * RB; base < n; base += gridDim.x * RB) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < n;
+        uint32_t pid = live ? (queued ? pid_q[i] : i) : 0u;
+        PathState ps{};
+        bool active = false;
+        if (live) {
+            ps = st[pid];
+            active = queued || !(ps.flags & 4u);  // plain form: bit 4 marks a finished path
+        }
+        Hit h = active ? hits[i] : Hit{NO_HIT, 0, 0, 0};
+        bool want_shadow = false, want_next = false, finish = false;
+        Ray sray, nray;
+        f3 contrib = mk3(0, 0, 0);
+        if (active && h.prim == NO_HIT) {
+            finish = true;  // a miss ends the path (no infinite lights)
+        } else if (active) {
+            const Ray r = rays_in[i];
+            const f3 d = mk3(r.dx, r.dy, r.dz);
+            const f3 p = add3(mk3(r.ox, r.oy, r.oz), mul3(d, h.t));
+            const FaceShade fs = P.faces[h.prim];
+            const f3 nn = ld3(fs.nn);
+            const f3 wo = neg3(d);
+            const float eps = h.t * 5e-4f;
+            f3 beta = ld3(ps.beta), L = ld3(ps.L);
+            const bool specular = (ps.flags & 1u) != 0u;
+            if (bounce == 0 || specular) {
+                f3 le = mk3(0, 0, 0);
+                if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+                L = add3(L, mulc(beta, le));
+            }
+            const MatRec mat = P.mats[fs.mat];
+            const uint64_t g = g0 + ps.gid;
+            const uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
+            float ua[4], ub[4];
+            rng4(P.seed, pixel, s, 1 + 2 * bounce, ua);
+            rng4(P.seed, pixel, s, 2 + 2 * bounce, ub);
+            if (P.nlights > 0 && mat.kind == 0u) {
+                uint32_t ln = (uint32_t)(ua[0] * (float)P.nlights);
+                if (ln > P.nlights - 1) ln = P.nlights - 1;
+                f3 c;
+                if (sample_light(P, ln, p, eps, nn, wo, mat, ua[1], ua[2], ua[3], &sray, &c)) {
+                    want_shadow = true;
+                    contrib = mulc(beta, mul3(c, (float)P.nlights));
+                }
+            }
+            bool cont = bounce != P.max_depth;
+            if (cont) {
+                // BSDF frame (bsdf/mod.rs:69-84) and world_to_local (:103-105)
+                const f3 tn = ld3(fs.tn);
+                const f3 sn = cross3(nn, tn);
+                const f3 wol = mk3(dot3(wo, sn), dot3(wo, tn), dot3(wo, nn));
+                f3 wil, f;
+                float pdf;
+                bool spec;
+                if (mat.kind == 0u) {
+                    float dx, dy;
+                    concentric_disk(ub[0], ub[1], &dx, &dy);
+                    float z = sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
+                    wil = mk3(dx, dy, z);
+                    if (wol.z < 0.0f) wil.z = -wil.z;
+                    pdf = (wol.z * wil.z > 0.0f) ? fabsf(wil.z) * (1.0f / PI_F) : 0.0f;
+                    f = mul3(mk3(mat.r, mat.g, mat.b), 1.0f / PI_F);
+                    spec = false;
+                } else {
+                    // SpecularReflection::sample_f (specular.rs:34-40), Fresnel::noop
+                    wil = mk3(-wol.x, -wol.y, wol.z);
+                    pdf = 1.0f;
+                    f = divs3(mk3(mat.r, mat.g, mat.b), fabsf(wil.z));
+                    spec = true;
+                }
+                if (black3(f) || pdf == 0.0f) {
+                    cont = false;
+                } else {
+                    // local_to_world: bsdf/mod.rs:107-111
+                    const f3 wi = mk3(sn.x * wil.x + tn.x * wil.y + nn.x * wil.z,
+                                      sn.y * wil.x + tn.y * wil.y + nn.y * wil.z,
+                                      sn.z * wil.x + tn.z * wil.y + nn.z * wil.z);
+                    ps.flags = spec ? (ps.flags | 1u) : (ps.flags & ~1u);
+                    beta = mulc(beta, divs3(mul3(f, fabsf(dot3(wi, nn))), pdf));
+                    if (bounce > P.rr_start) {
+                        const float y = 0.212671f * beta.x + 0.715160f * beta.y + 0.072169f * beta.z;
+                        const float q = (y < 0.5f) ? y : 0.5f;
+                        if (ub[2] > q) cont = false;
+                        else beta = divs3(beta, q);
+                    }
+                    if (cont) nray = Ray{p.x, p.y, p.z, eps, wi.x, wi.y, wi.z, FLT_MAX};
+                }
+            }
+            ps.beta[0] = beta.x; ps.beta[1] = beta.y; ps.beta[2] = beta.z;
+            ps.L[0] = L.x; ps.L[1] = L.y; ps.L[2] = L.z;
+            want_next = cont;
+            finish = !cont;
+        }
+        // paths that end with no shadow ray pending are final now
+        if (live && active) {
+            if (finish && !want_shadow) {
+                sample_l[g0 + ps.gid] = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
+                ps.flags |= 4u;
+            } else if (finish) {
+                ps.flags |= 2u;
+            }
+            st[pid] = ps;
+        }
+        if (queued) {
+            const uint32_t qs = wave_alloc(want_shadow, sh_count);
+            if (want_shadow) { sh[qs] = sray; sh_pid[qs] = pid; shc[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f); }
+            const uint32_t qn = wave_alloc(want_next, out_count);
+            if (want_next) { rays_out[qn] = nray; pid_out[qn] = pid; }
+        } else {
+            if (live) {
+                if (!want_shadow) set_dead(&sray);
+                if (!want_next) set_dead(&nray);
+                sh[i] = sray;
+                shc[i] = make_float4(contrib.x, contrib.y, contrib.z, want_shadow ? 1.0f : 0.0f);
+                rays_out[i] = nray;
+            }
+            const unsigned long long ms = __ballot(want_shadow), mn = __ballot(want_next);
+            if (lane_id() == 0) {
+                if (ms) atomicAdd(sh_count, (uint32_t)__popcll(ms));
+                if (mn) atomicAdd(out_count, (uint32_t)__popcll(mn));
+            }
+        }
+    }
+}
+
+// Adds the unoccluded light samples of this bounce; finishes paths that ended.
+__global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0, uint32_t cap,
+                                                    const uint32_t* sh_count, const uint32_t* sh_pid,
+                                                    const float4* shc, const uint32_t* occ, PathState* st,
+                                                    float4* sample_l) {
+    const uint32_t n = sh_count ? *sh_count : cap;
+    for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
+        const uint32_t pid = sh_pid ? sh_pid[i] : i;
+        const float4 c = shc[i];
+        PathState ps = st[pid];
+        if (!sh_pid) {
+            if (ps.flags & 4u) continue;   // plain form: finished earlier
+            if (c.w == 0.0f) continue;     // no light sample this bounce
+        }
+        if (!((occ[i >> 5] >> (i & 31u)) & 1u)) {
+            ps.L[0] = ps.L[0] + c.x; ps.L[1] = ps.L[1] + c.y; ps.L[2] = ps.L[2] + c.z;
+        }
+        if (ps.flags & 2u) {
+            sample_l[g0 + ps.gid] = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
+            ps.flags = (ps.flags & ~2u) | 4u;
+        }
+        st[pid] = ps;
+    }
+}
+
+// Paths still alive when the last bounce has been processed (plain form keeps
+// them in place) or finished in the plain form: nothing to do -- every path
+// writes its radiance when it ends.
+
+// Film: box filter of half-width 0.5 (Filter::mean, filter.rs:89;
+// Film::add_sample film.rs:192-249), gathered per pixel from the samples of
+// the 3x3 neighbourhood in row-major order; write_image's conversion with the
+// buffer and splat fixed (film.rs:316-354).
+__device__ __forceinline__ bool covers(float ixf, int32_t x) {
+    float dimg = ixf - 0.5f;
+    int32_t a = (int32_t)ceilf(dimg - 0.5f), b = (int32_t)floorf(dimg + 0.5f);
+    return a <= x && x <= b;
+}
+__global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sample_l, float* rgb) {
+    const uint32_t npix = P.xres * P.yres;
+    for (uint32_t pix = blockIdx.x * RB + threadIdx.x; pix < npix; pix += gridDim.x * RB) {
+        const int32_t x = (int32_t)(pix % P.xres), y = (int32_t)(pix / P.xres);
+        float xyz[3] = {0.0f, 0.0f, 0.0f};
+        float wsum = 0.0f;
+        for (int32_t qy = y - 1; qy <= y + 1; ++qy) {
+            if (qy < 0 || qy >= (int32_t)P.yres) continue;
+            for (int32_t qx = x - 1; qx <= x + 1; ++qx) {
+                if (qx < 0 || qx >= (int32_t)P.xres) continue;
+                const uint32_t qpix = (uint32_t)qy * P.xres + (uint32_t)qx;
+                for (uint32_t s = 0; s < P.spp; ++s) {
+                    float ix, iy;
+                    sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                    if (!covers(ix, x) || !covers(iy, y)) continue;
+                    const float4 v = sample_l[(size_t)qpix * P.spp + s];
+                    // rgb_to_xyz: spectrum.rs:37-41
+                    const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
+                    const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
+                    const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
+                    const float wt = 1.0f;
+                    xyz[0] = xyz[0] + wt * c0;
+                    xyz[1] = xyz[1] + wt * c1;
+                    xyz[2] = xyz[2] + wt * c2;
+                    wsum = wsum + wt;
+                }
+            }
+        }
+        // xyz_to_rgb: spectrum.rs:31-35
+        float o0 = 3.240479f * xyz[0] - 1.37150f * xyz[1] - 0.498535f * xyz[2];
+        float o1 = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
+        float o2 = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
+        if (wsum != 0.0f) {
+            const float inv = 1.0f / wsum;
+            o0 = fmaxf(o0 * inv, 0.0f); o1 = fmaxf(o1 * inv, 0.0f); o2 = fmaxf(o2 * inv, 0.0f);
+        }
+        rgb[3 * (size_t)pix] = o0; rgb[3 * (size_t)pix + 1] = o1; rgb[3 * (size_t)pix + 2] = o2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+// ---- Camera::perspective + Projection::new (camera/mod.rs:105-135,
+// projective.rs:48-72) with the LU inverse of matrix4x4.rs:66-160.
+struct HM4 {
+    float m[4][4];
+};
+HM4 hid() {
+    HM4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = i == j ? 1.0f : 0.0f;
+    return r;
+}
+HM4 hmul(const HM4& a, const HM4& b) {
+    HM4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j] +
+                        a.m[i][3] * b.m[3][j];
+    return r;
+}
+bool hinv(const HM4& a, HM4& out) {
+    float s[4];
+    for (int i = 0; i < 4; ++i) {
+        float acc = 0.0f;
+        for (int j = 0; j < 4; ++j) acc = fmaxf(fabsf(a.m[i][j]), acc);
+        s[i] = acc;
+    }
+    HM4 lu = a;
+    int piv[4] = {0, 1, 2, 3};
+    for (int k = 0; k < 3; ++k) {
+        float c = 0.0f;
+        int p = k;
+        for (int i = k; i < 4; ++i) {
+            float cc = fabsf(lu.m[i][k] / s[i]);
+            if (cc > c) { c = cc; p = i; }
+        }
+        piv[k] = p;
+        if (c == 0.0f) return false;
+        if (p != k)
+            for (int j = k; j < 4; ++j) { float t = lu.m[k][j]; lu.m[k][j] = lu.m[p][j]; lu.m[p][j] = t; }
+        for (int i = k + 1; i < 4; ++i) {
+            float mi = lu.m[i][k] / lu.m[k][k];
+            lu.m[i][k] = mi;
+            for (int j = k + 1; j < 4; ++j) lu.m[i][j] = lu.m[i][j] - mi * lu.m[k][j];
+        }
+    }
+    if (fabsf(lu.m[3][3]) < 1.0e-6f) return false;
+    HM4 rows;
+    for (int c = 0; c < 4; ++c) {
+        float b[4] = {0, 0, 0, 0};
+        b[c] = 1.0f;
+        for (int k = 0; k < 3; ++k) {
+            if (piv[k] != k) { float t = b[piv[k]]; b[piv[k]] = b[k]; b[k] = t; }
+            for (int i = k + 1; i < 4; ++i) b[i] = b[i] - lu.m[i][k] * b[k];
+        }
+        b[3] = b[3] / lu.m[3][3];
+        for (int i = 2; i >= 0; --i) {
+            float sum = 0.0f;
+            for (int j = i + 1; j < 4; ++j) sum = sum + lu.m[i][j] * b[j];
+            b[i] = (b[i] - sum) / lu.m[i][i];
+        }
+        for (int j = 0; j < 4; ++j) rows.m[c][j] = b[j];
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out.m[i][j] = rows.m[j][i];
+    return true;
+}
+struct HXf {
+    HM4 m, mi;
+};
+HXf hscale(float x, float y, float z) {
+    HXf t{hid(), hid()};
+    t.m.m[0][0] = x; t.m.m[1][1] = y; t.m.m[2][2] = z;
+    t.mi.m[0][0] = 1.0f / x; t.mi.m[1][1] = 1.0f / y; t.mi.m[2][2] = 1.0f / z;
+    return t;
+}
+HXf htranslate(float x, float y, float z) {
+    HXf t{hid(), hid()};
+    t.m.m[0][3] = x; t.m.m[1][3] = y; t.m.m[2][3] = z;
+    t.mi.m[0][3] = -x; t.mi.m[1][3] = -y; t.mi.m[2][3] = -z;
+    return t;
+}
+HXf hxmul(const HXf& a, const HXf& b) { return HXf{hmul(a.m, b.m), hmul(b.mi, a.mi)}; }
+HXf hxinv(const HXf& a) { return HXf{a.mi, a.m}; }
+
+bool perspective_r2c(uint32_t xres, uint32_t yres, const float screen[4], float fov, float out[16]) {
+    const float znear = 1e-2f, zfar = 1000.0f;
+    HM4 p = hid();
+    p.m[2][2] = zfar / (zfar - znear);
+    p.m[2][3] = -(zfar * znear) / (zfar - znear);
+    p.m[3][2] = 1.0f;
+    p.m[3][3] = 0.0f;
+    HM4 pinv;
+    if (!hinv(p, pinv)) return false;
+    const float inv_tan = 1.0f / tanf((fov * PI_F / 180.0f) / 2.0f);
+    HXf persp = hxmul(hscale(inv_tan, inv_tan, 1.0f), HXf{p, pinv});
+    HXf s2r = hxmul(hxmul(hscale((float)xres, (float)yres, 1.0f),
+                          hscale(1.0f / (screen[1] - screen[0]), 1.0f / (screen[2] - screen[3]), 1.0f)),
+                    htranslate(-screen[0], -screen[3], 0.0f));
+    HXf r2c = hxmul(hxinv(persp), hxinv(s2r));
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = r2c.m.m[i][j];
+    return true;
+}
+
+template <typename T>
+struct DevArr {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count) {
+        if (count <= n) return PBRTX_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T) + 16);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(render buffer)");
+        n = count;
+        return PBRTX_OK;
+    }
+    int upload(const std::vector<T>& v) {
+        int rc = ensure(v.size() ? v.size() : 1);
+        if (rc) return rc;
+        if (v.empty()) return PBRTX_OK;
+        hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(render upload)");
+        return PBRTX_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+int grid_for(size_t n) {
+    size_t g = (n + RB - 1) / RB;
+    size_t cap = (size_t)device_cu_count() * 16;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+struct EventPool {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    hipEvent_t next() {
+        if (used == ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev.push_back(e);
+        }
+        return ev[used++];
+    }
+    ~EventPool() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct Renderer {
+    RenderParams P{};
+    HostBvh bvh;
+    DeviceScene scene{};
+    uint32_t nfaces = 0;
+    uint64_t n_samples = 0;
+    uint32_t chunk = 1u << 22;
+    DevArr<FaceShade> faces;
+    DevArr<MatRec> mats;
+    DevArr<AreaTriRec> atris;
+    DevArr<uint32_t> a_first, a_count;
+    DevArr<float> a_total, a_le, points;
+    // per-chunk work buffers
+    DevArr<Ray> rays, rays2, sh;
+    DevArr<Hit> hits;
+    DevArr<float4> contrib, lbase, sample_l;
+    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters;
+    DevArr<PathState> st;
+    DevArr<float> rgb;
+    hipStream_t stream = nullptr;
+    EventPool events;
+    int integrator = 0;
+    double build_ms = 0.0;
+};
+
+static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
+    if (!d->verts || !d->idx || !d->face_mat || !d->mats) return fail(PBRTX_ERR_ARG, "null scene arrays");
+    if (d->nx == 0 || d->ny == 0 || d->xres == 0 || d->yres == 0) return fail(PBRTX_ERR_ARG, "empty image");
+    if (d->integrator != 0 && d->integrator != 1) return fail(PBRTX_ERR_ARG, "unknown integrator");
+    if (d->integrator == 0 && d->n_area > 0 && d->light_samples == 0)
+        return fail(PBRTX_ERR_ARG, "light_samples must be >= 1");
+    for (uint32_t f = 0; f < d->nfaces; ++f)
+        if (d->face_mat[f] >= d->nmats) return fail(PBRTX_ERR_ARG, "face material out of range");
+    std::string err;
+    if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, nullptr, 0, d->split, d->max_prims, &R->bvh, &err))
+        return fail(PBRTX_ERR_BUILD, err);
+    R->build_ms = R->bvh.build_ms;
+    int rc = scene_from_host(R->bvh, &R->scene);
+    if (rc) return rc;
+    R->nfaces = d->nfaces;
+    auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
+    std::vector<FaceShade> fsh(d->nfaces);
+    std::vector<std::vector<AreaTriRec>> per_light(d->n_area);
+    for (uint32_t f = 0; f < d->nfaces; ++f) {
+        // Triangle::intersect dg with the default uvs (mesh.rs:212-254)
+        const f3 p1 = V(d->idx[3 * f + 2]), p2 = V(d->idx[3 * f + 1]), p3 = V(d->idx[3 * f]);
+        const float du1 = 0.0f - 1.0f, du2 = 1.0f - 1.0f, dv1 = 0.0f - 1.0f, dv2 = 0.0f - 1.0f;
+        const f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
+        const float det = du1 * dv2 - dv1 * du2;
+        const float inv_det = 1.0f / det;
+        const f3 dpdu = mul3(sub3(mul3(dp1, dv2), mul3(dp2, dv1)), inv_det);
+        const f3 dpdv = mul3(add3(mul3(dp1, -du2), mul3(dp2, du1)), inv_det);
+        const f3 nn = normalize3(cross3(dpdu, dpdv));
+        const f3 tn = normalize3(dpdu);
+        FaceShade& s = fsh[f];
+        s.nn[0] = nn.x; s.nn[1] = nn.y; s.nn[2] = nn.z;
+        s.tn[0] = tn.x; s.tn[1] = tn.y; s.tn[2] = tn.z;
+        s.mat = d->face_mat[f];
+        s.light = d->face_light ? d->face_light[f] : -1;
+        if (s.light >= 0) {
+            if ((uint32_t)s.light >= d->n_area) return fail(PBRTX_ERR_ARG, "face light out of range");
+            AreaTriRec t{};
+            t.p1[0] = p1.x; t.p1[1] = p1.y; t.p1[2] = p1.z;
+            t.p2[0] = p2.x; t.p2[1] = p2.y; t.p2[2] = p2.z;
+            t.p3[0] = p3.x; t.p3[1] = p3.y; t.p3[2] = p3.z;
+            t.nn[0] = nn.x; t.nn[1] = nn.y; t.nn[2] = nn.z;
+            t.area = 0.5f * sqrtf(lensq3(cross3(sub3(p2, p1), sub3(p3, p1))));  // mesh.rs:100-103
+            per_light[(size_t)s.light].push_back(t);
+        }
+    }
+    std::vector<AreaTriRec> at;
+    std::vector<uint32_t> afirst, acount;
+    std::vector<float> atotal, ale;
+    for (uint32_t l = 0; l < d->n_area; ++l) {
+        if (per_light[l].empty()) return fail(PBRTX_ERR_ARG, "area light without triangles");
+        afirst.push_back((uint32_t)at.size());
+        acount.push_back((uint32_t)per_light[l].size());
+        float total = 0.0f;
+        for (auto& t : per_light[l]) { total = total + t.area; at.push_back(t); }
+        atotal.push_back(total);
+        for (int c = 0; c < 3; ++c) ale.push_back(d->area_le[3 * l + c]);
+    }
+    std::vector<MatRec> mats(d->nmats);
+    for (uint32_t m = 0; m < d->nmats; ++m)
+        mats[m] = MatRec{(uint32_t)d->mats[4 * m], d->mats[4 * m + 1], d->mats[4 * m + 2], d->mats[4 * m + 3]};
+    std::vector<float> pts(d->point_lights ? d->point_lights : (const float*)nullptr,
+                           d->point_lights ? d->point_lights + 6 * d->n_point : (const float*)nullptr);
+    if ((rc = R->faces.upload(fsh)) || (rc = R->mats.upload(mats)) || (rc = R->atris.upload(at)) ||
+        (rc = R->a_first.upload(afirst)) || (rc = R->a_count.upload(acount)) || (rc = R->a_total.upload(atotal)) ||
+        (rc = R->a_le.upload(ale)) || (rc = R->points.upload(pts)))
+        return rc;
+    RenderParams& P = R->P;
+    if (!perspective_r2c(d->xres, d->yres, d->screen, d->fov_deg, P.r2c))
+        return fail(PBRTX_ERR_ARG, "singular projection");
+    memcpy(P.c2w, d->cam_to_world, sizeof(P.c2w));
+    P.xres = d->xres; P.yres = d->yres; P.nx = d->nx; P.ny = d->ny; P.spp = d->nx * d->ny; P.seed = d->seed;
+    P.n_point = d->n_point; P.n_area = d->n_area; P.nlights = d->n_point + d->n_area;
+    P.light_samples = d->light_samples ? d->light_samples : 1;
+    P.max_depth = d->max_depth; P.rr_start = d->rr_start;
+    P.S = P.n_point + P.n_area * P.light_samples;
+    P.dx = 1.0f / (float)d->nx; P.dy = 1.0f / (float)d->ny;
+    P.faces = R->faces.p; P.mats = R->mats.p; P.atris = R->atris.p;
+    P.a_first = R->a_first.p; P.a_count = R->a_count.p; P.a_total = R->a_total.p; P.a_le = R->a_le.p;
+    P.points = R->points.p;
+    R->integrator = d->integrator;
+    R->n_samples = (uint64_t)d->xres * d->yres * P.spp;
+    if (R->n_samples >= (1ull << 40)) return fail(PBRTX_ERR_ARG, "image too large");
+    return PBRTX_OK;
+}
+
+#define RCHK(x)                                  \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, #x); \
+    } while (0)
+#define ICHK(x)              \
+    do {                     \
+        int r_ = (x);        \
+        if (r_) return r_;   \
+    } while (0)
+
+struct Timing {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, any, other;
+};
+
+static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
+    RenderParams& P = R->P;
+    hipStream_t st = R->stream;
+    const uint32_t spp = P.spp;
+    // chunk: whole pixels, at most R->chunk samples
+    uint64_t per = (uint64_t)R->chunk / spp * spp;
+    if (per < spp) per = spp;
+    if (per > R->n_samples) per = R->n_samples;
+    const uint32_t cap = (uint32_t)per;
+    ICHK(R->sample_l.ensure(R->n_samples));
+    ICHK(R->rays.ensure(cap));
+    ICHK(R->hits.ensure(cap));
+    ICHK(R->counters.ensure(16));
+    const bool batched = path == PATH_BATCHED;
+    const TraceTuning& tun = g_tuning[path];
+    TraceScratch* scr;
+    ICHK(stream_scratch(st, trace_spill_bytes(R->scene.view, path, tun), &scr));
+    R->events.used = 0;
+    Timing T;
+    auto ev = [&]() { return R->events.next(); };
+    uint32_t* cnt = R->counters.p;  // [0] shadow, [1] ext/next A, [2] next B, [3] camera, [4..] totals
+    RCHK(fill_u32(cnt, 0u, 16, st));
+    hipEvent_t e_start = ev();
+    RCHK(hipEventRecord(e_start, st));
+    const SceneView& sv = R->scene.view;
+    uint64_t total_camera = 0;
+    for (uint64_t g0 = 0; g0 < R->n_samples; g0 += per) {
+        const uint32_t n = (uint32_t)((R->n_samples - g0) < per ? (R->n_samples - g0) : per);
+        total_camera += n;
+        if (R->integrator == 0) {
+            const size_t slots = (size_t)n * P.S;
+            ICHK(R->sh.ensure(slots ? slots : 1));
+            ICHK(R->contrib.ensure(slots ? slots : 1));
+            ICHK(R->qidx.ensure(slots ? slots : 1));
+            ICHK(R->lbase.ensure(n));
+            ICHK(R->occ.ensure((slots + 31) / 32 + 1));
+            hipEvent_t a = ev(), b = ev();
+            RCHK(hipEventRecord(a, st));
+            hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p);
+            RCHK(hipGetLastError());
+            RCHK(hipEventRecord(b, st));
+            T.other.push_back({a, b});
+            a = ev(); b = ev();
+            RCHK(hipEventRecord(a, st));
+            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tun, scr, st));
+            RCHK(hipEventRecord(b, st));
+            T.closest.push_back({a, b});
+            a = ev(); b = ev();
+            RCHK(hipEventRecord(a, st));
+            RCHK(fill_u32(cnt + 5, 0u, 1, st));
+            hipLaunchKernelGGL(k_shade_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
+                               R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, batched ? 1 : 0);
+            RCHK(hipGetLastError());
+            RCHK(hipEventRecord(b, st));
+            T.other.push_back({a, b});
+            if (slots) {
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                if (batched)
+                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tun, scr, st, cnt + 5));
+                else
+                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tun, scr, st));
+                RCHK(hipEventRecord(b, st));
+                T.any.push_back({a, b});
+            }
+            a = ev(); b = ev();
+            RCHK(hipEventRecord(a, st));
+            hipLaunchKernelGGL(k_accum_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->lbase.p,
+                               R->contrib.p, R->qidx.p, R->occ.p, R->sample_l.p);
+            RCHK(hipGetLastError());
+            // running total of shadow rays
+            hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+            RCHK(hipGetLastError());
+            RCHK(hipEventRecord(b, st));
+            T.other.push_back({a, b});
+        } else {
+            ICHK(R->st.ensure(cap));
+            ICHK(R->rays2.ensure(cap));
+            ICHK(R->sh.ensure(cap));
+            ICHK(R->contrib.ensure(cap));
+            ICHK(R->occ.ensure((size_t)cap / 32 + 2));
+            ICHK(R->pid.ensure(cap));
+            ICHK(R->pid2.ensure(cap));
+            ICHK(R->shpid.ensure(cap));
+            hipEvent_t a = ev(), b = ev();
+            RCHK(hipEventRecord(a, st));
+            hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->st.p,
+                               batched ? R->pid.p : nullptr, batched ? cnt + 1 : nullptr);
+            RCHK(hipGetLastError());
+            RCHK(hipEventRecord(b, st));
+            T.other.push_back({a, b});
+            Ray* rin = R->rays.p;
+            Ray* rout = R->rays2.p;
+            uint32_t* pin = R->pid.p;
+            uint32_t* pout = R->pid2.p;
+            uint32_t* cin = cnt + 1;
+            uint32_t* cout = cnt + 2;
+            for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                if (batched)
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tun, scr, st, cin));
+                else
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tun, scr, st));
+                RCHK(hipEventRecord(b, st));
+                T.closest.push_back({a, b});
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                RCHK(fill_u32(cout, 0u, 1, st));
+                RCHK(fill_u32(cnt + 5, 0u, 1, st));
+                hipLaunchKernelGGL(k_path_shade, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, bounce, n,
+                                   batched ? cin : (const uint32_t*)nullptr, batched ? pin : (const uint32_t*)nullptr,
+                                   rin, R->hits.p, R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
+                                   batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p);
+                RCHK(hipGetLastError());
+                RCHK(hipEventRecord(b, st));
+                T.other.push_back({a, b});
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                if (batched)
+                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tun, scr, st, cnt + 5));
+                else
+                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tun, scr, st));
+                RCHK(hipEventRecord(b, st));
+                T.any.push_back({a, b});
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
+                                   batched ? cnt + 5 : (const uint32_t*)nullptr,
+                                   batched ? R->shpid.p : (const uint32_t*)nullptr, R->contrib.p, R->occ.p, R->st.p,
+                                   R->sample_l.p);
+                RCHK(hipGetLastError());
+                hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+                if (bounce < P.max_depth)
+                    hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
+                RCHK(hipGetLastError());
+                RCHK(hipEventRecord(b, st));
+                T.other.push_back({a, b});
+                std::swap(rin, rout);
+                std::swap(pin, pout);
+                std::swap(cin, cout);
+            }
+        }
+    }
+    ICHK(R->rgb.ensure((size_t)P.xres * P.yres * 3));
+    {
+        hipEvent_t a = ev(), b = ev();
+        RCHK(hipEventRecord(a, st));
+        hipLaunchKernelGGL(k_film, dim3(grid_for((size_t)P.xres * P.yres)), dim3(RB), 0, st, P, R->sample_l.p,
+                           R->rgb.p);
+        RCHK(hipGetLastError());
+        RCHK(hipEventRecord(b, st));
+        T.other.push_back({a, b});
+    }
+    hipEvent_t e_end = ev();
+    RCHK(hipEventRecord(e_end, st));
+    RCHK(hipEventSynchronize(e_end));
+    if (stats) {
+        float ms = 0.0f;
+        RCHK(hipEventElapsedTime(&ms, e_start, e_end));
+        stats->ms_total = ms;
+        auto sum = [&](const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double* out) -> int {
+            double t = 0.0;
+            for (auto& pr : v) {
+                float x = 0.0f;
+                RCHK(hipEventElapsedTime(&x, pr.first, pr.second));
+                t += x;
+            }
+            *out = t;
+            return PBRTX_OK;
+        };
+        ICHK(sum(T.closest, &stats->ms_trace_closest));
+        ICHK(sum(T.any, &stats->ms_trace_any));
+        ICHK(sum(T.other, &stats->ms_other));
+        stats->launches_closest = (uint32_t)T.closest.size();
+        stats->launches_any = (uint32_t)T.any.size();
+        uint32_t c[10];
+        RCHK(hipMemcpy(c, cnt, sizeof(c), hipMemcpyDeviceToHost));
+        stats->camera_rays = total_camera;
+        stats->shadow_rays = (uint64_t)c[6] | ((uint64_t)c[7] << 32);
+        stats->extension_rays = (uint64_t)c[8] | ((uint64_t)c[9] << 32);
+        stats->build_ms = R->build_ms;
+    }
+    return PBRTX_OK;
+}
+
+}  // namespace px
+
+using namespace px;
+
+struct pbrtx_renderer {
+    Renderer r;
+};
+
+extern "C" {
+
+int pbrtx_renderer_create(const pbrtx_render_desc* desc, pbrtx_renderer** out) {
+    if (!desc || !out) return fail(PBRTX_ERR_ARG, "null argument");
+    *out = nullptr;
+    pbrtx_renderer* h = new (std::nothrow) pbrtx_renderer();
+    if (!h) return fail(PBRTX_ERR_OOM, "host allocation");
+    int rc = renderer_setup(&h->r, desc);
+    if (rc) {
+        pbrtx_renderer_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return PBRTX_OK;
+}
+
+int pbrtx_renderer_set_chunk(pbrtx_renderer* h, uint32_t samples_per_chunk) {
+    if (!h || samples_per_chunk == 0) return fail(PBRTX_ERR_ARG, "bad chunk");
+    h->r.chunk = samples_per_chunk;
+    return PBRTX_OK;
+}
+
+int pbrtx_renderer_render(pbrtx_renderer* h, int32_t path, float* rgb_out, pbrtx_render_stats* stats) {
+    if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
+    if (path != PATH_PLAIN && path != PATH_BATCHED) return fail(PBRTX_ERR_ARG, "unknown path");
+    if (stats) memset(stats, 0, sizeof(*stats));
+    int rc = render_frame(&h->r, path, stats);
+    if (rc) return rc;
+    if (rgb_out) {
+        const size_t bytes = (size_t)h->r.P.xres * h->r.P.yres * 3 * sizeof(float);
+        hipError_t e = hipMemcpy(rgb_out, h->r.rgb.p, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(rgb)");
+    }
+    return PBRTX_OK;
+}
+
+int pbrtx_renderer_sample_radiance(pbrtx_renderer* h, float* out, uint64_t count) {
+    if (!h || !out) return fail(PBRTX_ERR_ARG, "null argument");
+    if (count > h->r.n_samples || !h->r.sample_l.p) return fail(PBRTX_ERR_ARG, "count exceeds the frame");
+    std::vector<float4> tmp(count);
+    hipError_t e = hipMemcpy(tmp.data(), h->r.sample_l.p, count * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(sample radiance)");
+    for (uint64_t i = 0; i < count; ++i) {
+        out[3 * i] = tmp[i].x; out[3 * i + 1] = tmp[i].y; out[3 * i + 2] = tmp[i].z;
+    }
+    return PBRTX_OK;
+}
+
+int pbrtx_renderer_destroy(pbrtx_renderer* h) {
+    if (!h) return PBRTX_OK;
+    Renderer& R = h->r;
+    (void)hipDeviceSynchronize();
+    scene_free(&R.scene);
+    R.faces.release(); R.mats.release(); R.atris.release(); R.a_first.release(); R.a_count.release();
+    R.a_total.release(); R.a_le.release(); R.points.release();
+    R.rays.release(); R.rays2.release(); R.sh.release(); R.hits.release(); R.contrib.release();
+    R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
+    R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
+    delete h;
+    return PBRTX_OK;
+}
+
+}  // extern "C"

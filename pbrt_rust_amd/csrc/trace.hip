@@ -237,11 +237,12 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_plain(SceneView sv, Ray* ra
 // PATH_BATCHED
 // ---------------------------------------------------------------------------
 template <bool ANY>
-__global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n,
-                                                          Hit* hits, uint32_t* occ,
-                                                          uint32_t* counter, uint2* spill,
-                                                          uint32_t spill_levels, uint32_t refill,
-                                                          uint32_t* err) {
+__global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
+                                                          const uint32_t* n_dev, Hit* hits,
+                                                          uint32_t* occ, uint32_t* counter,
+                                                          uint2* spill, uint32_t spill_levels,
+                                                          uint32_t refill, uint32_t* err) {
+    const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[STK_WIDE][MAX_BLOCK];
     __shared__ float s_tn[STK_WIDE][MAX_BLOCK];
     const uint32_t tid = threadIdx.x;
@@ -448,7 +449,8 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
 
 template <bool ANY>
 static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, uint32_t* occ, int path,
-                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
+                         const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     const int block = tun.block;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
@@ -458,6 +460,7 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     }
     uint32_t* err = scr->counter + 1;
     if (path == PATH_PLAIN) {
+        if (n_dev) return hipErrorInvalidValue;
         const int grid = plain_grid(n, block);
         const uint32_t lv = plain_levels(s);
         if ((size_t)lv * grid * block * sizeof(uint32_t) > scr->spill_bytes) return hipErrorInvalidValue;
@@ -471,18 +474,18 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    hipLaunchKernelGGL(k_trace_wide<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, hits, occ, scr->counter,
-                       (uint2*)scr->spill, lv, refill, err);
+    hipLaunchKernelGGL(k_trace_wide<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ,
+                       scr->counter, (uint2*)scr->spill, lv, refill, err);
     return hipGetLastError();
 }
 
 hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, int path,
-                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
-    return launch<false>(s, rays, n, hits, nullptr, path, tun, scr, st);
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
+    return launch<false>(s, rays, n, hits, nullptr, path, tun, scr, st, n_dev);
 }
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
-                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
-    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st);
+                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
+    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st, n_dev);
 }
 
 }  // namespace px

@@ -36,10 +36,15 @@ struct TraceScratch {
 // Launch helpers; return hipSuccess or the launch error.
 // closest: hits[i] for each ray, and rays[i].tmax = t on a hit.
 // any: occluded bit i of occ (words zeroed by the launcher).
+// n_dev (batched path only, may be null): the ray count is read from device
+// memory when the kernel starts (queues filled by an earlier kernel); n is then
+// the capacity (the number of occlusion words zeroed).
 hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, int path,
-                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st);
+                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
+                         const uint32_t* n_dev = nullptr);
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
-                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st);
+                     const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
+                     const uint32_t* n_dev = nullptr);
 // Number of spill bytes a launch needs for this scene and tuning.
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
