@@ -38,8 +38,21 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default=None, help="autotuner JSON to apply (pbrt_rust_amd.autotune)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    import pbrt_rust_amd as px
+    from pbrt_rust_amd._lib import check
+    a = px.DeviceBuffer(nbytes)
+    b = px.DeviceBuffer(nbytes)
+    check(px.lib().pbrtx_device_copy(b.ptr, a.ptr, nbytes, None))
+    px.synchronize()
+    print(json.dumps({"copy_bytes": nbytes}))
 
 
 def log(args, *a):
@@ -53,8 +66,13 @@ def main():
         print("bench.py: this build measures one GPU only (N=1); multi-GPU runs are not part of it.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
     import pbrt_rust_amd as px
-    from pbrt_rust_amd import scenes
+    from pbrt_rust_amd import autotune, scenes
+    if args.tuning:
+        autotune.load(args.tuning)
 
     t0 = time.time()
     scene = scenes.direct_lighting_scene(level=args.level)

@@ -95,9 +95,10 @@ int pbrtx_intersect(pbrtx_scene* scene, pbrtx_ray* d_rays, uint32_t n, pbrtx_hit
 int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, int32_t path, void* stream);
 
-/* Launch shape of a path: block in {64,128,256}; blocks_per_cu 0 = occupancy
-   maximum (batched path); refill_pct = idle-lane refill threshold. */
-int pbrtx_set_tuning(int32_t path, int32_t block, int32_t blocks_per_cu, int32_t refill_pct);
+/* Launch shape of a path's closest-hit (kind 0) or occlusion (kind 1)
+   kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
+   (batched path); refill_pct = idle-lane refill threshold (batched path). */
+int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct);
 
 /* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */
 /* Scene + camera + sampler + integrator description.  Triangles (world-space
@@ -153,6 +154,10 @@ int pbrtx_device_free(void* d_ptr);
 int pbrtx_memcpy_htod(void* d_dst, const void* src, uint64_t bytes);
 int pbrtx_memcpy_dtoh(void* dst, const void* d_src, uint64_t bytes);
 int pbrtx_synchronize(void);
+/* device-to-device copy by a plain streaming kernel (also the reference
+   point for calibrating memory-traffic counters: it moves exactly `bytes`
+   in and `bytes` out) */
+int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
 /* HIP-event timer on a stream: start, then stop returns elapsed ms */
 int pbrtx_timer_start(void* stream);
 int pbrtx_timer_stop(void* stream, float* ms);

@@ -289,8 +289,14 @@ class Aggregate:
         return BVHAccelerator(prims, max_prims, sm)
 
 
-def set_tuning(path, block=256, blocks_per_cu=0, refill_pct=50):
-    check(lib().pbrtx_set_tuning(_path_code(path), block, blocks_per_cu, refill_pct))
+KINDS = {"closest": 0, "any": 1}
+
+
+def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0):
+    """Launch shape of the closest-hit ("closest") or occlusion ("any")
+    kernel of a path."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    check(lib().pbrtx_set_tuning(_path_code(path), k, block, blocks_per_cu, refill_pct))
 
 
 # ---------------------------------------------------------------------------

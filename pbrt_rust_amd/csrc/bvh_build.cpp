@@ -353,47 +353,56 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
         r.w[11] = bits_f(kind | 0x100u);
     }
 
-    // child-pair layout
-    std::vector<int32_t> widx(nodes.size(), -1);
-    int32_t nw = 0;
-    for (size_t i = 0; i < nodes.size(); ++i)
-        if ((nodes[i].meta & 3u) != 3u) widx[i] = nw++;
-    out->wnodes.assign((size_t)nw, WNode{});
-    auto child_ref = [&](size_t c, uint32_t* count) -> int32_t {
-        if ((nodes[c].meta & 3u) == 3u) {
-            *count = nodes[c].meta >> 2;
-            return ~(int32_t)nodes[c].off;
-        }
-        *count = 0;
-        return widx[c];
+    // four-wide layout: records in depth-first pre-order of the record tree
+    auto is_leaf = [&](size_t i) { return (nodes[i].meta & 3u) == 3u; };
+    auto slots_of = [&](size_t a, size_t s[4]) {
+        const size_t b = a + 1, c = nodes[a].off;
+        const size_t none = (size_t)-1;
+        if (is_leaf(b)) { s[0] = b; s[1] = none; } else { s[0] = b + 1; s[1] = nodes[b].off; }
+        if (is_leaf(c)) { s[2] = c; s[3] = none; } else { s[2] = c + 1; s[3] = nodes[c].off; }
     };
-    for (size_t i = 0; i < nodes.size(); ++i) {
-        if (widx[i] < 0) continue;
-        WNode& w = out->wnodes[(size_t)widx[i]];
-        size_t c0 = i + 1, c1 = nodes[i].off;
-        uint32_t n0, n1;
-        w.ref0 = child_ref(c0, &n0);
-        w.ref1 = child_ref(c1, &n1);
-        if (n0 > 0x7FFFu || n1 > 0x7FFFu) {
-            *err = "leaf with more than 32767 primitives";
-            return false;
+    std::vector<int32_t> qidx(nodes.size(), -1);
+    std::vector<size_t> order_q;
+    uint32_t qdepth = 0;
+    if (!is_leaf(0)) {
+        std::vector<std::pair<size_t, uint32_t>> stk;
+        stk.push_back({0, 1});
+        while (!stk.empty()) {
+            auto [a, dq] = stk.back();
+            stk.pop_back();
+            qidx[a] = (int32_t)order_q.size();
+            order_q.push_back(a);
+            qdepth = std::max(qdepth, dq);
+            size_t s[4];
+            slots_of(a, s);
+            for (int k = 3; k >= 0; --k)
+                if (s[k] != (size_t)-1 && !is_leaf(s[k])) stk.push_back({s[k], dq + 1});
         }
-        w.meta = (nodes[i].meta & 3u) | (n0 << 2) | (n1 << 17);
-        w.pad = 0;
-        w.c0lx = nodes[c0].lo[0]; w.c0hx = nodes[c0].hi[0];
-        w.c0ly = nodes[c0].lo[1]; w.c0hy = nodes[c0].hi[1];
-        w.c0lz = nodes[c0].lo[2]; w.c0hz = nodes[c0].hi[2];
-        w.c1lx = nodes[c1].lo[0]; w.c1hx = nodes[c1].hi[0];
-        w.c1ly = nodes[c1].lo[1]; w.c1hy = nodes[c1].hi[1];
-        w.c1lz = nodes[c1].lo[2]; w.c1hz = nodes[c1].hi[2];
     }
-    if ((nodes[0].meta & 3u) == 3u) {
-        out->wroot = ~(int32_t)nodes[0].off;
-        out->wroot_count = nodes[0].meta >> 2;
-    } else {
-        out->wroot = 0;
-        out->wroot_count = 0;
+    out->qnodes.assign(order_q.size(), QNode{});
+    for (size_t qi = 0; qi < order_q.size(); ++qi) {
+        const size_t a = order_q[qi];
+        QNode& q = out->qnodes[qi];
+        size_t s[4];
+        slots_of(a, s);
+        const uint32_t ab = is_leaf(a + 1) ? 0u : (nodes[a + 1].meta & 3u);
+        const uint32_t ac = is_leaf(nodes[a].off) ? 0u : (nodes[nodes[a].off].meta & 3u);
+        q.meta = (nodes[a].meta & 3u) | (ab << 2) | (ac << 4);
+        for (int k = 0; k < 4; ++k) {
+            if (s[k] == (size_t)-1) {
+                q.ref[k] = QEMPTY;
+                q.lx[k] = q.ly[k] = q.lz[k] = FLT_MAX;
+                q.hx[k] = q.hy[k] = q.hz[k] = -FLT_MAX;
+                continue;
+            }
+            const RNode& c = nodes[s[k]];
+            q.ref[k] = is_leaf(s[k]) ? ~(int32_t)c.off : qidx[s[k]];
+            q.lx[k] = c.lo[0]; q.ly[k] = c.lo[1]; q.lz[k] = c.lo[2];
+            q.hx[k] = c.hi[0]; q.hy[k] = c.hi[1]; q.hz[k] = c.hi[2];
+        }
     }
+    out->qroot = is_leaf(0) ? ~(int32_t)nodes[0].off : 0;
+    out->qdepth = qdepth;
     auto t1 = std::chrono::steady_clock::now();
     out->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return true;

@@ -22,9 +22,9 @@ struct HostBvh {
     std::vector<uint32_t> order;    // BVH-ordered prim ids (input numbering)
     std::vector<PrimRec> prims;     // prim records in BVH order
     std::vector<SphereRec> spheres;
-    std::vector<WNode> wnodes;      // child-pair layout
-    int32_t wroot = 0;              // root ref of the child-pair layout
-    uint32_t wroot_count = 0;       // prim count when the root is a leaf
+    std::vector<QNode> qnodes;      // four-wide layout
+    int32_t qroot = 0;              // root ref of the four-wide layout
+    uint32_t qdepth = 0;            // record levels of the four-wide layout
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     uint32_t depth = 0;             // deepest node level (root = 0)
     uint32_t nfaces = 0, nspheres = 0;

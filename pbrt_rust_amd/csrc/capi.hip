@@ -30,7 +30,7 @@ int hip_fail(hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? PBRTX_ERR_OOM : PBRTX_ERR_HIP;
 }
 
-TraceTuning g_tuning[2] = {TraceTuning{256, 0, 50}, TraceTuning{256, 0, 50}};
+TraceTuning g_tuning[2][2] = {{TraceTuning{256, 0, 0}, TraceTuning{256, 0, 0}}, {TraceTuning{256, 0, 0}, TraceTuning{256, 0, 10}}};
 
 namespace {
 std::mutex g_scr_mu;
@@ -78,29 +78,30 @@ int scene_from_host(const HostBvh& h, DeviceScene* s) {
     memset(s, 0, sizeof(*s));
     int rc;
     if ((rc = upload(&s->d_rnodes, h.rnodes.data(), h.rnodes.size() * sizeof(RNode)))) return rc;
-    if ((rc = upload(&s->d_wnodes, h.wnodes.data(), h.wnodes.size() * sizeof(WNode)))) return rc;
+    if ((rc = upload(&s->d_qnodes, h.qnodes.data(), h.qnodes.size() * sizeof(QNode)))) return rc;
     if ((rc = upload(&s->d_prims, h.prims.data(), h.prims.size() * sizeof(PrimRec)))) return rc;
     if ((rc = upload(&s->d_spheres, h.spheres.data(), h.spheres.size() * sizeof(SphereRec)))) return rc;
-    s->bytes = h.rnodes.size() * sizeof(RNode) + h.wnodes.size() * sizeof(WNode) +
+    s->bytes = h.rnodes.size() * sizeof(RNode) + h.qnodes.size() * sizeof(QNode) +
                h.prims.size() * sizeof(PrimRec) + h.spheres.size() * sizeof(SphereRec);
     SceneView& v = s->view;
     v.rnodes = (const RNode*)s->d_rnodes;
-    v.wnodes = (const WNode*)s->d_wnodes;
+    v.qnodes = (const QNode*)s->d_qnodes;
     v.prims = (const PrimRec*)s->d_prims;
     v.spheres = (const SphereRec*)s->d_spheres;
     v.n_rnodes = (uint32_t)h.rnodes.size();
-    v.n_wnodes = (uint32_t)h.wnodes.size();
+    v.n_qnodes = (uint32_t)h.qnodes.size();
     v.n_prims = (uint32_t)h.prims.size();
     v.n_spheres = (uint32_t)h.spheres.size();
-    v.wroot = h.wroot;
+    v.qroot = h.qroot;
     v.depth = h.depth;
+    v.qdepth = h.qdepth;
     for (int k = 0; k < 3; ++k) { v.root_lo[k] = h.root_lo[k]; v.root_hi[k] = h.root_hi[k]; }
     return PBRTX_OK;
 }
 
 void scene_free(DeviceScene* s) {
     if (s->d_rnodes) (void)hipFree(s->d_rnodes);
-    if (s->d_wnodes) (void)hipFree(s->d_wnodes);
+    if (s->d_qnodes) (void)hipFree(s->d_qnodes);
     if (s->d_prims) (void)hipFree(s->d_prims);
     if (s->d_spheres) (void)hipFree(s->d_spheres);
     memset(s, 0, sizeof(*s));
@@ -108,7 +109,7 @@ void scene_free(DeviceScene* s) {
 
 int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, hipStream_t st) {
     if (path != PATH_PLAIN && path != PATH_BATCHED) return fail(PBRTX_ERR_ARG, "unknown path");
-    const TraceTuning& tun = g_tuning[path];
+    const TraceTuning& tun = g_tuning[path][0];
     TraceScratch* scr;
     int rc = stream_scratch(st, trace_spill_bytes(v, path, tun), &scr);
     if (rc) return rc;
@@ -119,7 +120,7 @@ int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, 
 
 int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int path, hipStream_t st) {
     if (path != PATH_PLAIN && path != PATH_BATCHED) return fail(PBRTX_ERR_ARG, "unknown path");
-    const TraceTuning& tun = g_tuning[path];
+    const TraceTuning& tun = g_tuning[path][1];
     TraceScratch* scr;
     int rc = stream_scratch(st, trace_spill_bytes(v, path, tun), &scr);
     if (rc) return rc;
@@ -238,12 +239,13 @@ int pbrtx_intersect_p(pbrtx_scene* s, const pbrtx_ray* d_rays, uint32_t n, uint3
     return run_any(s->s.view, reinterpret_cast<const Ray*>(d_rays), n, d_occ, path, (hipStream_t)stream);
 }
 
-int pbrtx_set_tuning(int32_t path, int32_t block, int32_t blocks_per_cu, int32_t refill_pct) {
+int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct) {
     if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");
+    if (kind != 0 && kind != 1) return fail(PBRTX_ERR_ARG, "kind must be 0 (closest hit) or 1 (occlusion)");
     if (block != 64 && block != 128 && block != 256) return fail(PBRTX_ERR_ARG, "block must be 64, 128 or 256");
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
     if (refill_pct < 0 || refill_pct > 100) return fail(PBRTX_ERR_ARG, "refill_pct out of range");
-    g_tuning[path] = TraceTuning{block, blocks_per_cu, refill_pct};
+    g_tuning[path][kind] = TraceTuning{block, blocks_per_cu, refill_pct};
     return PBRTX_OK;
 }
 
@@ -281,6 +283,13 @@ int pbrtx_synchronize(void) {
         if (e != hipSuccess) return hip_fail(e, "hipMemcpy flag");
         if (flag) return fail(PBRTX_ERR_STATE, "traversal stack deeper than the scene's recorded depth");
     }
+    return PBRTX_OK;
+}
+
+int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
+    if (!d_dst || !d_src || (bytes % 16) != 0) return fail(PBRTX_ERR_ARG, "copy needs device pointers and a multiple of 16 bytes");
+    hipError_t e = copy_bytes(d_dst, d_src, bytes, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "copy");
     return PBRTX_OK;
 }
 

@@ -11,7 +11,7 @@ namespace px {
 
 struct DeviceScene {
     SceneView view;
-    void *d_rnodes, *d_wnodes, *d_prims, *d_spheres;
+    void *d_rnodes, *d_qnodes, *d_prims, *d_spheres;
     uint64_t bytes;
 };
 
@@ -22,6 +22,6 @@ int scene_from_host(const HostBvh& h, DeviceScene* s);
 void scene_free(DeviceScene* s);
 int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, hipStream_t st);
 int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int path, hipStream_t st);
-extern TraceTuning g_tuning[2];
+extern TraceTuning g_tuning[2][2];  // [path][0 closest hit, 1 occlusion]
 
 }  // namespace px

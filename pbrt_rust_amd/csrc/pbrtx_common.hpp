@@ -55,19 +55,23 @@ struct RNode {
 };
 static_assert(sizeof(RNode) == 32, "RNode must be 32 bytes");
 
-// Child-pair layout: one 64-byte record per interior reference node holding
-// both children's boxes, so a node fetch tests two boxes.  child refs: >= 0 an
-// interior record, < 0 a leaf ~first_prim_position.  meta: axis in bits 0..1,
-// child-0 leaf count in bits 2..16, child-1 leaf count in bits 17..31.
-// Child 0 is the reference's first child (node+1), child 1 its second child.
-struct WNode {
-    float c0lx, c0hx, c0ly, c0hy;
-    float c0lz, c0hz, c1lx, c1hx;
-    float c1ly, c1hy, c1lz, c1hz;
-    int32_t ref0, ref1;
-    uint32_t meta, pad;
+// Four-wide layout: one 128-byte record (one cache line) per two levels of the
+// reference tree.  For an interior node A with children B (= A+1) and C
+// (= A's second child), the slots are [B0, B1, C0, C1]: the children of B and
+// of C; a leaf B (or C) takes the first slot of its half and leaves the
+// second EMPTY.  Boxes are stored per axis so four boxes load with six 16-byte
+// loads.  refs: >= 0 another record, QEMPTY no child, otherwise ~first
+// position of a leaf's prims in the ordered prim array.  meta: split axis of
+// A in bits 0-1, of B in bits 2-3, of C in bits 4-5 -- the three axis signs
+// that fix the reference's near-first visiting order of the four slots.
+struct QNode {
+    float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
+    int32_t ref[4];
+    uint32_t meta;
+    uint32_t pad[3];
 };
-static_assert(sizeof(WNode) == 64, "WNode must be 64 bytes");
+static_assert(sizeof(QNode) == 128, "QNode must be 128 bytes");
+static const int32_t QEMPTY = (int32_t)0x80000000;
 
 // Primitive record in BVH leaf order, 48 bytes: triangle p1, e1 = p2-p1,
 // e2 = p3-p1 (mesh.rs:45-46, precomputed on the host: the same floats) with

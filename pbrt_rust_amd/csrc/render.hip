@@ -197,12 +197,77 @@ __global__ __launch_bounds__(RB) void k_camera(RenderParams P, uint64_t g0, uint
 // Direct lighting shade (deviation D1 of the build: emitted light plus, per
 // light, the mean over its samples of the unoccluded contributions).
 // Slots per sample: point lights first, then light_samples per area light.
+// Exclusive prefix sum of v over the RB threads of a block (wave shuffles,
+// then one LDS round for the four wave totals).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[RB / 64];
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t w = 0; w < RB / 64; ++w) {
+        const uint32_t s = wsum[w];
+        if (w < wid) before += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+// Offsets of per-block counts: one block of SCAN_T threads, each summing a
+// contiguous run, then a block-wide scan of the run sums.  offsets[b] =
+// sum of counts[0..b); *total = sum of all.
+constexpr int SCAN_T = 1024;
+__global__ __launch_bounds__(SCAN_T) void k_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets,
+                                                        uint32_t* total) {
+    __shared__ uint32_t wsum[SCAN_T / 64];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wid = t / 64;
+    const uint32_t per = (nb + SCAN_T - 1) / SCAN_T;
+    const uint32_t lo = t * per, hi = (lo + per < nb) ? lo + per : nb;
+    uint32_t s = 0;
+    for (uint32_t b = lo; b < hi; ++b) s += counts[b];
+    uint32_t x = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t w = 0; w < SCAN_T / 64; ++w) {
+        const uint32_t v = wsum[w];
+        if (w < wid) before += v;
+        tot += v;
+    }
+    uint32_t run = before + x - s;
+    for (uint32_t b = lo; b < hi; ++b) {
+        offsets[b] = run;
+        run += counts[b];
+    }
+    if (t == 0) *total = tot;
+}
+
+// Shade modes: SHADE_PLAIN writes every slot in place (dead slots carry an
+// empty interval) and counts live rays with one atomic per block;
+// SHADE_COUNT only counts live rays per block; SHADE_WRITE writes the live
+// rays densely at the scanned block offsets, in sample order.  The count and
+// write passes evaluate the same pure function of (hit, sample id).
+enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_COUNT = 1, SHADE_WRITE = 2 };
+
 __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
                                                      const Hit* hits, Ray* sh, float4* contrib,
                                                      uint32_t* qidx, float4* lbase, uint32_t* sh_count,
-                                                     int compact) {
+                                                     uint32_t* block_counts, const uint32_t* block_offsets,
+                                                     int mode) {
     const uint32_t S = P.S;
-    for (uint32_t base = blockIdx.x * RB; base < n; base += gridDim.x * RB) {
+    {
+        const uint32_t base = blockIdx.x * RB;
         const uint32_t i = base + threadIdx.x;
         const bool live = i < n;
         Hit h = live ? hits[i] : Hit{NO_HIT, 0, 0, 0};
@@ -221,39 +286,67 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
             mat = P.mats[fs.mat];
             f3 le = mk3(0, 0, 0);
             if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
-            lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
+            if (mode != SHADE_COUNT) lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
             uint64_t g = g0 + i;
             pixel = (uint32_t)(g / P.spp); s = (uint32_t)(g % P.spp);
-        } else if (live) {
+        } else if (live && mode != SHADE_COUNT) {
             lbase[i] = make_float4(0, 0, 0, 0);
         }
+        auto slot_sample = [&](uint32_t k, Ray* sr, f3* c) -> bool {
+            if (!hit) return false;
+            uint32_t li, j;
+            if (k < P.n_point) { li = k; j = 0; }
+            else { li = P.n_point + (k - P.n_point) / P.light_samples; j = (k - P.n_point) % P.light_samples; }
+            float u[4] = {0, 0, 0, 0};
+            if (li >= P.n_point) rng4(P.seed, pixel, s, 1 + (li - P.n_point) * P.light_samples + j, u);
+            return sample_light(P, li, p, eps, nn, wo, mat, u[0], u[1], u[2], sr, c);
+        };
+        if (mode == SHADE_COUNT) {
+            uint32_t cnt = 0;
+            for (uint32_t k = 0; k < S; ++k) {
+                Ray sr;
+                f3 c;
+                cnt += slot_sample(k, &sr, &c) ? 1u : 0u;
+            }
+            uint32_t tot;
+            block_exclusive_scan(cnt, &tot);
+            if (threadIdx.x == 0) block_counts[blockIdx.x] = tot;
+            return;
+        }
+        uint32_t cnt = 0, pos = 0;
+        if (mode == SHADE_WRITE) {
+            for (uint32_t k = 0; k < S; ++k) {
+                Ray sr;
+                f3 c;
+                cnt += slot_sample(k, &sr, &c) ? 1u : 0u;
+            }
+            uint32_t tot;
+            pos = block_offsets[blockIdx.x] + block_exclusive_scan(cnt, &tot);
+        }
+        uint32_t live_rays = 0;
         for (uint32_t k = 0; k < S; ++k) {
-            bool valid = false;
             Ray sr;
             f3 c = mk3(0, 0, 0);
-            if (hit) {
-                uint32_t li, j;
-                if (k < P.n_point) { li = k; j = 0; }
-                else { li = P.n_point + (k - P.n_point) / P.light_samples; j = (k - P.n_point) % P.light_samples; }
-                float u[4] = {0, 0, 0, 0};
-                if (li >= P.n_point) rng4(P.seed, pixel, s, 1 + (li - P.n_point) * P.light_samples + j, u);
-                valid = sample_light(P, li, p, eps, nn, wo, mat, u[0], u[1], u[2], &sr, &c);
-            }
+            const bool valid = slot_sample(k, &sr, &c);
             const size_t slot = (size_t)i * S + k;
-            if (compact) {
-                const uint32_t q = wave_alloc(valid, sh_count);
-                if (valid) sh[q] = sr;
-                if (live) qidx[slot] = q;
+            if (mode == SHADE_WRITE) {
+                if (valid) sh[pos] = sr;
+                if (live) qidx[slot] = valid ? pos : NONE;
+                if (valid) ++pos;
             } else {
                 if (!valid) set_dead(&sr);
                 if (live) {
                     sh[slot] = sr;
                     qidx[slot] = valid ? (uint32_t)slot : NONE;
                 }
-                const unsigned long long m = __ballot(valid);
-                if (lane_id() == 0 && m) atomicAdd(sh_count, (uint32_t)__popcll(m));
+                live_rays += valid ? 1u : 0u;
             }
             if (live) contrib[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+        if (mode == SHADE_PLAIN) {
+            uint32_t tot;
+            block_exclusive_scan(live_rays, &tot);
+            if (threadIdx.x == 0 && tot) atomicAdd(sh_count, tot);
         }
     }
 }
@@ -316,10 +409,15 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                                                    const Ray* rays_in, const Hit* hits, PathState* st,
                                                    Ray* rays_out, uint32_t* pid_out, uint32_t* out_count,
                                                    Ray* sh, uint32_t* sh_pid, float4* shc,
-                                                   uint32_t* sh_count, float4* sample_l) {
+                                                   uint32_t* sh_count, float4* sample_l,
+                                                   uint32_t* bc, const uint32_t* bo, uint32_t nblocks,
+                                                   int mode) {
+    // bc / bo: per-block counts / offsets, shadow rays at [0, nblocks), next
+    // rays at [nblocks, 2*nblocks)
     const uint32_t n = q_count ? *q_count : cap;
     const bool queued = pid_q != nullptr;
-    for (uint32_t base = blockIdx.x * RB; base < n; base += gridDim.x * RB) {
+    {
+        const uint32_t base = blockIdx.x * RB;
         const uint32_t i = base + threadIdx.x;
         const bool live = i < n;
         uint32_t pid = live ? (queued ? pid_q[i] : i) : 0u;
@@ -413,6 +511,13 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
             want_next = cont;
             finish = !cont;
         }
+        if (mode == SHADE_COUNT) {
+            uint32_t ts, tn;
+            block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
+            block_exclusive_scan(want_next ? 1u : 0u, &tn);
+            if (threadIdx.x == 0) { bc[blockIdx.x] = ts; bc[nblocks + blockIdx.x] = tn; }
+            return;
+        }
         // paths that end with no shadow ray pending are final now
         if (live && active) {
             if (finish && !want_shadow) {
@@ -423,10 +528,12 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
             }
             st[pid] = ps;
         }
-        if (queued) {
-            const uint32_t qs = wave_alloc(want_shadow, sh_count);
+        uint32_t ts, tn;
+        const uint32_t rs = block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
+        const uint32_t rn = block_exclusive_scan(want_next ? 1u : 0u, &tn);
+        if (mode == SHADE_WRITE) {
+            const uint32_t qs = bo[blockIdx.x] + rs, qn = bo[nblocks + blockIdx.x] + rn;
             if (want_shadow) { sh[qs] = sray; sh_pid[qs] = pid; shc[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f); }
-            const uint32_t qn = wave_alloc(want_next, out_count);
             if (want_next) { rays_out[qn] = nray; pid_out[qn] = pid; }
         } else {
             if (live) {
@@ -436,10 +543,9 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                 shc[i] = make_float4(contrib.x, contrib.y, contrib.z, want_shadow ? 1.0f : 0.0f);
                 rays_out[i] = nray;
             }
-            const unsigned long long ms = __ballot(want_shadow), mn = __ballot(want_next);
-            if (lane_id() == 0) {
-                if (ms) atomicAdd(sh_count, (uint32_t)__popcll(ms));
-                if (mn) atomicAdd(out_count, (uint32_t)__popcll(mn));
+            if (threadIdx.x == 0) {
+                if (ts) atomicAdd(sh_count, ts);
+                if (tn) atomicAdd(out_count, tn);
             }
         }
     }
@@ -703,7 +809,7 @@ struct Renderer {
     DevArr<Ray> rays, rays2, sh;
     DevArr<Hit> hits;
     DevArr<float4> contrib, lbase, sample_l;
-    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters;
+    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters, bcounts, boffsets;
     DevArr<PathState> st;
     DevArr<float> rgb;
     hipStream_t stream = nullptr;
@@ -826,9 +932,14 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     ICHK(R->hits.ensure(cap));
     ICHK(R->counters.ensure(16));
     const bool batched = path == PATH_BATCHED;
-    const TraceTuning& tun = g_tuning[path];
+    const TraceTuning& tunC = g_tuning[path][0];
+    const TraceTuning& tunA = g_tuning[path][1];
     TraceScratch* scr;
-    ICHK(stream_scratch(st, trace_spill_bytes(R->scene.view, path, tun), &scr));
+    {
+        const size_t a = trace_spill_bytes(R->scene.view, path, tunC);
+        const size_t b = trace_spill_bytes(R->scene.view, path, tunA);
+        ICHK(stream_scratch(st, a > b ? a : b, &scr));
+    }
     R->events.used = 0;
     Timing T;
     auto ev = [&]() { return R->events.next(); };
@@ -856,24 +967,39 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             T.other.push_back({a, b});
             a = ev(); b = ev();
             RCHK(hipEventRecord(a, st));
-            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tun, scr, st));
+            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunC, scr, st));
             RCHK(hipEventRecord(b, st));
             T.closest.push_back({a, b});
             a = ev(); b = ev();
             RCHK(hipEventRecord(a, st));
             RCHK(fill_u32(cnt + 5, 0u, 1, st));
-            hipLaunchKernelGGL(k_shade_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
-                               R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, batched ? 1 : 0);
-            RCHK(hipGetLastError());
+            {
+                const uint32_t nb = (n + RB - 1) / RB;
+                ICHK(R->bcounts.ensure(2 * (size_t)nb));
+                ICHK(R->boffsets.ensure(2 * (size_t)nb));
+                if (batched) {
+                    hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
+                                       R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
+                                       (const uint32_t*)R->boffsets.p, (int)SHADE_COUNT);
+                    RCHK(hipGetLastError());
+                    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p, nb, R->boffsets.p,
+                                       cnt + 5);
+                    RCHK(hipGetLastError());
+                }
+                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
+                                   R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
+                                   (const uint32_t*)R->boffsets.p, (int)(batched ? SHADE_WRITE : SHADE_PLAIN));
+                RCHK(hipGetLastError());
+            }
             RCHK(hipEventRecord(b, st));
             T.other.push_back({a, b});
             if (slots) {
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
                 if (batched)
-                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tun, scr, st, cnt + 5));
+                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st, cnt + 5));
                 else
-                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tun, scr, st));
+                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st));
                 RCHK(hipEventRecord(b, st));
                 T.any.push_back({a, b});
             }
@@ -913,28 +1039,48 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
                 if (batched)
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tun, scr, st, cin));
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tunC, scr, st, cin));
                 else
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tun, scr, st));
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tunC, scr, st));
                 RCHK(hipEventRecord(b, st));
                 T.closest.push_back({a, b});
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
                 RCHK(fill_u32(cout, 0u, 1, st));
                 RCHK(fill_u32(cnt + 5, 0u, 1, st));
-                hipLaunchKernelGGL(k_path_shade, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, bounce, n,
-                                   batched ? cin : (const uint32_t*)nullptr, batched ? pin : (const uint32_t*)nullptr,
-                                   rin, R->hits.p, R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
-                                   batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p);
-                RCHK(hipGetLastError());
+                {
+                    const uint32_t nb = (n + RB - 1) / RB;
+                    ICHK(R->bcounts.ensure(2 * (size_t)nb));
+                    ICHK(R->boffsets.ensure(2 * (size_t)nb));
+                    if (batched) {
+                        hipLaunchKernelGGL(k_path_shade, dim3(nb), dim3(RB), 0, st, P, g0, bounce, n,
+                                           (const uint32_t*)cin, (const uint32_t*)pin, rin, R->hits.p, R->st.p, rout,
+                                           pout, cout, R->sh.p, R->shpid.p, R->contrib.p, cnt + 5, R->sample_l.p,
+                                           R->bcounts.p, (const uint32_t*)R->boffsets.p, nb, (int)SHADE_COUNT);
+                        RCHK(hipGetLastError());
+                        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p, nb,
+                                           R->boffsets.p, cnt + 5);
+                        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p + nb, nb,
+                                           R->boffsets.p + nb, cout);
+                        RCHK(hipGetLastError());
+                    }
+                    hipLaunchKernelGGL(k_path_shade, dim3(nb), dim3(RB), 0, st, P, g0, bounce, n,
+                                       batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
+                                       batched ? (const uint32_t*)pin : (const uint32_t*)nullptr, rin, R->hits.p,
+                                       R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
+                                       batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p,
+                                       R->bcounts.p, (const uint32_t*)R->boffsets.p, nb,
+                                       (int)(batched ? SHADE_WRITE : SHADE_PLAIN));
+                    RCHK(hipGetLastError());
+                }
                 RCHK(hipEventRecord(b, st));
                 T.other.push_back({a, b});
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
                 if (batched)
-                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tun, scr, st, cnt + 5));
+                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st, cnt + 5));
                 else
-                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tun, scr, st));
+                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st));
                 RCHK(hipEventRecord(b, st));
                 T.any.push_back({a, b});
                 a = ev(); b = ev();
@@ -1064,6 +1210,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.rays.release(); R.rays2.release(); R.sh.release(); R.hits.release(); R.contrib.release();
     R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
     R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
+    R.bcounts.release(); R.boffsets.release();
     delete h;
     return PBRTX_OK;
 }

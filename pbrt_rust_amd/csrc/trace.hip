@@ -11,10 +11,16 @@
 //              record, box test at pop time (exactly the reference loop), a
 //              per-lane LDS stack of node indices with a global spill.
 // PATH_BATCHED persistent waves pulling rays from a global counter and
-//              refilling idle lanes (wave ballot + prefix rank), 64-byte
-//              child-pair nodes whose two boxes are tested together; a child
-//              is pushed with its entry distance and re-checked against the
-//              current tmax when popped, which is the reference's pop-time test.
+//              refilling idle lanes (wave ballot + prefix rank), 128-byte
+//              four-wide records (two reference levels per fetch).  The four
+//              grandchild boxes are tested when their record is fetched; a
+//              passing slot is pushed with its entry distance in the
+//              reference's visiting order and re-checked against the current
+//              tmax when popped -- the reference's pop-time test.  Skipping
+//              the middle level's box test changes nothing: a box contains
+//              its children's boxes, and the rounded slab distances are
+//              monotone in the box bounds, so whenever a grandchild passes,
+//              its parent passes too (at the same or a larger tmax).
 #include "trace.hpp"
 
 namespace px {
@@ -236,6 +242,16 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_plain(SceneView sv, Ray* ra
 // ---------------------------------------------------------------------------
 // PATH_BATCHED
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t sel4i(int i, int32_t a, int32_t b, int32_t c, int32_t d) {
+    return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+__device__ __forceinline__ float sel4f(int i, float a, float b, float c, float d) {
+    return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+__device__ __forceinline__ bool sel4b(int i, bool a, bool b, bool c, bool d) {
+    return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
                                                           const uint32_t* n_dev, Hit* hits,
@@ -244,21 +260,36 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                                                           uint32_t refill, uint32_t* err) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[STK_WIDE][MAX_BLOCK];
-    __shared__ float s_tn[STK_WIDE][MAX_BLOCK];
+    __shared__ float s_tn[ANY ? 1 : STK_WIDE][MAX_BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + tid;
     const uint32_t total = gridDim.x * blockDim.x;
     const uint32_t lane = __lane_id();
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int32_t POP = (int32_t)0x80000000;  // "take the next stack entry"
 
     RayState r{};
     bool neg0 = false, neg1 = false, neg2 = false;
     uint32_t rid = 0, hprim = NO_HIT;
     float hb1 = 0.0f, hb2 = 0.0f;
-    int32_t cur = POP;
+    int32_t cur = 0;
+    bool have_cur = false;  // false: take the next stack entry
     int sp = 0;
     bool has = false, drained = false;
+    // An entry pushed at record time is re-checked against tmax when popped
+    // (for occlusion tmax never shrinks before the exit, so that check and the
+    // stored distance are not needed there).
+    auto push = [&](int32_t ref, float tn) {
+        if (sp < STK_WIDE) {
+            s_ref[sp][tid] = ref;
+            if (!ANY) s_tn[sp][tid] = tn;
+            ++sp;
+        } else if ((uint32_t)(sp - STK_WIDE) < spill_levels) {
+            spill[(size_t)(sp - STK_WIDE) * total + gtid] = make_uint2((uint32_t)ref, as_u(tn));
+            ++sp;
+        } else {
+            atomicOr(err, 1u);  // deeper than the scene's recorded depth: never expected
+        }
+    };
 
     while (true) {
         // ---- refill idle lanes with the next rays (one atomic per wave)
@@ -277,11 +308,10 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                     sp = 0;
                     float tn;
                     // the root's own box (popped first by the reference)
-                    if (slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2],
-                             sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &tn))
-                        cur = sv.wroot;
-                    else
-                        cur = POP;
+                    have_cur = slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1],
+                                    sv.root_lo[2], sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz,
+                                    r.tmin, r.tmax, &tn);
+                    cur = sv.qroot;
                     has = true;
                 } else {
                     drained = true;
@@ -291,62 +321,71 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
         if (!__any(has)) break;
         // ---- traverse until too few lanes are busy
         while (has) {
-            if (cur == POP) {
-                bool found = false;
+            if (!have_cur) {
                 while (sp > 0) {
                     --sp;
                     int32_t ref;
-                    float tn;
-                    if (sp < STK_WIDE) { ref = s_ref[sp][tid]; tn = s_tn[sp][tid]; }
-                    else {
+                    float tn = 0.0f;
+                    if (sp < STK_WIDE) {
+                        ref = s_ref[sp][tid];
+                        if (!ANY) tn = s_tn[sp][tid];
+                    } else {
                         const uint2 e = spill[(size_t)(sp - STK_WIDE) * total + gtid];
                         ref = (int32_t)e.x; tn = as_f(e.y);
                     }
-                    if (tn <= r.tmax) { cur = ref; found = true; break; }
+                    if (ANY || tn <= r.tmax) { cur = ref; have_cur = true; break; }
                 }
-                if (!found) {
+                if (!have_cur) {
                     // ray finished
-                    if (ANY) {
-                        // no hit: bit stays 0
-                    } else {
+                    if (!ANY) {
                         hits[rid] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
                         if (hprim != NO_HIT) rays[rid].tmax = r.tmax;
                     }
                     has = false;
                 }
-            } else if (cur >= 0) {
-                const float4* q = reinterpret_cast<const float4*>(sv.wnodes + cur);
-                const float4 qa = q[0], qb = q[1], qc = q[2];
-                const int4 qd = reinterpret_cast<const int4*>(sv.wnodes + cur)[3];
-                float tn0, tn1;
-                const bool p0 = slab(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, r.ox, r.oy, r.oz, r.ix, r.iy,
-                                     r.iz, r.tmin, r.tmax, &tn0);
-                const bool p1 = slab(qb.z, qb.w, qc.x, qc.y, qc.z, qc.w, r.ox, r.oy, r.oz, r.ix, r.iy,
-                                     r.iz, r.tmin, r.tmax, &tn1);
-                const uint32_t axis = (uint32_t)qd.z & 3u;
-                const bool ng = axis == 0 ? neg0 : (axis == 1 ? neg1 : neg2);
-                // near child = second child when the direction is negative (bvh.rs:409-415)
-                const int32_t nref = ng ? qd.y : qd.x, fref = ng ? qd.x : qd.y;
-                const bool np = ng ? p1 : p0, fp = ng ? p0 : p1;
-                const float ftn = ng ? tn0 : tn1;
-                if (np && fp) {
-                    if (sp < STK_WIDE) {
-                        s_ref[sp][tid] = fref; s_tn[sp][tid] = ftn; ++sp;
-                    } else if ((uint32_t)(sp - STK_WIDE) < spill_levels) {
-                        spill[(size_t)(sp - STK_WIDE) * total + gtid] = make_uint2((uint32_t)fref, as_u(ftn));
-                        ++sp;
-                    } else {
-                        atomicOr(err, 1u);  // deeper than the scene's recorded depth: never expected
-                    }
-                    cur = nref;
-                } else if (np) {
-                    cur = nref;
-                } else if (fp) {
-                    cur = fref;
-                } else {
-                    cur = POP;
-                }
-            } else {
+            }
+            if (has && cur >= 0) {
+                const float4* q = reinterpret_cast<const float4*>(sv.qnodes + cur);
+                const float4 LX = q[0], HX = q[1], LY = q[2], HY = q[3], LZ = q[4], HZ = q[5];
+                const int4 RF = reinterpret_cast<const int4*>(q)[6];
+                const uint32_t meta = reinterpret_cast<const uint32_t*>(q)[28];
+                float t0, t1, t2, t3;
+                const bool p0 = RF.x != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
+                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t0);
+                const bool p1 = RF.y != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
+                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t1);
+                const bool p2 = RF.z != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
+                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t2);
+                const bool p3 = RF.w != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
+                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t3);
+                // visiting order of the reference (bvh.rs:409-415 applied at A,
+                // then at B and C): near half first, near child first in each
+                const uint32_t aA = meta & 3u, aB = (meta >> 2) & 3u, aC = (meta >> 4) & 3u;
+                const bool nA = aA == 0 ? neg0 : (aA == 1 ? neg1 : neg2);
+                const bool nB = aB == 0 ? neg0 : (aB == 1 ? neg1 : neg2);
+                const bool nC = aC == 0 ? neg0 : (aC == 1 ? neg1 : neg2);
+                const int b0 = nB ? 1 : 0, b1 = nB ? 0 : 1, c0 = nC ? 3 : 2, c1 = nC ? 2 : 3;
+                const int o0 = nA ? c0 : b0, o1 = nA ? c1 : b1, o2 = nA ? b0 : c0, o3 = nA ? b1 : c1;
+                // the first passing slot is taken now, the others pushed so
+                // that they pop in order
+                bool pend = false;
+                int32_t pref = 0;
+                float ptn = 0.0f;
+#define PX_VISIT(O)                                                                    \
+    if (sel4b(O, p0, p1, p2, p3)) {                                                     \
+        if (pend) push(pref, ptn);                                                      \
+        pend = true;                                                                    \
+        pref = sel4i(O, RF.x, RF.y, RF.z, RF.w);                                        \
+        ptn = sel4f(O, t0, t1, t2, t3);                                                 \
+    }
+                PX_VISIT(o3)
+                PX_VISIT(o2)
+                PX_VISIT(o1)
+                PX_VISIT(o0)
+#undef PX_VISIT
+                have_cur = pend;
+                cur = pref;
+            } else if (has) {
                 // leaf: prims in order until the record flagged last
                 uint32_t pos = (uint32_t)(~cur);
                 bool hitnow = false;
@@ -359,7 +398,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                     if (last) break;
                     ++pos;
                 }
-                cur = POP;
+                have_cur = false;
                 if (ANY && hitnow) {
                     atomicOr(occ + (rid >> 5), 1u << (rid & 31u));
                     has = false;
@@ -376,6 +415,12 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_copy16(uint4* dst, const uint4* src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 int g_cu_count = 0;
@@ -416,6 +461,16 @@ int device_cu_count() {
     return g_cu_count;
 }
 
+hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    const size_t cap = (size_t)device_cu_count() * 8;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n);
+    return hipGetLastError();
+}
+
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;
@@ -430,7 +485,8 @@ static uint32_t plain_levels(const SceneView& s) {
     return need > (uint32_t)STK_PLAIN ? need - (uint32_t)STK_PLAIN : 0u;
 }
 static uint32_t wide_levels(const SceneView& s) {
-    uint32_t need = s.depth + 1;
+    // at most three pending slots per record level on the current path
+    uint32_t need = 3 * s.qdepth + 1;
     return need > (uint32_t)STK_WIDE ? need - (uint32_t)STK_WIDE : 0u;
 }
 

@@ -9,12 +9,13 @@ namespace px {
 // Device-resident scene (all pointers in HBM).
 struct SceneView {
     const RNode* rnodes;
-    const WNode* wnodes;
+    const QNode* qnodes;
     const PrimRec* prims;
     const SphereRec* spheres;
-    uint32_t n_rnodes, n_wnodes, n_prims, n_spheres;
-    int32_t wroot;
-    uint32_t depth;
+    uint32_t n_rnodes, n_qnodes, n_prims, n_spheres;
+    int32_t qroot;
+    uint32_t depth;   // levels of the reference tree
+    uint32_t qdepth;  // levels of the four-wide records
     float root_lo[3], root_hi[3];
 };
 
@@ -48,6 +49,8 @@ hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* 
 // Number of spill bytes a launch needs for this scene and tuning.
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
+// streaming 16-byte-vector copy (bytes a multiple of 16)
+hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st);
 int device_cu_count();
 
 }  // namespace px
