@@ -24,7 +24,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trace_any_direct1024_traffic.json")
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32
+# HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# workload, scaled by a device-copy calibration (written next to its summary)
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_direct1024.json")
+KERNEL_ANY = "k_trace_wide<true>"
+KERNEL_CLOSEST = "k_trace_wide<false>"
 
 
 def parse():
@@ -36,8 +41,8 @@ def parse():
     ap.add_argument("--spp-side", type=int, default=4)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="bounded CPU baseline sample (also the source of the per-ray work figures)")
     ap.add_argument("--tuning", default=None, help="autotuner JSON to apply (pbrt_rust_amd.autotune)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device copy of this many bytes (counter calibration) and exit")
@@ -107,31 +112,37 @@ def main():
     mrays_p = rays / (ms_p * 1e-3) / 1e6
     log(args, "batched %.3f ms/frame (wall %.3f)  plain %.3f ms/frame" % (ms_b, wall_b * 1e3, ms_p))
 
-    # ---- roofline of the dominant kernel (per-launch averages, HIP events) ----
+    # ---- CPU baseline (bounded sample of the same frame); it also yields the
+    # reference traversal's per-ray work used for the algorithmic figures ----
+    cpu, work = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
+    log(args, "cpu baseline %s" % json.dumps(cpu))
+
+    # ---- roofline of the dominant kernel (per-launch averages, HIP events on
+    # the render stream) ----
     any_ms = float(np.mean([s["ms_trace_any"] / max(1, s["launches_any"]) for s in sts_b]))
     cl_ms = float(np.mean([s["ms_trace_closest"] / max(1, s["launches_closest"]) for s in sts_b]))
     dominant = "trace_any" if any_ms >= cl_ms else "trace_closest"
-    # algorithmic bytes per ray from the instrumented oracle traversal on a
-    # sample of the same rays: 32 B per node visited + 36 B per triangle
-    # tested + 32 B ray in + 16 B hit out (4 B bit for occlusion)
-    alg = algorithmic_bytes(scene, cam, nx, ny, seed, r, px)
+    # algorithmic bytes per ray: 32 B per node popped + 36 B per triangle
+    # tested + 32 B ray in + 16 B hit out (1 bit for occlusion); flops per
+    # ray ~ 15 per node + 50 per triangle
     if dominant == "trace_any":
-        per_launch_bytes = alg["any_bytes_per_ray"] * st["shadow_rays"]
+        nodes, tris, out_b, n_rays = work["any_nodes"], work["any_tris"], 1 / 8.0, st["shadow_rays"]
         kernel_ms = any_ms
+        kname = KERNEL_ANY
     else:
-        per_launch_bytes = alg["closest_bytes_per_ray"] * st["camera_rays"]
+        nodes, tris, out_b, n_rays = work["closest_nodes"], work["closest_tris"], 16.0, st["camera_rays"]
         kernel_ms = cl_ms
+        kname = KERNEL_CLOSEST
+    bytes_per_ray = 32 * nodes + 36 * tris + 32 + out_b
+    per_launch_bytes = bytes_per_ray * n_rays
     achieved = per_launch_bytes / (kernel_ms * 1e-3) / 1e9
+    flops_launch = (15 * nodes + 50 * tris) * n_rays
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         try:
-            traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
-        except Exception:
+            traffic = json.load(open(PROFILE_TRAFFIC))["kernels"][kname]["hbm_bytes"]
+        except (KeyError, ValueError):
             traffic = None
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
 
     out = {
         "metric": "Mrays/s batched vs plain GPU kernel; p50/p99 request latency at fixed rate",
@@ -167,10 +178,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "alg_bytes_per_launch": int(per_launch_bytes),
+            "traffic": None if traffic is None else int(traffic),
             "kernel_ms_per_launch": round(kernel_ms, 4),
-            "nodes_per_ray": alg["nodes_per_ray"],
+            "alg_bytes_per_ray": round(bytes_per_ray, 2),
+            "rays_per_launch": int(n_rays),
+            "measured_hbm_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
+            "valu": {"achieved": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+            "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "cpu_baseline": cpu,
         "device": px.device_name(),
@@ -179,52 +194,12 @@ def main():
     r.close()
 
 
-def algorithmic_bytes(scene, cam, nx, ny, seed, renderer, px, n_pixels=4096):
-    """Mean reference-traversal work per ray on a sample of the frame's rays,
-    counted by the oracle (tests/ helpers are test infrastructure; the oracle
-    here is only the counter, never what is measured)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    L = ol.lib()
-    o = ol.OracleScene()
-    o.add_mesh(scene["verts"], scene["idx"])
-    o.build("sah", 1)
-    rng = np.random.default_rng(0)
-    xres, yres = cam["xres"], cam["yres"]
-    pix = rng.choice(xres * yres, n_pixels, replace=False)
-    r2c = np.zeros(16, np.float32)
-    L.orc_camera_raster_to_camera(xres, yres, ol.f32(cam["screen"]), cam["fov"], r2c)
-    xy = np.zeros((n_pixels, 2), np.float32)
-    buf = np.zeros(2 * nx * ny, np.float32)
-    for k, p in enumerate(pix):
-        L.orc_pixel_samples(seed, xres, int(p % xres), int(p // xres), nx, ny, 1, buf)
-        xy[k] = buf[:2]
-    oo = np.zeros(3 * n_pixels, np.float32)
-    dd = np.zeros(3 * n_pixels, np.float32)
-    L.orc_camera_rays(r2c, ol.f32(cam["cam_to_world"]).reshape(16), n_pixels, xy.reshape(-1), oo, dd)
-    res = o.intersect(oo, dd, 0.0, np.finfo(np.float32).max, nthreads=os.cpu_count() or 1)
-    n_cl, t_cl = res["stats"][0] / n_pixels, res["stats"][1] / n_pixels
-    # shadow segments toward the light's centre from every hit
-    hit = res["prim"] != 0xFFFFFFFF
-    t = res["t"][hit]
-    p = (oo.reshape(-1, 3)[hit] + dd.reshape(-1, 3)[hit] * t[:, None]).astype(np.float32)
-    light_c = scene["verts"][scene["idx"][scene["face_light"] >= 0].reshape(-1)].mean(0).astype(np.float32)
-    w = light_c - p
-    dist = np.sqrt((w * w).sum(1)).astype(np.float32)
-    dirn = (w / dist[:, None]).astype(np.float32)
-    occ, st = o.intersect_p(p, dirn, (t * np.float32(5e-4)).astype(np.float32), (dist * np.float32(0.999)),
-                            nthreads=os.cpu_count() or 1)
-    n_sh = max(1, p.shape[0])
-    n_an, t_an = st[0] / n_sh, st[1] / n_sh
-    return {"closest_bytes_per_ray": 32 * n_cl + 36 * t_cl + 32 + 16,
-            "any_bytes_per_ray": 32 * n_an + 36 * t_an + 32 + 4 / 32.0,
-            "nodes_per_ray": {"closest": round(float(n_cl), 2), "any": round(float(n_an), 2),
-                              "tris_closest": round(float(t_cl), 2), "tris_any": round(float(t_an), 2)}}
-
-
 def cpu_baseline(scene, cam, nx, ny, seed, seconds):
     """The oracle's render (CPU restatement of the same path) timed on the
-    host cores over a bounded band of image rows."""
+    host cores over a bounded band of image rows of the same frame.  Also
+    returns the reference traversal's work per ray on that band (nodes popped
+    and prims tested; occlusion queries counted up to their first hit), the
+    per-unit figures the roofline uses."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     cores = os.cpu_count() or 1
@@ -238,12 +213,15 @@ def cpu_baseline(scene, cam, nx, ny, seed, seconds):
     per_row = max(1e-4, (first - build_s) / 2)
     rows = int(max(2, min(cam["yres"] - y0, (seconds - build_s) / per_row)))
     res = ol.render(scene, cam, integ, y0=y0, y1=y0 + rows, nthreads=cores)
-    st = res["stats"]
+    st = [int(v) for v in res["stats"]]
     trace_s = st[4] * 1e-9
-    rays = int(st[0] + st[1])
-    return {"value": round(rays / trace_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": "oracle render of rows %d..%d of the same frame (%d camera + %d shadow rays), "
-                      "BVH build excluded" % (y0, y0 + rows, int(st[0]), int(st[1]))}
+    rays = st[0] + st[1]
+    cpu = {"value": round(rays / trace_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+           "sample": "oracle render of rows %d..%d of the same frame (%d camera + %d shadow rays, "
+                     "shading included), BVH build excluded" % (y0, y0 + rows, st[0], st[1])}
+    work = {"closest_nodes": st[5] / max(1, st[0]), "closest_tris": st[6] / max(1, st[0]),
+            "any_nodes": st[7] / max(1, st[1]), "any_tris": st[8] / max(1, st[1])}
+    return cpu, work
 
 
 if __name__ == "__main__":

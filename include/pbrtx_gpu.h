@@ -98,7 +98,8 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
 /* Launch shape of a path's closest-hit (kind 0) or occlusion (kind 1)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
    (batched path); refill_pct = idle-lane refill threshold (batched path). */
-int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct);
+int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
+                     int32_t scheme);
 
 /* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */
 /* Scene + camera + sampler + integrator description.  Triangles (world-space

@@ -654,10 +654,13 @@ static bool prim_intersect(const Scene& sc, const Prim& p, Ray& r, float* t, flo
     return true;
 }
 
-// BVHAccelerator::intersect: bvh.rs:377-421
+// BVHAccelerator::intersect: bvh.rs:377-421.  nvis / ntest count the nodes
+// popped and prims tested; first (may be NULL) receives the same two counts
+// up to the first hit, i.e. the work of an occlusion query that stops there.
 static bool bvh_intersect(const Scene& sc, Ray& r, uint32_t* prim, float* t, float* b1,
-                          float* b2, uint64_t* nvis, uint64_t* ntest) {
+                          float* b2, uint64_t* nvis, uint64_t* ntest, uint64_t* first = nullptr) {
     if (sc.nodes.empty()) return false;
+    const uint64_t v0 = *nvis, t0 = *ntest;
     V3 inv = mk(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     bool neg[3] = {inv.x < 0.0f, inv.y < 0.0f, inv.z < 0.0f};
     // the reference's Vec with capacity 64; the stack never exceeds depth + 1
@@ -677,6 +680,7 @@ static bool bvh_intersect(const Scene& sc, Ray& r, uint32_t* prim, float* t, flo
                 ++*ntest;
                 float tt, bb1, bb2;
                 if (prim_intersect(sc, sc.prims[pi], r, &tt, &bb1, &bb2)) {
+                    if (!hit && first) { first[0] = *nvis - v0; first[1] = *ntest - t0; }
                     hit = true; *prim = pi; *t = tt; *b1 = bb1; *b2 = bb2;
                 }
             }
@@ -910,7 +914,11 @@ static bool trace_closest(const Scene& sc, Ray& r, uint32_t* prim, float* t, flo
 static bool trace_any(const Scene& sc, const Ray& r0, uint64_t* st) {
     Ray r = r0;
     uint32_t p; float t, b1, b2;
-    return trace_closest(sc, r, &p, &t, &b1, &b2, st);
+    uint64_t nv = 0, nt = 0, first[2] = {0, 0};
+    bool h = bvh_intersect(sc, r, &p, &t, &b1, &b2, &nv, &nt, first);
+    st[5] += h ? first[0] : nv;
+    st[6] += h ? first[1] : nt;
+    return h;
 }
 
 struct HitGeom {
@@ -1513,10 +1521,10 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
                 bool in = !(b < (int32_t)y0 || a >= (int32_t)y1);
                 need[((size_t)(yy - ya) * xres + x) * spp + s] = in ? 1 : 0;
             }
-    std::atomic<uint64_t> st_cam{0}, st_sh{0}, st_ext{0};
+    std::atomic<uint64_t> st_cam{0}, st_sh{0}, st_ext{0}, st_cn{0}, st_ct{0}, st_an{0}, st_at{0};
     uint32_t nwork = rows * xres;
     parallel_for(nwork, std::max(1u, nthreads), [&](uint32_t lo, uint32_t hi) {
-        uint64_t st[5] = {0, 0, 0, 0, 0};
+        uint64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
         for (uint32_t w = lo; w < hi; ++w) {
             uint32_t yy = ya + w / xres, x = w % xres;
             uint32_t pixel = yy * xres + x;
@@ -1531,6 +1539,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
             }
         }
         st_cam += st[2]; st_sh += st[3]; st_ext += st[4];
+        st_cn += st[0]; st_ct += st[1]; st_an += st[5]; st_at += st[6];
     });
     // film: gather per pixel
     for (uint32_t y = y0; y < y1; ++y)
@@ -1581,6 +1590,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
         stats[0] = st_cam.load(); stats[1] = st_sh.load(); stats[2] = st_ext.load();
         stats[3] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(tr - tb).count();
         stats[4] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(te - tr).count();
+        stats[5] = st_cn.load(); stats[6] = st_ct.load(); stats[7] = st_an.load(); stats[8] = st_at.load();
     }
     return 0;
 }

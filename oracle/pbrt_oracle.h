@@ -143,8 +143,11 @@ typedef struct orc_render_desc {
    intended semantics (sampler_renderer.rs:61-144,184-206; deviations D1-D7).
    rgb_out: xres*(y1-y0)*3 floats (final film rgb of those rows).
    sample_l (may be NULL): per (pixel,sample) radiance, 3 floats each.
-   stats: [0] camera rays, [1] shadow rays, [2] extension rays, [3] build ms x1000,
-   [4] trace+shade ns.  nthreads >= 1. */
+   stats (9 entries): [0] camera rays, [1] shadow rays, [2] extension rays,
+   [3] build ms x1000, [4] trace+shade ns, [5] nodes visited and [6] prims
+   tested by closest-hit queries (camera and extension rays), [7] nodes
+   visited and [8] prims tested by occlusion queries up to their first hit
+   (the work of an early-exit any-hit query).  nthreads >= 1. */
 int orc_render(const orc_render_desc* desc, uint32_t y0, uint32_t y1, uint32_t nthreads,
                float* rgb_out, float* sample_l, uint64_t* stats);
 

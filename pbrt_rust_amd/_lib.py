@@ -78,7 +78,7 @@ def lib():
     L.pbrtx_scene_device_bytes.argtypes = [vp, c.POINTER(u64)]
     L.pbrtx_intersect.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_intersect_p.argtypes = [vp, vp, u32, vp, i32, vp]
-    L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32]
+    L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
     L.pbrtx_device_copy.argtypes = [vp, vp, u64, vp]
     L.pbrtx_device_malloc.argtypes = [c.POINTER(vp), u64]
     L.pbrtx_device_free.argtypes = [vp]

@@ -292,11 +292,16 @@ class Aggregate:
 KINDS = {"closest": 0, "any": 1}
 
 
-def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0):
+SCHEMES = {"ray": 0, "packet": 1}
+
+
+def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray"):
     """Launch shape of the closest-hit ("closest") or occlusion ("any")
-    kernel of a path."""
+    kernel of a path, and its traversal scheme: "ray" (a stack per ray) or
+    "packet" (a stack per wave of 64 consecutive rays; batched occlusion)."""
     k = KINDS[kind] if isinstance(kind, str) else int(kind)
-    check(lib().pbrtx_set_tuning(_path_code(path), k, block, blocks_per_cu, refill_pct))
+    sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+    check(lib().pbrtx_set_tuning(_path_code(path), k, block, blocks_per_cu, refill_pct, sc))
 
 
 # ---------------------------------------------------------------------------

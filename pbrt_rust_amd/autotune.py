@@ -1,10 +1,11 @@
 """Autotuner for the batched traversal kernels' launch shape.
 
 For the closest-hit and the occlusion kernel separately it sweeps threads per
-block, persistent blocks per CU and the idle-lane refill threshold on a real
-frame of a renderer, keeps the fastest configuration of each (timed with HIP
-events on the render stream), and stores both as JSON so later runs (bench,
-serving) start from them:
+block, persistent blocks per CU, the idle-lane refill threshold and (for
+occlusion) the traversal scheme -- a stack per ray or one per wave of 64
+consecutive rays -- on a real frame of a renderer, keeps the fastest
+configuration of each (timed with HIP events on the render stream), and
+stores both as JSON so later runs (bench, serving) start from them:
 
     python -m pbrt_rust_amd.autotune --out tuning.json
 """
@@ -22,6 +23,8 @@ DEFAULT_SPACE = {
     "blocks_per_cu": [0, 2, 4, 8],
     "refill_pct": [0, 10, 25, 40, 60],
 }
+# the packet scheme has no refill threshold
+PACKET_SPACE = {"block": [64, 128, 256], "blocks_per_cu": [0, 2, 4, 8, 16]}
 STAGE = {"closest": "ms_trace_closest", "any": "ms_trace_any"}
 
 
@@ -31,21 +34,26 @@ def sweep(renderer, space=None, repeats=2, log=None):
     space = space or DEFAULT_SPACE
     out = {}
     for kind in ("closest", "any"):
+        cands = [(b, c, f, "ray") for b, c, f in
+                 itertools.product(space["block"], space["blocks_per_cu"], space["refill_pct"])]
+        if kind == "any":
+            cands += [(b, c, 0, "packet") for b, c in
+                      itertools.product(PACKET_SPACE["block"], PACKET_SPACE["blocks_per_cu"])]
         results = []
-        for block, bpc, refill in itertools.product(space["block"], space["blocks_per_cu"], space["refill_pct"]):
+        for block, bpc, refill, scheme in cands:
             try:
-                api.set_tuning("batched", kind, block, bpc, refill)
+                api.set_tuning("batched", kind, block, bpc, refill, scheme)
             except Exception as e:
-                results.append(dict(block=block, blocks_per_cu=bpc, refill_pct=refill, error=str(e)))
+                results.append(dict(block=block, blocks_per_cu=bpc, refill_pct=refill, scheme=scheme, error=str(e)))
                 continue
             renderer.render("batched", want_image=False)
             ms = min(renderer.render("batched", want_image=False)[1][STAGE[kind]] for _ in range(repeats))
-            r = dict(kind=kind, block=block, blocks_per_cu=bpc, refill_pct=refill, ms=ms)
+            r = dict(kind=kind, block=block, blocks_per_cu=bpc, refill_pct=refill, scheme=scheme, ms=ms)
             results.append(r)
             if log:
                 log(r)
         best = min((r for r in results if "error" not in r), key=lambda r: r["ms"])
-        api.set_tuning("batched", kind, best["block"], best["blocks_per_cu"], best["refill_pct"])
+        apply({kind: best})
         out[kind] = (results, best)
     return out
 
@@ -54,7 +62,7 @@ def apply(cfg):
     for kind in ("closest", "any"):
         b = cfg.get(kind)
         if b:
-            api.set_tuning("batched", kind, b["block"], b["blocks_per_cu"], b["refill_pct"])
+            api.set_tuning("batched", kind, b["block"], b["blocks_per_cu"], b["refill_pct"], b.get("scheme", "ray"))
 
 
 def load(path):

@@ -323,25 +323,28 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
             uint32_t tot;
             pos = block_offsets[blockIdx.x] + block_exclusive_scan(cnt, &tot);
         }
+        // The queue index of slot k of sample i is stored light-sample major
+        // (at k*n + i) so that a wave's stores are contiguous; the
+        // contribution of a traced ray sits at its queue position.  Rays stay
+        // sample major in both forms (the light samples of one pixel are
+        // neighbours in the queue, which keeps waves coherent).
         uint32_t live_rays = 0;
         for (uint32_t k = 0; k < S; ++k) {
             Ray sr;
             f3 c = mk3(0, 0, 0);
             const bool valid = slot_sample(k, &sr, &c);
-            const size_t slot = (size_t)i * S + k;
+            uint32_t q = NONE;
             if (mode == SHADE_WRITE) {
-                if (valid) sh[pos] = sr;
-                if (live) qidx[slot] = valid ? pos : NONE;
-                if (valid) ++pos;
+                if (valid) { q = pos++; sh[q] = sr; }
             } else {
+                const size_t slot = (size_t)i * S + k;
                 if (!valid) set_dead(&sr);
-                if (live) {
-                    sh[slot] = sr;
-                    qidx[slot] = valid ? (uint32_t)slot : NONE;
-                }
+                if (live) sh[slot] = sr;
+                if (valid) q = (uint32_t)slot;
                 live_rays += valid ? 1u : 0u;
             }
-            if (live) contrib[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+            if (valid) contrib[q] = make_float4(c.x, c.y, c.z, 0.0f);
+            if (live) qidx[(size_t)k * n + i] = q;
         }
         if (mode == SHADE_PLAIN) {
             uint32_t tot;
@@ -362,11 +365,10 @@ __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0
             const uint32_t ns = li < P.n_point ? 1u : P.light_samples;
             f3 Ld = mk3(0, 0, 0);
             for (uint32_t j = 0; j < ns; ++j, ++k) {
-                const size_t slot = (size_t)i * P.S + k;
-                const uint32_t q = qidx[slot];
+                const uint32_t q = qidx[(size_t)k * n + i];
                 if (q == NONE) continue;
                 if ((occ[q >> 5] >> (q & 31u)) & 1u) continue;
-                const float4 c = contrib[slot];
+                const float4 c = contrib[q];
                 Ld = add3(Ld, mk3(c.x, c.y, c.z));
             }
             L = add3(L, divs3(Ld, (float)ns));

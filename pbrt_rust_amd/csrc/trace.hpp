@@ -21,10 +21,16 @@ struct SceneView {
 
 enum TracePath : int { PATH_PLAIN = 0, PATH_BATCHED = 1 };
 
+// Traversal scheme of the batched path: one stack per ray, or one stack per
+// wave of 64 consecutive rays (occlusion only; used while the scene's stack
+// bound fits the per-wave stack, else the per-ray kernel runs).
+enum TraceScheme : int { SCHEME_RAY = 0, SCHEME_PACKET = 1 };
+
 struct TraceTuning {
     int block = 256;          // threads per block
     int blocks_per_cu = 0;    // persistent grid: blocks per CU (0 = occupancy maximum)
     int refill_pct = 50;      // refill when fewer than this % of a wave's lanes trace
+    int scheme = SCHEME_RAY;
 };
 
 // Scratch owned by the caller (per stream): spill stacks and a work counter.

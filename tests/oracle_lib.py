@@ -224,7 +224,7 @@ def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False):
     if y1 is None:
         y1 = cam["yres"]
     out = np.zeros((y1 - y0) * cam["xres"] * 3, np.float32)
-    st = np.zeros(5, np.uint64)
+    st = np.zeros(9, np.uint64)
     spp = integ["nx"] * integ["ny"]
     samples = np.zeros((y1 - y0) * cam["xres"] * spp * 3, np.float32) if want_samples else None
     r = L.orc_render(ctypes.byref(d), y0, y1, nthreads, out,

@@ -119,6 +119,26 @@ def test_edge_cases(path):
 
 
 @pytest.mark.parametrize("path", PATHS)
+def test_mixed_finite_and_infinite_inverse_directions(path):
+    # Rays with finite and with infinite 1/d components (NaN slab bounds)
+    # interleaved in one wave must match the oracle.
+    v, idx = scenes.random_triangles(2000, 8)
+    bvh, orc = mesh_pair(v, idx)
+    rng = np.random.default_rng(77)
+    n = 4096
+    o, d = random_rays(n, 78)
+    k = rng.integers(0, 3, n)
+    sel = rng.random(n) < 0.3
+    # subnormal components: 1/d overflows to +-inf
+    d[sel, k[sel]] = rng.choice([1e-40, -1e-40, 0.0, -0.0, 1e-45], int(sel.sum())).astype(F32)
+    o[::5] = np.round(o[::5], 1)
+    check_closest(bvh, orc, o, d, path=path)
+    check_any(bvh, orc, o, d, path=path)
+    # NaN tmin is ignored by the slab max (Rust f32::max) on both sides
+    check_any(bvh, orc, o, d, mint=float("nan"), maxt=2.0, path=path)
+
+
+@pytest.mark.parametrize("path", PATHS)
 def test_deep_tree_uses_spill_stack(path):
     # triangles in the planes x = 16^k: every SAH split peels off the farthest
     # one (all other centroids share bucket 0), giving a 29-level chain that is
@@ -177,3 +197,37 @@ def test_paths_agree_at_full_size():
     occ = bvh.intersect_p(px.RayBatch(o, d), "batched")
     assert np.array_equal(occ, ha["prim"] != px.NO_HIT)
     assert np.array_equal(bvh.intersect_p(px.RayBatch(o, d), "plain"), occ)
+
+
+@pytest.fixture
+def packet_occlusion():
+    px.set_tuning("batched", "any", 256, 0, 0, "packet")
+    yield
+    px.set_tuning("batched", "any", 256, 0, 10, "ray")
+
+
+def test_packet_occlusion_scheme_matches_oracle(packet_occlusion):
+    # one stack per wave of 64 rays: incoherent rays, the point-light shadow
+    # segments of the ray-query workload, the 8-sphere fixture, ragged counts
+    v, idx = scenes.random_triangles(3000, 11)
+    bvh, orc = mesh_pair(v, idx)
+    for n in (1, 63, 64, 65, 20000):
+        o, d = random_rays(n, 300 + n)
+        check_any(bvh, orc, o, d, path="batched")
+        check_any(bvh, orc, o, d, mint=1e-3, maxt=0.5, path="batched")
+    s = scenes.ray_query_scene(16384)
+    bvh, orc = mesh_pair(s["verts"], s["idx"], "sah")
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-1, 1, (50000, 3)).astype(F32)
+    lp = np.array([2, 2, -2], F32)
+    dist = np.sqrt(np.sum((p - lp) ** 2, axis=1)).astype(F32)
+    dirn = ((lp - p) * (F32(1) / dist)[:, None]).astype(F32)
+    occ = check_any(bvh, orc, p, dirn, F32(1e-4), dist, path="batched")
+    assert 0.05 < occ.mean() < 0.95
+    g = GOLD["spheres8"]
+    prims = []
+    for c in g["centers"]:
+        t = px.Transform.translate(c)
+        prims.append(px.Primitive.simple(px.Sphere(t, t.inverse(), False, 1.0, -1.0, 1.0, 360.0)))
+    assert px.Aggregate.bvh(prims, 1, "sah").intersect_p(
+        px.RayBatch([r["o"] for r in g["rays"]], [r["d"] for r in g["rays"]]), "batched").all()
