@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="bounded CPU baseline sample (also the source of the per-ray work figures)")
     ap.add_argument("--tuning", default=None, help="autotuner JSON to apply (pbrt_rust_amd.autotune)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--serve-pool", type=int, default=48, help="distinct requests in the serving pool")
+    ap.add_argument("--serve-requests", type=int, default=600, help="requests in the fixed-rate run")
+    ap.add_argument("--serve-batch", type=int, default=1 << 22, help="rays per serving step")
+    ap.add_argument("--serve-arena-gb", type=int, default=8)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--verbose", action="store_true")
@@ -116,6 +121,9 @@ def main():
     # reference traversal's per-ray work used for the algorithmic figures ----
     cpu, work = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
     log(args, "cpu baseline %s" % json.dumps(cpu))
+    serve = None
+    if not args.no_serve:
+        serve = serving(args, px, scenes, lambda m: log(args, m))
 
     # ---- roofline of the dominant kernel (per-launch averages, HIP events on
     # the render stream) ----
@@ -187,11 +195,126 @@ def main():
                      "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
+        "serving": serve,
         "cpu_baseline": cpu,
         "device": px.device_name(),
     }
     print(json.dumps(out), flush=True)
     r.close()
+
+
+def serving(args, px, scenes, log_fn):
+    """Ray-query serving: 8 resident scenes of random triangles (10K-2M,
+    log-uniform), requests of 1K-4M rays (log-uniform), 70% closest hit / 30%
+    shadow segments, origins uniform in the scene box and directions uniform
+    on the sphere.  Saturated throughput with every request queued at once,
+    then p50/p99 latency under Poisson arrivals at 70% of the batched
+    server's saturated rate, for the continuous-batching server and for the
+    one-plain-kernel-per-request server at the same rate."""
+    import threading
+    rng = np.random.default_rng(0x5E4E)
+    counts = np.exp(rng.uniform(np.log(1e4), np.log(2e6), 8)).astype(np.int64)
+    bvhs = []
+    for k, c in enumerate(counts):
+        v, idx = scenes.random_triangles(int(c), 0xC3 + k)
+        bvhs.append(px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, idx, v))], 4, "sah"))
+    # request pool in page-locked memory
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(4e6), args.serve_pool)).astype(np.int64)
+    pool = []
+    for j, n in enumerate(sizes):
+        s = int(rng.integers(0, len(bvhs)))
+        kind = "closest" if rng.random() < 0.7 else "any"
+        lo, hi = bvhs[s].world_bound()
+        r = px.PinnedArray((int(n),), px.RAY_DTYPE)
+        a = r.array
+        o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+        d = rng.normal(size=(n, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        a["ox"], a["oy"], a["oz"] = o[:, 0], o[:, 1], o[:, 2]
+        a["dx"], a["dy"], a["dz"] = d[:, 0], d[:, 1], d[:, 2]
+        a["tmin"] = 0.0
+        diag = float(np.linalg.norm(hi - lo))
+        a["tmax"] = np.finfo(np.float32).max if kind == "closest" else \
+            rng.uniform(0.0, 0.5 * diag, n).astype(np.float32)
+        out = px.PinnedArray((int(n),), px.HIT_DTYPE) if kind == "closest" else \
+            px.PinnedArray((int((n + 31) // 32),), np.uint32)
+        pool.append((s, kind, r, out))
+    mean_rays = float(np.mean(sizes))
+
+    def make(mode):
+        srv = px.RayServer(arena_bytes=args.serve_arena_gb << 30, batch_rays=args.serve_batch, mode=mode)
+        for b in bvhs:
+            srv.add_scene(b)
+        for s in range(len(bvhs)):  # first launch per scene (scratch sizing) outside the clock
+            w = pool[0][2].array[:64].copy()
+            srv.wait(srv.submit(s, "closest", w))
+        return srv
+
+    def saturated(mode):
+        srv = make(mode)
+        t = time.perf_counter()
+        tick = [srv.submit(s, kind, r.array, out.array) for s, kind, r, out in pool]
+        for tk in tick:
+            srv.wait(tk)
+        dt = time.perf_counter() - t
+        srv.close()
+        return float(sizes.sum()) / dt / 1e6
+
+    def at_rate(mode, req_per_s, nreq):
+        srv = make(mode)
+        arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / req_per_s, nreq))
+        tickets = [None] * nreq
+        ready = threading.Semaphore(0)
+
+        def producer():
+            t0 = time.perf_counter()
+            for i in range(nreq):
+                while True:
+                    lag = arr[i] - (time.perf_counter() - t0)
+                    if lag <= 0:
+                        break
+                    if lag > 2e-3:
+                        time.sleep(lag - 1e-3)
+                s, kind, r, out = pool[i % len(pool)]
+                tickets[i] = srv.submit(s, kind, r.array, out.array)
+                ready.release()
+
+        th = threading.Thread(target=producer)
+        t = time.perf_counter()
+        th.start()
+        lat = []
+        for i in range(nreq):
+            ready.acquire()
+            _, st = srv.wait(tickets[i])
+            lat.append(st["latency_ms"])
+        th.join()
+        dt = time.perf_counter() - t
+        rays = sum(int(sizes[i % len(pool)]) for i in range(nreq))
+        srv.close()
+        lat = np.array(lat)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3), "achieved_mrays_per_s": round(rays / dt / 1e6, 2)}
+
+    sat_b = saturated("batched")
+    sat_p = saturated("plain")
+    log_fn("serving saturated: batched %.1f Mrays/s, plain %.1f Mrays/s" % (sat_b, sat_p))
+    rate = 0.7 * sat_b
+    req_per_s = rate * 1e6 / mean_rays
+    nreq = args.serve_requests
+    lat_b = at_rate("batched", req_per_s, nreq)
+    lat_p = at_rate("plain", req_per_s, nreq)
+    log_fn("serving at %.1f Mrays/s: batched %s plain %s" % (rate, lat_b, lat_p))
+    return {
+        "workload": "8 resident scenes of random triangles (%d tris total), %d-request pool of 1K-4M rays "
+                    "(log-uniform), 70%% closest / 30%% shadow, random origins and directions, host page-locked "
+                    "buffers (latency includes both copies)" % (int(counts.sum()), len(pool)),
+        "saturated_mrays_per_s": {"batched": round(sat_b, 2), "plain": round(sat_p, 2)},
+        "served_over_plain": round(sat_b / sat_p, 3),
+        "rate_mrays_per_s": round(rate, 2),
+        "rate_requests_per_s": round(req_per_s, 2),
+        "requests": nreq,
+        "latency": {"batched": lat_b, "plain": lat_p},
+    }
 
 
 def cpu_baseline(scene, cam, nx, ny, seed, seconds):

@@ -27,7 +27,7 @@ extern "C" {
 
 /* traversal paths */
 #define PBRTX_PATH_PLAIN 0   /* one lane per ray, reference node layout */
-#define PBRTX_PATH_BATCHED 1 /* persistent waves, child-pair nodes */
+#define PBRTX_PATH_BATCHED 1 /* persistent waves, four-wide records */
 
 /* Ray record (32 B): Ray{o, d, mint, maxt} of src/ray.rs:7-61 without the
    time / depth fields the queries do not read. */
@@ -97,7 +97,9 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
 
 /* Launch shape of a path's closest-hit (kind 0) or occlusion (kind 1)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
-   (batched path); refill_pct = idle-lane refill threshold (batched path). */
+   (batched path); refill_pct = idle-lane refill threshold (batched path);
+   scheme 0 = a traversal stack per ray, 1 = one stack per wave of 64
+   consecutive rays (batched occlusion only).  Process-wide setting. */
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme);
 
@@ -149,6 +151,56 @@ int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx
 int pbrtx_renderer_sample_radiance(pbrtx_renderer* r, float* out, uint64_t count);
 int pbrtx_renderer_destroy(pbrtx_renderer* r);
 
+/* ---- request server ------------------------------------------------------ */
+/* A server of ray-query requests (Intersectable::intersect / intersect_p,
+   intersection.rs:61-66, over BVHAccelerator scenes) against many
+   device-resident scenes.  The reference has no server: this is the serving
+   loop around the same queries.  One worker thread owns the GPU work.
+   PBRTX_SERVE_BATCHED: requests are admitted in arrival order while their
+   device buffers (32 B per ray + results) fit arena_bytes; each step traces
+   up to batch_rays rays taken from every admitted request, one launch per
+   (scene, kind), so requests that arrive while others run join the next step
+   and large requests progress in chunks.  PBRTX_SERVE_PLAIN: one request at a
+   time, one plain-path kernel per request.  A ray's result does not depend on
+   the other requests in flight. */
+#define PBRTX_SERVE_BATCHED 0
+#define PBRTX_SERVE_PLAIN 1
+typedef struct pbrtx_server pbrtx_server;
+typedef struct pbrtx_server_config {
+    uint64_t arena_bytes;        /* HBM for in-flight requests (admission budget) */
+    uint64_t scene_budget_bytes; /* cap on resident scene bytes, 0 = none */
+    uint32_t batch_rays;         /* rays per step (batched mode) */
+    int32_t mode;                /* PBRTX_SERVE_* */
+} pbrtx_server_config;
+/* per-request record; times in ms since the server was created */
+typedef struct pbrtx_request_stats {
+    uint64_t n_rays;
+    int32_t status;              /* 0 or the error code of a failed request */
+    uint32_t launches;           /* launches that carried part of it */
+    double submit_ms, admit_ms, first_launch_ms, done_ms;
+} pbrtx_request_stats;
+typedef struct pbrtx_server_stats {
+    uint64_t rays_done, requests_done, steps, launches;
+    double busy_ms;              /* time spent in GPU steps */
+    double elapsed_ms;           /* since creation */
+    uint64_t arena_bytes, scene_bytes;
+} pbrtx_server_counters;
+int pbrtx_server_create(const pbrtx_server_config* cfg, pbrtx_server** out);
+/* registers a scene (it must outlive the server); fails with
+   PBRTX_ERR_OOM past scene_budget_bytes */
+int pbrtx_server_add_scene(pbrtx_server* srv, const pbrtx_scene* scene, uint32_t* scene_id);
+/* kind 0: closest hit, out = n pbrtx_hit; kind 1: occlusion, out =
+   ceil(n/32) bit words.  rays and out are host memory that must stay valid
+   until pbrtx_server_wait returns for the ticket. */
+int pbrtx_server_submit(pbrtx_server* srv, uint32_t scene_id, int32_t kind, const pbrtx_ray* rays, uint32_t n,
+                        void* out, uint64_t* ticket);
+/* blocks until the request is done; fills *stats (may be NULL) and retires
+   the ticket */
+int pbrtx_server_wait(pbrtx_server* srv, uint64_t ticket, pbrtx_request_stats* stats);
+int pbrtx_server_stats(pbrtx_server* srv, pbrtx_server_counters* stats);
+/* finishes the requests already submitted, then stops */
+int pbrtx_server_destroy(pbrtx_server* srv);
+
 /* ---- device memory and timing plumbing ----------------------------------- */
 int pbrtx_device_malloc(void** d_ptr, uint64_t bytes);
 int pbrtx_device_free(void* d_ptr);
@@ -163,6 +215,10 @@ int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stre
 int pbrtx_timer_start(void* stream);
 int pbrtx_timer_stop(void* stream, float* ms);
 int pbrtx_device_name(char* buf, uint32_t len);
+/* page-locked host memory (request buffers whose copies should overlap
+   device work) */
+int pbrtx_host_malloc(void** ptr, uint64_t bytes);
+int pbrtx_host_free(void* ptr);
 
 #ifdef __cplusplus
 }

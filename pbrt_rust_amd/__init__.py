@@ -4,13 +4,14 @@ The compute runs in hand-written HIP kernels inside libpbrtx_gpu.so (C ABI:
 include/pbrtx_gpu.h); this package is the host-side mirror of the reference's
 aggregate / renderer interfaces.
 """
-from ._lib import HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, device_name, lib, synchronize
+from ._lib import (HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, PinnedArray, device_name, lib,
+                   synchronize)
 from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, PathIntegrator, Primitive, RayBatch,
-                  SamplerRenderer, Sphere, StratifiedSampler, Transform, TriangleMesh, set_tuning)
+                  RayServer, SamplerRenderer, Sphere, StratifiedSampler, Transform, TriangleMesh, set_tuning)
 
 __all__ = [
     "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "PathIntegrator", "SamplerRenderer",
-    "StratifiedSampler", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "Primitive",
-    "RAY_DTYPE", "RayBatch", "Sphere", "Transform", "TriangleMesh", "device_name", "lib", "set_tuning",
+    "StratifiedSampler", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "PinnedArray", "Primitive",
+    "RAY_DTYPE", "RayBatch", "RayServer", "Sphere", "Transform", "TriangleMesh", "device_name", "lib", "set_tuning",
     "synchronize",
 ]

@@ -54,6 +54,26 @@ class RenderStats(ctypes.Structure):
                 ("launches_closest", ctypes.c_uint32), ("launches_any", ctypes.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    """pbrtx_server_config (include/pbrtx_gpu.h)."""
+    _fields_ = [("arena_bytes", ctypes.c_uint64), ("scene_budget_bytes", ctypes.c_uint64),
+                ("batch_rays", ctypes.c_uint32), ("mode", ctypes.c_int32)]
+
+
+class RequestStats(ctypes.Structure):
+    """pbrtx_request_stats (include/pbrtx_gpu.h)."""
+    _fields_ = [("n_rays", ctypes.c_uint64), ("status", ctypes.c_int32), ("launches", ctypes.c_uint32),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("first_launch_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
+
+
+class ServerCounters(ctypes.Structure):
+    """pbrtx_server_counters (include/pbrtx_gpu.h)."""
+    _fields_ = [("rays_done", ctypes.c_uint64), ("requests_done", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("busy_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
+                ("arena_bytes", ctypes.c_uint64), ("scene_bytes", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -87,6 +107,14 @@ def lib():
     L.pbrtx_timer_start.argtypes = [vp]
     L.pbrtx_timer_stop.argtypes = [vp, c.POINTER(c.c_float)]
     L.pbrtx_device_name.argtypes = [c.c_char_p, u32]
+    L.pbrtx_host_malloc.argtypes = [c.POINTER(vp), u64]
+    L.pbrtx_host_free.argtypes = [vp]
+    L.pbrtx_server_create.argtypes = [c.POINTER(ServerConfig), c.POINTER(vp)]
+    L.pbrtx_server_add_scene.argtypes = [vp, vp, c.POINTER(u32)]
+    L.pbrtx_server_submit.argtypes = [vp, u32, i32, vp, u32, vp, c.POINTER(u64)]
+    L.pbrtx_server_wait.argtypes = [vp, u64, c.POINTER(RequestStats)]
+    L.pbrtx_server_stats.argtypes = [vp, c.POINTER(ServerCounters)]
+    L.pbrtx_server_destroy.argtypes = [vp]
     if hasattr(L, "pbrtx_renderer_create"):
         L.pbrtx_renderer_create.argtypes = [c.POINTER(RenderDesc), c.POINTER(vp)]
         L.pbrtx_renderer_render.argtypes = [vp, i32, vp, c.POINTER(RenderStats)]
@@ -135,6 +163,31 @@ class DeviceBuffer:
     def free(self):
         if getattr(self, "ptr", None):
             lib().pbrtx_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A numpy array over page-locked host memory (pbrtx_host_malloc)."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().pbrtx_host_malloc(ctypes.byref(p), max(n, 4)))
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(n, 4)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=np.uint8, count=n).view(dtype).reshape(shape)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().pbrtx_host_free(self.ptr)
             self.ptr = None
 
     def __del__(self):

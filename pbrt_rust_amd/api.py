@@ -144,6 +144,13 @@ class RayBatch:
         a["tmax"] = np.broadcast_to(np.asarray(maxt, F32), (n,))
         self.a = a
 
+    @classmethod
+    def from_records(cls, a):
+        """wrap an existing RAY_DTYPE array (no copy)"""
+        r = cls.__new__(cls)
+        r.a = np.ascontiguousarray(a, dtype=RAY_DTYPE)
+        return r
+
     def __len__(self):
         return self.a.shape[0]
 
@@ -273,6 +280,79 @@ class BVHAccelerator:
         if self._bvh:
             L.pbrtx_bvh_free(self._bvh)
             self._bvh = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RayServer:
+    """Request server over device-resident BVHs (pbrtx_server_*): requests of
+    rays for Intersectable::intersect ("closest") or intersect_p ("any")
+    against registered scenes, admitted by an HBM budget and traced with
+    continuous batching ("batched") or one plain kernel per request
+    ("plain").  submit() returns a ticket; wait(ticket) blocks and returns the
+    result (hit records or a bool array) and the request's timings (ms since
+    the server started)."""
+
+    MODES = {"batched": 0, "plain": 1}
+
+    def __init__(self, arena_bytes=4 << 30, batch_rays=1 << 22, mode="batched", scene_budget_bytes=0):
+        cfg = _lib.ServerConfig(int(arena_bytes), int(scene_budget_bytes), int(batch_rays), self.MODES[mode])
+        h = ctypes.c_void_p()
+        check(lib().pbrtx_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h.value
+        self._scenes = []
+        self._pending = {}
+
+    def add_scene(self, bvh):
+        sid = ctypes.c_uint32()
+        check(lib().pbrtx_server_add_scene(self._h, bvh.scene, ctypes.byref(sid)))
+        self._scenes.append(bvh)
+        return sid.value
+
+    def submit(self, scene_id, kind, rays, out=None):
+        """rays: a RayBatch or a RAY_DTYPE array; out (optional): the result
+        buffer, n HIT_DTYPE records ("closest") or ceil(n/32) uint32 words
+        ("any") -- page-locked arrays (PinnedArray) let copies overlap device
+        work.  Both are kept alive until wait()."""
+        a = np.ascontiguousarray(rays.a if isinstance(rays, RayBatch) else rays, dtype=RAY_DTYPE)
+        n = a.shape[0]
+        k = KINDS[kind] if isinstance(kind, str) else int(kind)
+        if out is None:
+            out = np.zeros(n, HIT_DTYPE) if k == 0 else np.zeros((n + 31) // 32, np.uint32)
+        elif out.nbytes < (n * HIT_DTYPE.itemsize if k == 0 else (n + 31) // 32 * 4) or not out.flags.c_contiguous:
+            raise ValueError("result buffer too small or not contiguous")
+        t = ctypes.c_uint64()
+        check(lib().pbrtx_server_submit(self._h, int(scene_id), k, a.ctypes.data, n, out.ctypes.data,
+                                        ctypes.byref(t)))
+        self._pending[t.value] = (a, out, k, n)
+        return t.value
+
+    def wait(self, ticket):
+        st = _lib.RequestStats()
+        rc = lib().pbrtx_server_wait(self._h, int(ticket), ctypes.byref(st))
+        a, out, k, n = self._pending.pop(ticket)
+        check(rc)
+        if k == 1:
+            out = np.unpackbits(out.view(np.uint8).reshape(-1)[:((n + 31) // 32) * 4], bitorder="little")[:n].astype(bool)
+        else:
+            out = out.view(HIT_DTYPE).reshape(-1)[:n]
+        return out, dict(n_rays=st.n_rays, launches=st.launches, submit_ms=st.submit_ms, admit_ms=st.admit_ms,
+                         first_launch_ms=st.first_launch_ms, done_ms=st.done_ms,
+                         latency_ms=st.done_ms - st.submit_ms)
+
+    def counters(self):
+        c = _lib.ServerCounters()
+        check(lib().pbrtx_server_stats(self._h, ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbrtx_server_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

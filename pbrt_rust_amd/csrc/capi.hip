@@ -132,10 +132,6 @@ int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int 
 
 }  // namespace px
 
-struct pbrtx_scene {
-    DeviceScene s;
-};
-
 namespace {
 hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 std::mutex g_ev_mu;
@@ -258,6 +254,18 @@ int pbrtx_device_malloc(void** p, uint64_t bytes) {
     if (!p) return fail(PBRTX_ERR_ARG, "null pointer");
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return PBRTX_OK;
+}
+int pbrtx_host_malloc(void** p, uint64_t bytes) {
+    if (!p) return fail(PBRTX_ERR_ARG, "null pointer");
+    hipError_t e = hipHostMalloc(p, bytes ? bytes : 4, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc");
+    return PBRTX_OK;
+}
+int pbrtx_host_free(void* p) {
+    if (!p) return PBRTX_OK;
+    hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) return hip_fail(e, "hipHostFree");
     return PBRTX_OK;
 }
 int pbrtx_device_free(void* p) {

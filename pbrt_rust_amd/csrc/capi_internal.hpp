@@ -25,3 +25,8 @@ int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int 
 extern TraceTuning g_tuning[2][2];  // [path][0 closest hit, 1 occlusion]
 
 }  // namespace px
+
+// the opaque handle of the C ABI
+struct pbrtx_scene {
+    px::DeviceScene s;
+};

@@ -252,12 +252,28 @@ __device__ __forceinline__ bool sel4b(int i, bool a, bool b, bool c, bool d) {
     return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
 }
 
-template <bool ANY>
+// Segment of a served launch: launch rays [start, start+count) are the
+// chunk's rays at `rays`; results go to hits / to the bits of occ (whose bit 0
+// is the chunk's first ray: chunks start at multiples of 32 of a request).
+// Segment starts are ascending.
+__device__ __forceinline__ uint32_t find_seg(const TraceSeg* segs, uint32_t nseg, uint32_t id) {
+    uint32_t lo = 0, hi = nseg;  // last segment with start <= id
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].start <= id) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// SEG: the rays come from a segment table (served launch) instead of one array.
+template <bool ANY, bool SEG>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
                                                           const uint32_t* n_dev, Hit* hits,
                                                           uint32_t* occ, uint32_t* counter,
                                                           uint2* spill, uint32_t spill_levels,
-                                                          uint32_t refill, uint32_t* err) {
+                                                          uint32_t refill, uint32_t* err,
+                                                          const TraceSeg* segs, uint32_t nseg) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[STK_WIDE][MAX_BLOCK];
     __shared__ float s_tn[ANY ? 1 : STK_WIDE][MAX_BLOCK];
@@ -270,6 +286,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
     RayState r{};
     bool neg0 = false, neg1 = false, neg2 = false;
     uint32_t rid = 0, hprim = NO_HIT;
+    uint32_t rseg = 0;  // SEG: segment of the ray; rid is then its index in the segment
     float hb1 = 0.0f, hb2 = 0.0f;
     int32_t cur = 0;
     bool have_cur = false;  // false: take the next stack entry
@@ -301,8 +318,14 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
             if (!has && !drained) {
                 const uint32_t id = b + (uint32_t)__popcll(need & lt_mask);
                 if (id < n) {
-                    rid = id;
-                    load_ray(rays, id, r);
+                    if (SEG) {
+                        rseg = find_seg(segs, nseg, id);
+                        rid = id - segs[rseg].start;
+                        load_ray(segs[rseg].rays, rid, r);
+                    } else {
+                        rid = id;
+                        load_ray(rays, id, r);
+                    }
                     neg0 = r.ix < 0.0f; neg1 = r.iy < 0.0f; neg2 = r.iz < 0.0f;
                     hprim = NO_HIT; hb1 = 0.0f; hb2 = 0.0f;
                     sp = 0;
@@ -338,8 +361,10 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                 if (!have_cur) {
                     // ray finished
                     if (!ANY) {
-                        hits[rid] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
-                        if (hprim != NO_HIT) rays[rid].tmax = r.tmax;
+                        Hit* hp = SEG ? segs[rseg].hits : hits;
+                        Ray* rp = SEG ? segs[rseg].rays : rays;
+                        hp[rid] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
+                        if (hprim != NO_HIT) rp[rid].tmax = r.tmax;
                     }
                     has = false;
                 }
@@ -400,7 +425,8 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                 }
                 have_cur = false;
                 if (ANY && hitnow) {
-                    atomicOr(occ + (rid >> 5), 1u << (rid & 31u));
+                    uint32_t* ow = SEG ? segs[rseg].occ : occ;
+                    atomicOr(ow + (rid >> 5), 1u << (rid & 31u));
                     has = false;
                 }
             }
@@ -632,8 +658,12 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
         return (size_t)plain_levels(s) * threads * sizeof(uint32_t);
     }
     TraceTuning t = tun;
-    size_t threads = (size_t)wide_grid(k_trace_wide<false>, t) * block;
-    size_t threads2 = (size_t)wide_grid(k_trace_wide<true>, t) * block;
+    size_t threads = (size_t)wide_grid(k_trace_wide<false, false>, t) * block;
+    size_t threads2 = (size_t)wide_grid(k_trace_wide<true, false>, t) * block;
+    const size_t threads3 = (size_t)wide_grid(k_trace_wide<false, true>, t) * block;
+    const size_t threads4 = (size_t)wide_grid(k_trace_wide<true, true>, t) * block;
+    if (threads3 > threads2) threads2 = threads3;
+    if (threads4 > threads2) threads2 = threads4;
     if (threads2 > threads) threads = threads2;
     return (size_t)wide_levels(s) * threads * sizeof(uint2);
 }
@@ -666,14 +696,14 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
                            rays, n, n_dev, occ, scr->counter, err);
         return hipGetLastError();
     }
-    const int grid = wide_grid(k_trace_wide<ANY>, tun);
+    const int grid = wide_grid(k_trace_wide<ANY, false>, tun);
     const uint32_t lv = wide_levels(s);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    hipLaunchKernelGGL(k_trace_wide<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ,
-                       scr->counter, (uint2*)scr->spill, lv, refill, err);
+    hipLaunchKernelGGL((k_trace_wide<ANY, false>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ,
+                       scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     return hipGetLastError();
 }
 
@@ -684,6 +714,29 @@ hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, i
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
     return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st, n_dev);
+}
+
+hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, uint32_t nseg, uint32_t total,
+                          const TraceTuning& tun, TraceScratch* scr, hipStream_t st) {
+    if (total == 0 || nseg == 0) return hipSuccess;
+    const int block = tun.block;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    const uint32_t lv = wide_levels(s);
+    const int grid = any ? wide_grid(k_trace_wide<true, true>, tun) : wide_grid(k_trace_wide<false, true>, tun);
+    if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
+    hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
+    uint32_t* err = scr->counter + 1;
+    if (any)
+        hipLaunchKernelGGL((k_trace_wide<true, true>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,
+                           (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,
+                           (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
+    else
+        hipLaunchKernelGGL((k_trace_wide<false, true>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,
+                           (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,
+                           (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
+    return hipGetLastError();
 }
 
 }  // namespace px

@@ -52,6 +52,18 @@ hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, i
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
                      const uint32_t* n_dev = nullptr);
+// One request chunk of a served launch (see trace_segments).
+struct TraceSeg {
+    Ray* rays;       // the chunk's rays (tmax updated on a closest hit)
+    Hit* hits;       // closest hit: one record per ray
+    uint32_t* occ;   // occlusion: bit k = ray k of the chunk (words pre-zeroed)
+    uint32_t start;  // first launch-wide ray index of the chunk (ascending)
+    uint32_t count;
+};
+// Batched-path traversal over `total` rays gathered from nseg chunks
+// (d_segs in device memory); closest hit (any = false) or occlusion.
+hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, uint32_t nseg, uint32_t total,
+                          const TraceTuning& tun, TraceScratch* scr, hipStream_t st);
 // Number of spill bytes a launch needs for this scene and tuning.
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
