@@ -75,6 +75,12 @@ int pbrtx_bvh_info(const pbrtx_bvh* bvh, uint32_t* n_nodes, uint32_t* depth, uin
    the BVH-ordered prim ids order[n_prims] */
 int pbrtx_bvh_export(const pbrtx_bvh* bvh, float* bounds, uint32_t* off, uint32_t* meta,
                      uint8_t* leaf, uint32_t* order);
+/* The device layouts the build produced, for inspection: counts of the
+   four-wide records (128 B each) and of the compressed records (64 B each,
+   0 when the scene has none), then copies of them and of the compressed
+   layout's 64-byte prim records (any pointer may be NULL). */
+int pbrtx_bvh_layout_info(const pbrtx_bvh* bvh, uint32_t* n_wide, uint32_t* n_compressed, uint32_t* n_prims);
+int pbrtx_bvh_export_layouts(const pbrtx_bvh* bvh, void* wide, void* compressed, void* prims64);
 int pbrtx_bvh_free(pbrtx_bvh* bvh);
 
 /* ---- device scene -------------------------------------------------------- */
@@ -98,8 +104,10 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
 /* Launch shape of a path's closest-hit (kind 0) or occlusion (kind 1)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
    (batched path); refill_pct = idle-lane refill threshold (batched path);
-   scheme 0 = a traversal stack per ray, 1 = one stack per wave of 64
-   consecutive rays (batched occlusion only).  Process-wide setting. */
+   scheme 0 = a traversal stack per ray over the compressed records (the
+   exact ones when a scene has none), 1 = one stack per wave of 64
+   consecutive rays (batched occlusion only), 2 = a stack per ray over the
+   exact records.  Process-wide setting. */
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme);
 

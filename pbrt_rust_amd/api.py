@@ -159,6 +159,11 @@ class RayBatch:
         return self.a["tmax"]
 
 
+QNODE_DTYPE = np.dtype([("lx", "<f4", 4), ("hx", "<f4", 4), ("ly", "<f4", 4), ("hy", "<f4", 4),
+                        ("lz", "<f4", 4), ("hz", "<f4", 4), ("ref", "<i4", 4), ("meta", "<u4"), ("pad", "<u4", 3)])
+CNODE_DTYPE = np.dtype([("o", "<f4", 3), ("emeta", "<u4"), ("q", "u1", 24), ("ref", "<i4", 4), ("pad", "<u4", 2)])
+
+
 class BVHAccelerator:
     """GPU aggregate with the reference's BVHAccelerator::new signature."""
 
@@ -220,6 +225,19 @@ class BVHAccelerator:
         check(lib().pbrtx_bvh_export(self._bvh, b.ctypes.data, off.ctypes.data, meta.ctypes.data,
                                      leaf.ctypes.data, order.ctypes.data))
         return b, off, meta, leaf, order
+
+    def layouts(self):
+        """The device layouts as numpy arrays: four-wide records (QNODE_DTYPE),
+        compressed records (CNODE_DTYPE, empty if none) and the compressed
+        layout's prim records (n x 16 float32)."""
+        nq, nc, npr = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().pbrtx_bvh_layout_info(self._bvh, ctypes.byref(nq), ctypes.byref(nc), ctypes.byref(npr)))
+        q = np.zeros(nq.value, QNODE_DTYPE)
+        c = np.zeros(nc.value, CNODE_DTYPE)
+        p = np.zeros((npr.value if nc.value else 0, 16), F32)
+        check(lib().pbrtx_bvh_export_layouts(self._bvh, q.ctypes.data, c.ctypes.data if c.size else None,
+                                             p.ctypes.data if p.size else None))
+        return q, c, p
 
     def world_bound(self):
         b = self.nodes()[0]
@@ -372,13 +390,15 @@ class Aggregate:
 KINDS = {"closest": 0, "any": 1}
 
 
-SCHEMES = {"ray": 0, "packet": 1}
+SCHEMES = {"ray": 0, "packet": 1, "ray_exact": 2}
 
 
 def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray"):
     """Launch shape of the closest-hit ("closest") or occlusion ("any")
-    kernel of a path, and its traversal scheme: "ray" (a stack per ray) or
-    "packet" (a stack per wave of 64 consecutive rays; batched occlusion)."""
+    kernel of a path, and its traversal scheme: "ray" (a stack per ray, the
+    compressed 64-byte records when the scene has them), "ray_exact" (a stack
+    per ray on the exact 128-byte records) or "packet" (a stack per wave of 64
+    consecutive rays; batched occlusion)."""
     k = KINDS[kind] if isinstance(kind, str) else int(kind)
     sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
     check(lib().pbrtx_set_tuning(_path_code(path), k, block, blocks_per_cu, refill_pct, sc))

@@ -145,6 +145,93 @@ inline float bits_f(uint32_t u) {
 
 }  // namespace
 
+// The float the GPU decodes from byte q at origin o and scale s (q * s is
+// exact: an 8-bit integer times a power of two, so the device's fused
+// multiply-add rounds the same sum once, as here).
+static float cq_decode(float o, uint32_t q, float s) {
+    const float t = (float)q * s;
+    return o + t;
+}
+
+// Quantizes one axis of a record: picks the scale exponent, then for every
+// slot the largest q whose decode is <= lo and the smallest whose decode is
+// >= hi.  Returns false when the union box is not finite or no exponent fits.
+static bool cq_axis(float ulo, float uhi, const float* lo, const float* hi, const bool* used, uint8_t* qlo,
+                    uint8_t* qhi, uint32_t* ebyte) {
+    if (!(std::fabs(ulo) <= FLT_MAX) || !(std::fabs(uhi) <= FLT_MAX)) return false;
+    const double ext = (double)uhi - (double)ulo;
+    int e = -126;
+    if (ext > 0.0) {
+        e = (int)std::ceil(std::log2(ext / 255.0));
+        if (e < -126) e = -126;
+    }
+    for (;; ++e) {
+        // q * 2^e must stay finite for every byte q (the device decodes with
+        // one fused multiply-add, equal to multiply-then-add only then)
+        if (e > 119) return false;
+        const float s = std::ldexp(1.0f, e);
+        if (cq_decode(ulo, 255u, s) >= uhi) break;
+    }
+    const float s = std::ldexp(1.0f, e);
+    for (int k = 0; k < 4; ++k) {
+        if (!used[k]) { qlo[k] = 255; qhi[k] = 0; continue; }
+        double a = std::floor(((double)lo[k] - (double)ulo) / (double)s);
+        int ql = a < 0 ? 0 : (a > 255 ? 255 : (int)a);
+        while (ql > 0 && cq_decode(ulo, (uint32_t)ql, s) > lo[k]) --ql;
+        double b = std::ceil(((double)hi[k] - (double)ulo) / (double)s);
+        int qh = b < 0 ? 0 : (b > 255 ? 255 : (int)b);
+        while (qh < 255 && cq_decode(ulo, (uint32_t)qh, s) < hi[k]) ++qh;
+        if (cq_decode(ulo, (uint32_t)ql, s) > lo[k] || cq_decode(ulo, (uint32_t)qh, s) < hi[k]) return false;
+        qlo[k] = (uint8_t)ql;
+        qhi[k] = (uint8_t)qh;
+    }
+    *ebyte = (uint32_t)(e + 127);
+    return true;
+}
+
+// Builds the compressed layout from the four-wide records and the leaves of
+// the reference tree.  Returns false if some record cannot be represented
+// (non-finite bounds); the exact layout is used then.
+static bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
+    out->cnodes.assign(out->qnodes.size(), CNode{});
+    for (size_t i = 0; i < out->qnodes.size(); ++i) {
+        const QNode& q = out->qnodes[i];
+        CNode& c = out->cnodes[i];
+        bool used[4];
+        float u[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int k = 0; k < 4; ++k) {
+            used[k] = q.ref[k] != QEMPTY;
+            c.ref[k] = q.ref[k];
+            if (!used[k]) continue;
+            u[0] = std::min(u[0], q.lx[k]); u[1] = std::min(u[1], q.ly[k]); u[2] = std::min(u[2], q.lz[k]);
+            u[3] = std::max(u[3], q.hx[k]); u[4] = std::max(u[4], q.hy[k]); u[5] = std::max(u[5], q.hz[k]);
+        }
+        uint32_t ex, ey, ez;
+        if (!cq_axis(u[0], u[3], q.lx, q.hx, used, c.q + 0, c.q + 4, &ex)) return false;
+        if (!cq_axis(u[1], u[4], q.ly, q.hy, used, c.q + 8, c.q + 12, &ey)) return false;
+        if (!cq_axis(u[2], u[5], q.lz, q.hz, used, c.q + 16, c.q + 20, &ez)) return false;
+        c.ox = u[0]; c.oy = u[1]; c.oz = u[2];
+        c.emeta = ex | (ey << 8) | (ez << 16) | ((q.meta & 63u) << 24);
+    }
+    // prim records with their leaf's exact box
+    out->prims64.assign(out->prims.size(), PrimRec64{});
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if ((nodes[i].meta & 3u) != 3u) continue;
+        const uint32_t first = nodes[i].off, cnt = nodes[i].meta >> 2;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const PrimRec& p = out->prims[first + k];
+            PrimRec64& r = out->prims64[first + k];
+            for (int j = 0; j < 9; ++j) r.w[j] = p.w[j];
+            uint32_t kind;
+            memcpy(&kind, &p.w[11], 4);
+            if ((kind & 0xFFu) != 0u) r.w[0] = p.w[9];  // sphere record index
+            for (int j = 0; j < 3; ++j) { r.w[9 + j] = nodes[i].lo[j]; r.w[12 + j] = nodes[i].hi[j]; }
+            r.w[15] = p.w[11];
+        }
+    }
+    return true;
+}
+
 bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
                const SphereIn* sph, uint32_t nsph, int split, uint32_t max_prims, HostBvh* out,
                std::string* err) {
@@ -403,6 +490,10 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
     }
     out->qroot = is_leaf(0) ? ~(int32_t)nodes[0].off : 0;
     out->qdepth = qdepth;
+    if (!compress_layout(nodes, out)) {
+        out->cnodes.clear();
+        out->prims64.clear();
+    }
     auto t1 = std::chrono::steady_clock::now();
     out->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return true;

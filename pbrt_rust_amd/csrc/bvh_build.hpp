@@ -23,6 +23,8 @@ struct HostBvh {
     std::vector<PrimRec> prims;     // prim records in BVH order
     std::vector<SphereRec> spheres;
     std::vector<QNode> qnodes;      // four-wide layout
+    std::vector<CNode> cnodes;      // compressed four-wide layout (empty if not representable)
+    std::vector<PrimRec64> prims64; // prim records of the compressed layout
     int32_t qroot = 0;              // root ref of the four-wide layout
     uint32_t qdepth = 0;            // record levels of the four-wide layout
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};

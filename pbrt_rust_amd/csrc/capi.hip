@@ -82,13 +82,24 @@ int scene_from_host(const HostBvh& h, DeviceScene* s) {
     if ((rc = upload(&s->d_qnodes, h.qnodes.data(), h.qnodes.size() * sizeof(QNode)))) return rc;
     if ((rc = upload(&s->d_prims, h.prims.data(), h.prims.size() * sizeof(PrimRec)))) return rc;
     if ((rc = upload(&s->d_spheres, h.spheres.data(), h.spheres.size() * sizeof(SphereRec)))) return rc;
+    if ((rc = upload(&s->d_cnodes, h.cnodes.data(), h.cnodes.size() * sizeof(CNode)))) return rc;
+    if ((rc = upload(&s->d_prims64, h.prims64.data(), h.prims64.size() * sizeof(PrimRec64)))) return rc;
+    if ((rc = upload(&s->d_order, h.order.data(), h.order.size() * sizeof(uint32_t)))) return rc;
     s->bytes = h.rnodes.size() * sizeof(RNode) + h.qnodes.size() * sizeof(QNode) +
-               h.prims.size() * sizeof(PrimRec) + h.spheres.size() * sizeof(SphereRec);
+               h.prims.size() * sizeof(PrimRec) + h.spheres.size() * sizeof(SphereRec) +
+               h.cnodes.size() * sizeof(CNode) + h.prims64.size() * sizeof(PrimRec64) +
+               h.order.size() * sizeof(uint32_t);
     SceneView& v = s->view;
     v.rnodes = (const RNode*)s->d_rnodes;
     v.qnodes = (const QNode*)s->d_qnodes;
     v.prims = (const PrimRec*)s->d_prims;
     v.spheres = (const SphereRec*)s->d_spheres;
+    // the compressed layout exists for a tree of records (a root that is a
+    // leaf has none) whose bounds are all finite
+    const bool cq = !h.cnodes.empty() && !h.prims64.empty();
+    v.cnodes = cq ? (const CNode*)s->d_cnodes : nullptr;
+    v.prims64 = cq ? (const PrimRec64*)s->d_prims64 : nullptr;
+    v.order = (const uint32_t*)s->d_order;
     v.n_rnodes = (uint32_t)h.rnodes.size();
     v.n_qnodes = (uint32_t)h.qnodes.size();
     v.n_prims = (uint32_t)h.prims.size();
@@ -105,6 +116,9 @@ void scene_free(DeviceScene* s) {
     if (s->d_qnodes) (void)hipFree(s->d_qnodes);
     if (s->d_prims) (void)hipFree(s->d_prims);
     if (s->d_spheres) (void)hipFree(s->d_spheres);
+    if (s->d_cnodes) (void)hipFree(s->d_cnodes);
+    if (s->d_prims64) (void)hipFree(s->d_prims64);
+    if (s->d_order) (void)hipFree(s->d_order);
     memset(s, 0, sizeof(*s));
 }
 
@@ -186,6 +200,23 @@ int pbrtx_bvh_export(const pbrtx_bvh* b, float* bounds, uint32_t* off, uint32_t*
     return PBRTX_OK;
 }
 
+int pbrtx_bvh_layout_info(const pbrtx_bvh* b, uint32_t* n_wide, uint32_t* n_compressed, uint32_t* n_prims) {
+    if (!b) return fail(PBRTX_ERR_ARG, "bvh is null");
+    if (n_wide) *n_wide = (uint32_t)b->h.qnodes.size();
+    if (n_compressed) *n_compressed = (uint32_t)b->h.cnodes.size();
+    if (n_prims) *n_prims = (uint32_t)b->h.prims.size();
+    return PBRTX_OK;
+}
+
+int pbrtx_bvh_export_layouts(const pbrtx_bvh* b, void* wide, void* compressed, void* prims64) {
+    if (!b) return fail(PBRTX_ERR_ARG, "bvh is null");
+    const HostBvh& h = b->h;
+    if (wide) memcpy(wide, h.qnodes.data(), h.qnodes.size() * sizeof(QNode));
+    if (compressed) memcpy(compressed, h.cnodes.data(), h.cnodes.size() * sizeof(CNode));
+    if (prims64) memcpy(prims64, h.prims64.data(), h.prims64.size() * sizeof(PrimRec64));
+    return PBRTX_OK;
+}
+
 int pbrtx_bvh_free(pbrtx_bvh* b) {
     delete b;
     return PBRTX_OK;
@@ -243,7 +274,8 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
     if (block != 64 && block != 128 && block != 256) return fail(PBRTX_ERR_ARG, "block must be 64, 128 or 256");
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
     if (refill_pct < 0 || refill_pct > 100) return fail(PBRTX_ERR_ARG, "refill_pct out of range");
-    if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET) return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
+    if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET && scheme != SCHEME_RAY_EXACT)
+        return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
     if (scheme == SCHEME_PACKET && (path != 1 || kind != 1))
         return fail(PBRTX_ERR_ARG, "the packet scheme exists for batched occlusion only");
     g_tuning[path][kind] = TraceTuning{block, blocks_per_cu, refill_pct, scheme};

@@ -83,6 +83,37 @@ struct PrimRec {
 };
 static_assert(sizeof(PrimRec) == 48, "PrimRec must be 48 bytes");
 
+// Compressed four-wide layout: the record tree of QNode with each slot box
+// quantized OUTWARD to 8 bits against the union box of the record:
+//   bound = o + q * 2^e   (one rounding: q * 2^e is exact)
+// with the origin o = the union's low corner and one power-of-two scale per
+// axis.  The host picks every q so that the decoded float box contains the
+// exact slot box, so a slot test may pass where the exact one fails but never
+// the reverse; the exact box of every leaf is tested when the leaf is popped
+// (it travels in the PrimRec64 records), which keeps results exact.  64 bytes
+// = four 16-byte loads per visit instead of eight.
+//   emeta: biased exponent byte of the x, y, z scales in bits 0-23 (scale =
+//   the float with that exponent field), QNode.meta in bits 24-29.
+//   q: lx[4], hx[4], ly[4], hy[4], lz[4], hz[4].
+struct CNode {
+    float ox, oy, oz;
+    uint32_t emeta;
+    uint8_t q[24];
+    int32_t ref[4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(CNode) == 64, "CNode must be 64 bytes");
+
+// Primitive record of the compressed layout, 64 bytes, BVH leaf order:
+// w[0..8] as PrimRec (a sphere keeps its record index in w[0]), w[9..14] the
+// exact box of the record's leaf (lo xyz, hi xyz), w[15] = kind | 0x100 on
+// the leaf's last record.  The input prim id is looked up in the ordered id
+// array only for a hit.
+struct PrimRec64 {
+    float w[16];
+};
+static_assert(sizeof(PrimRec64) == 64, "PrimRec64 must be 64 bytes");
+
 // Sphere (sphere.rs:16-44): world->object matrix rows, radius, z range, phi_max.
 struct SphereRec {
     float w2o[16];

@@ -266,8 +266,57 @@ __device__ __forceinline__ uint32_t find_seg(const TraceSeg* segs, uint32_t nseg
     return lo;
 }
 
+// Decoded bound of slot k (byte k of word w) of a compressed record:
+// o + q * s rounded once (q * s is exact: an 8-bit integer times a power of
+// two no larger than 2^119), the value the host checked against the exact box.
+__device__ __forceinline__ float cq_bound(uint32_t w, int k, float o, float s) {
+    const float q = (float)((w >> (8 * k)) & 0xFFu);
+    return __builtin_fmaf(q, s, o);
+}
+
+// Slab test of an interior record of the compressed layout, with the near
+// plane chosen by the sign of 1/d (nb_* near bounds, fb_* far bounds).  These
+// boxes only need to be conservative -- the exact leaf box is tested when the
+// leaf is popped -- and per axis this interval contains the reference's
+// (bbox.rs:185-209): equal to it whenever no product is NaN, and wider in the
+// one case where they differ (d = -0 with the ray origin on the box's low
+// face, where the reference's swap test sees a NaN).  Its entry distance is
+// therefore never above the exact one, so the pop-time re-check stays
+// conservative as well.
+__device__ __forceinline__ bool slab_signed(float nbx, float fbx, float nby, float fby, float nbz, float fbz,
+                                            float ox, float oy, float oz, float ix, float iy, float iz,
+                                            float tmin, float tmax, float* tn_out) {
+    const float nx = (nbx - ox) * ix, fx = (fbx - ox) * ix;
+    const float ny = (nby - oy) * iy, fy = (fby - oy) * iy;
+    const float nz = (nbz - oz) * iz, fz = (fbz - oz) * iz;
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, nx), ny), nz);
+    const float t1 = fminf(fminf(fminf(tmax, fx), fy), fz);
+    *tn_out = t0;
+    return !(t0 > t1);
+}
+__device__ __forceinline__ float cq_scale(uint32_t ebyte) { return as_f(ebyte << 23); }
+
+// Test one record of the compressed layout whose first three words are in
+// a, b, c (the fourth is loaded here); on a hit tmax becomes t and *pos_hit
+// the record's position.
+__device__ __forceinline__ bool prim_test64(const SceneView& sv, uint32_t pos, const float4 a, const float4 b,
+                                            const float4 c, uint32_t kind, RayState& r, uint32_t* pos_hit,
+                                            float* b1, float* b2) {
+    float t, u = 0.0f, v = 0.0f;
+    bool hit;
+    if ((kind & 0xFFu) == 0u) hit = tri_test(a, b, c, r, &t, &u, &v);
+    else hit = sphere_test(sv.spheres[as_u(a.x)], r, &t);
+    if (hit) {
+        r.tmax = t;
+        *pos_hit = pos;
+        *b1 = u; *b2 = v;
+    }
+    return hit;
+}
+
 // SEG: the rays come from a segment table (served launch) instead of one array.
-template <bool ANY, bool SEG>
+// CQ: the compressed records (CNode, PrimRec64) instead of QNode / PrimRec.
+template <bool ANY, bool SEG, bool CQ>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
                                                           const uint32_t* n_dev, Hit* hits,
                                                           uint32_t* occ, uint32_t* counter,
@@ -363,6 +412,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                     if (!ANY) {
                         Hit* hp = SEG ? segs[rseg].hits : hits;
                         Ray* rp = SEG ? segs[rseg].rays : rays;
+                        if (CQ && hprim != NO_HIT) hprim = sv.order[hprim];  // position -> prim id
                         hp[rid] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
                         if (hprim != NO_HIT) rp[rid].tmax = r.tmax;
                     }
@@ -370,19 +420,48 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                 }
             }
             if (has && cur >= 0) {
-                const float4* q = reinterpret_cast<const float4*>(sv.qnodes + cur);
-                const float4 LX = q[0], HX = q[1], LY = q[2], HY = q[3], LZ = q[4], HZ = q[5];
-                const int4 RF = reinterpret_cast<const int4*>(q)[6];
-                const uint32_t meta = reinterpret_cast<const uint32_t*>(q)[28];
+                int4 RF;
+                uint32_t meta;
                 float t0, t1, t2, t3;
-                const bool p0 = RF.x != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
-                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t0);
-                const bool p1 = RF.y != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
-                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t1);
-                const bool p2 = RF.z != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
-                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t2);
-                const bool p3 = RF.w != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
-                                                       r.ix, r.iy, r.iz, r.tmin, r.tmax, &t3);
+                bool p0, p1, p2, p3;
+                if constexpr (CQ) {
+                    const float4* q = reinterpret_cast<const float4*>(sv.cnodes + cur);
+                    const float4 W0 = q[0], W1 = q[1], W2 = q[2], W3 = q[3];
+                    const uint32_t em = as_u(W0.w);
+                    const float sx = cq_scale(em & 0xFFu), sy = cq_scale((em >> 8) & 0xFFu),
+                                sz = cq_scale((em >> 16) & 0xFFu);
+                    // near / far bytes of all four slots by the ray's signs
+                    const uint32_t nqx = neg0 ? as_u(W1.y) : as_u(W1.x), fqx = neg0 ? as_u(W1.x) : as_u(W1.y);
+                    const uint32_t nqy = neg1 ? as_u(W1.w) : as_u(W1.z), fqy = neg1 ? as_u(W1.z) : as_u(W1.w);
+                    const uint32_t nqz = neg2 ? as_u(W2.y) : as_u(W2.x), fqz = neg2 ? as_u(W2.x) : as_u(W2.y);
+                    RF = make_int4((int)as_u(W2.z), (int)as_u(W2.w), (int)as_u(W3.x), (int)as_u(W3.y));
+                    meta = em >> 24;
+#define PX_CQ_SLOT(K, P, T)                                                                                  \
+    P = RF.K != QEMPTY &&                                                                                     \
+        slab_signed(cq_bound(nqx, KI_##K, W0.x, sx), cq_bound(fqx, KI_##K, W0.x, sx),                         \
+                    cq_bound(nqy, KI_##K, W0.y, sy), cq_bound(fqy, KI_##K, W0.y, sy),                         \
+                    cq_bound(nqz, KI_##K, W0.z, sz), cq_bound(fqz, KI_##K, W0.z, sz), r.ox, r.oy, r.oz, r.ix,  \
+                    r.iy, r.iz, r.tmin, r.tmax, &T);
+                    constexpr int KI_x = 0, KI_y = 1, KI_z = 2, KI_w = 3;
+                    PX_CQ_SLOT(x, p0, t0)
+                    PX_CQ_SLOT(y, p1, t1)
+                    PX_CQ_SLOT(z, p2, t2)
+                    PX_CQ_SLOT(w, p3, t3)
+#undef PX_CQ_SLOT
+                } else {
+                    const float4* q = reinterpret_cast<const float4*>(sv.qnodes + cur);
+                    const float4 LX = q[0], HX = q[1], LY = q[2], HY = q[3], LZ = q[4], HZ = q[5];
+                    RF = reinterpret_cast<const int4*>(q)[6];
+                    meta = reinterpret_cast<const uint32_t*>(q)[28];
+                    p0 = RF.x != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
+                                                r.ix, r.iy, r.iz, r.tmin, r.tmax, &t0);
+                    p1 = RF.y != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
+                                                r.ix, r.iy, r.iz, r.tmin, r.tmax, &t1);
+                    p2 = RF.z != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
+                                                r.ix, r.iy, r.iz, r.tmin, r.tmax, &t2);
+                    p3 = RF.w != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
+                                                r.ix, r.iy, r.iz, r.tmin, r.tmax, &t3);
+                }
                 // visiting order of the reference (bvh.rs:409-415 applied at A,
                 // then at B and C): near half first, near child first in each
                 const uint32_t aA = meta & 3u, aB = (meta >> 2) & 3u, aC = (meta >> 4) & 3u;
@@ -414,14 +493,38 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                 // leaf: prims in order until the record flagged last
                 uint32_t pos = (uint32_t)(~cur);
                 bool hitnow = false;
-                while (true) {
-                    bool last;
-                    if (prim_test(sv, pos, r, &hprim, &hb1, &hb2, &last)) {
-                        hitnow = true;
-                        if (ANY) break;
+                if constexpr (CQ) {
+                    // the leaf's exact box at the current tmax: the reference's
+                    // pop-time test (the record boxes above it are wider)
+                    const float4* p = reinterpret_cast<const float4*>(sv.prims64 + pos);
+                    float4 P2 = p[2], P3 = p[3];
+                    float tn;
+                    if (slab(P2.y, P3.x, P2.z, P3.y, P2.w, P3.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax,
+                             &tn)) {
+                        while (true) {
+                            const float4 P0 = p[0], P1 = p[1];
+                            const uint32_t kind = as_u(P3.w);
+                            if (prim_test64(sv, pos, P0, P1, P2, kind, r, &hprim, &hb1, &hb2)) {
+                                hitnow = true;
+                                if (ANY) break;
+                            }
+                            if (kind & 0x100u) break;
+                            ++pos;
+                            p += 4;
+                            P2 = p[2];
+                            P3 = p[3];
+                        }
                     }
-                    if (last) break;
-                    ++pos;
+                } else {
+                    while (true) {
+                        bool last;
+                        if (prim_test(sv, pos, r, &hprim, &hb1, &hb2, &last)) {
+                            hitnow = true;
+                            if (ANY) break;
+                        }
+                        if (last) break;
+                        ++pos;
+                    }
                 }
                 have_cur = false;
                 if (ANY && hitnow) {
@@ -650,6 +753,9 @@ static uint32_t wide_levels(const SceneView& s) {
 // the packet kernel's stack holds at most three pending slots per record
 // level on the current path plus one
 static bool packet_fits(const SceneView& s) { return 3 * s.qdepth + 1 <= (uint32_t)PSTK; }
+static bool use_compressed(const SceneView& s, const TraceTuning& tun) {
+    return s.cnodes != nullptr && tun.scheme != SCHEME_RAY_EXACT;
+}
 
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
     const int block = tun.block;
@@ -658,12 +764,14 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
         return (size_t)plain_levels(s) * threads * sizeof(uint32_t);
     }
     TraceTuning t = tun;
-    size_t threads = (size_t)wide_grid(k_trace_wide<false, false>, t) * block;
-    size_t threads2 = (size_t)wide_grid(k_trace_wide<true, false>, t) * block;
-    const size_t threads3 = (size_t)wide_grid(k_trace_wide<false, true>, t) * block;
-    const size_t threads4 = (size_t)wide_grid(k_trace_wide<true, true>, t) * block;
-    if (threads3 > threads2) threads2 = threads3;
-    if (threads4 > threads2) threads2 = threads4;
+    // the largest persistent grid of any batched traversal kernel
+    size_t threads = 0;
+    for (size_t g : {wide_grid(k_trace_wide<false, false, false>, t), wide_grid(k_trace_wide<true, false, false>, t),
+                     wide_grid(k_trace_wide<false, true, false>, t), wide_grid(k_trace_wide<true, true, false>, t),
+                     wide_grid(k_trace_wide<false, false, true>, t), wide_grid(k_trace_wide<true, false, true>, t),
+                     wide_grid(k_trace_wide<false, true, true>, t), wide_grid(k_trace_wide<true, true, true>, t)})
+        threads = std::max(threads, g * (size_t)block);
+    size_t threads2 = threads;
     if (threads2 > threads) threads = threads2;
     return (size_t)wide_levels(s) * threads * sizeof(uint2);
 }
@@ -696,14 +804,19 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
                            rays, n, n_dev, occ, scr->counter, err);
         return hipGetLastError();
     }
-    const int grid = wide_grid(k_trace_wide<ANY, false>, tun);
+    const bool cq = use_compressed(s, tun);
+    const int grid = cq ? wide_grid(k_trace_wide<ANY, false, true>, tun) : wide_grid(k_trace_wide<ANY, false, false>, tun);
     const uint32_t lv = wide_levels(s);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    hipLaunchKernelGGL((k_trace_wide<ANY, false>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ,
-                       scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
+    if (cq)
+        hipLaunchKernelGGL((k_trace_wide<ANY, false, true>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
+                           occ, scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
+    else
+        hipLaunchKernelGGL((k_trace_wide<ANY, false, false>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
+                           occ, scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     return hipGetLastError();
 }
 
@@ -722,20 +835,24 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     const int block = tun.block;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     const uint32_t lv = wide_levels(s);
-    const int grid = any ? wide_grid(k_trace_wide<true, true>, tun) : wide_grid(k_trace_wide<false, true>, tun);
+    const bool cq = use_compressed(s, tun);
+    int grid;
+    if (cq) grid = any ? wide_grid(k_trace_wide<true, true, true>, tun) : wide_grid(k_trace_wide<false, true, true>, tun);
+    else grid = any ? wide_grid(k_trace_wide<true, true, false>, tun) : wide_grid(k_trace_wide<false, true, false>, tun);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
     uint32_t* err = scr->counter + 1;
-    if (any)
-        hipLaunchKernelGGL((k_trace_wide<true, true>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,
-                           (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,
-                           (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
-    else
-        hipLaunchKernelGGL((k_trace_wide<false, true>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,
-                           (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,
-                           (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
+#define PX_SEG_LAUNCH(A, C)                                                                                   \
+    hipLaunchKernelGGL((k_trace_wide<A, true, C>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,    \
+                       (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,             \
+                       (uint2*)scr->spill, lv, refill, err, d_segs, nseg)
+    if (any && cq) PX_SEG_LAUNCH(true, true);
+    else if (any) PX_SEG_LAUNCH(true, false);
+    else if (cq) PX_SEG_LAUNCH(false, true);
+    else PX_SEG_LAUNCH(false, false);
+#undef PX_SEG_LAUNCH
     return hipGetLastError();
 }
 

@@ -12,6 +12,9 @@ struct SceneView {
     const QNode* qnodes;
     const PrimRec* prims;
     const SphereRec* spheres;
+    const CNode* cnodes;       // compressed layout, null when the scene has none
+    const PrimRec64* prims64;
+    const uint32_t* order;     // input prim id of each leaf-order position
     uint32_t n_rnodes, n_qnodes, n_prims, n_spheres;
     int32_t qroot;
     uint32_t depth;   // levels of the reference tree
@@ -24,7 +27,10 @@ enum TracePath : int { PATH_PLAIN = 0, PATH_BATCHED = 1 };
 // Traversal scheme of the batched path: one stack per ray, or one stack per
 // wave of 64 consecutive rays (occlusion only; used while the scene's stack
 // bound fits the per-wave stack, else the per-ray kernel runs).
-enum TraceScheme : int { SCHEME_RAY = 0, SCHEME_PACKET = 1 };
+// SCHEME_RAY reads the compressed records when the scene has them and the
+// exact 128-byte records otherwise; SCHEME_RAY_EXACT always reads the exact
+// ones.
+enum TraceScheme : int { SCHEME_RAY = 0, SCHEME_PACKET = 1, SCHEME_RAY_EXACT = 2 };
 
 struct TraceTuning {
     int block = 256;          // threads per block

@@ -95,3 +95,63 @@ def test_unknown_split_warns_and_uses_sah():
         a = px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, idx, v))], 1, "bogus")
     b = px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, idx, v))], 1, "sah")
     assert np.array_equal(a.nodes()[0], b.nodes()[0])
+
+
+def build_product(v, idx, split, max_prims):
+    return px.BVHAccelerator([px.Primitive.simple(px.TriangleMesh(None, None, False, idx, v))], max_prims, split)
+
+
+def decode_compressed(c):
+    """float32 decode of every slot bound, with the kernel's operations:
+    o + q * 2^e (the product is exact)"""
+    em = c["emeta"]
+    out = {}
+    for a, name in enumerate("xyz"):
+        e = ((em >> (8 * a)) & 0xFF).astype(np.uint32)
+        s = (e << 23).view(np.float32)
+        o = c["o"][:, a].astype(np.float32)
+        for side, off in (("l", 0), ("h", 4)):
+            q = c["q"][:, 8 * a + off: 8 * a + off + 4].astype(np.float32)
+            out[side + name] = (o[:, None] + (q * s[:, None]).astype(np.float32)).astype(np.float32)
+    return out
+
+
+def check_compressed(bvh):
+    q, c, p = bvh.layouts()
+    if q.size == 0:
+        return
+    assert c.size == q.size
+    assert np.array_equal(c["ref"], q["ref"])
+    assert np.array_equal(c["emeta"] >> 24, q["meta"] & 63)
+    d = decode_compressed(c)
+    used = q["ref"] != -2 ** 31
+    for ax in "xyz":
+        assert np.all(d["l" + ax][used] <= q["l" + ax][used]), ax
+        assert np.all(d["h" + ax][used] >= q["h" + ax][used]), ax
+    # the wider box is not much wider: one quantization step per side, the
+    # step being the power of two at or above extent/255 (so < 2 extent/255)
+    ext = np.maximum.reduce([np.where(used, q["h" + ax], -np.inf).max(1) - np.where(used, q["l" + ax], np.inf).min(1)
+                             for ax in "xyz"])
+    for ax in "xyz":
+        slack = np.where(used, (q["l" + ax] - d["l" + ax]) + (d["h" + ax] - q["h" + ax]), 0)
+        assert np.all(slack <= 2 * (2 * ext[:, None] / 255) * 1.01 + 1e-6)
+    # leaf boxes of the 64-byte prim records are the reference leaves' boxes
+    b, off, meta, leaf, order = bvh.nodes()
+    for i in np.nonzero(leaf)[0][:2000]:
+        for k in range(meta[i]):
+            assert np.array_equal(p[off[i] + k, 9:15], b[i]), i
+        kinds = p[off[i]:off[i] + meta[i], 15].view(np.uint32)
+        assert (kinds[-1] & 0x100) and not np.any(kinds[:-1] & 0x100)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_compressed_records_contain_exact_boxes(seed):
+    v, idx = scenes.random_triangles(20000, seed)
+    check_compressed(build_product(v, idx, "sah", 1))
+    # flat boxes (axis-aligned quads), far-apart scales, the c1 scene shape
+    sc = scenes.direct_lighting_scene(level=4)
+    v, idx = sc["verts"], sc["idx"]
+    check_compressed(build_product(v, idx, "sah", 1))
+    v = np.concatenate([scenes.random_triangles(500, seed)[0] * 1e-3, scenes.random_triangles(500, seed + 9)[0] * 1e4])
+    idx = np.arange(v.shape[0], dtype=np.uint32).reshape(-1, 3)
+    check_compressed(build_product(v.astype(np.float32), idx, "middle", 4))

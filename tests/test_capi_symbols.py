@@ -53,5 +53,5 @@ def test_errors_are_codes_not_aborts():
     assert L.pbrtx_set_tuning(1, 1, 256, 0, 101, 0) < 0
     assert L.pbrtx_set_tuning(1, 0, 256, 0, 10, 1) < 0   # packet: occlusion only
     assert L.pbrtx_set_tuning(0, 1, 256, 0, 10, 1) < 0   # packet: batched only
-    assert L.pbrtx_set_tuning(1, 1, 256, 0, 10, 2) < 0
+    assert L.pbrtx_set_tuning(1, 1, 256, 0, 10, 3) < 0
     assert L.pbrtx_intersect(None, None, 1, None, 0, None) < 0

@@ -231,3 +231,24 @@ def test_packet_occlusion_scheme_matches_oracle(packet_occlusion):
         prims.append(px.Primitive.simple(px.Sphere(t, t.inverse(), False, 1.0, -1.0, 1.0, 360.0)))
     assert px.Aggregate.bvh(prims, 1, "sah").intersect_p(
         px.RayBatch([r["o"] for r in g["rays"]], [r["d"] for r in g["rays"]]), "batched").all()
+
+
+@pytest.fixture
+def exact_records():
+    px.set_tuning("batched", "closest", 256, 0, 0, "ray_exact")
+    px.set_tuning("batched", "any", 256, 0, 10, "ray_exact")
+    yield
+    px.set_tuning("batched", "closest", 256, 0, 0, "ray")
+    px.set_tuning("batched", "any", 256, 0, 10, "ray")
+
+
+def test_exact_record_scheme_matches_oracle(exact_records):
+    # the batched kernels on the exact 128-byte records (the default reads the
+    # compressed ones): the same fixtures and edge cases
+    test_random_scene_parity("batched", "sah")
+    test_random_scene_parity("batched", "equal")
+    test_edge_cases("batched")
+    test_mixed_finite_and_infinite_inverse_directions("batched")
+    test_deep_tree_uses_spill_stack("batched")
+    test_mixed_triangles_and_spheres("batched")
+    test_eight_spheres_prim_ids("batched", "middle")
