@@ -334,6 +334,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
 
     RayState r{};
     bool neg0 = false, neg1 = false, neg2 = false;
+    uint32_t negm = 0;  // bit a: 1/d < 0 on axis a
     uint32_t rid = 0, hprim = NO_HIT;
     uint32_t rseg = 0;  // SEG: segment of the ray; rid is then its index in the segment
     float hb1 = 0.0f, hb2 = 0.0f;
@@ -376,6 +377,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                         load_ray(rays, id, r);
                     }
                     neg0 = r.ix < 0.0f; neg1 = r.iy < 0.0f; neg2 = r.iz < 0.0f;
+                    negm = (neg0 ? 1u : 0u) | (neg1 ? 2u : 0u) | (neg2 ? 4u : 0u);
                     hprim = NO_HIT; hb1 = 0.0f; hb2 = 0.0f;
                     sp = 0;
                     float tn;
@@ -463,29 +465,40 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
                                                 r.ix, r.iy, r.iz, r.tmin, r.tmax, &t3);
                 }
                 // visiting order of the reference (bvh.rs:409-415 applied at A,
-                // then at B and C): near half first, near child first in each
-                const uint32_t aA = meta & 3u, aB = (meta >> 2) & 3u, aC = (meta >> 4) & 3u;
-                const bool nA = aA == 0 ? neg0 : (aA == 1 ? neg1 : neg2);
-                const bool nB = aB == 0 ? neg0 : (aB == 1 ? neg1 : neg2);
-                const bool nC = aC == 0 ? neg0 : (aC == 1 ? neg1 : neg2);
-                const int b0 = nB ? 1 : 0, b1 = nB ? 0 : 1, c0 = nC ? 3 : 2, c1 = nC ? 2 : 3;
-                const int o0 = nA ? c0 : b0, o1 = nA ? c1 : b1, o2 = nA ? b0 : c0, o3 = nA ? b1 : c1;
+                // then at B and C): near half first, near child first in each.
+                // The slots [B0 B1 C0 C1] are permuted in registers into that
+                // order by three conditional swaps.
+                const bool nA = (negm >> (meta & 3u)) & 1u;
+                const bool nB = (negm >> ((meta >> 2) & 3u)) & 1u;
+                const bool nC = (negm >> ((meta >> 4) & 3u)) & 1u;
+                int32_t r0 = RF.x, r1 = RF.y, r2 = RF.z, r3 = RF.w;
+#define PX_SWAP(C, I, J)                                                                 \
+    if (C) {                                                                              \
+        const bool pb = p##I; p##I = p##J; p##J = pb;                                     \
+        const int32_t rb = r##I; r##I = r##J; r##J = rb;                                  \
+        const float tb = t##I; t##I = t##J; t##J = tb;                                    \
+    }
+                PX_SWAP(nB, 0, 1)
+                PX_SWAP(nC, 2, 3)
+                PX_SWAP(nA, 0, 2)
+                PX_SWAP(nA, 1, 3)
+#undef PX_SWAP
                 // the first passing slot is taken now, the others pushed so
                 // that they pop in order
                 bool pend = false;
                 int32_t pref = 0;
                 float ptn = 0.0f;
-#define PX_VISIT(O)                                                                    \
-    if (sel4b(O, p0, p1, p2, p3)) {                                                     \
-        if (pend) push(pref, ptn);                                                      \
-        pend = true;                                                                    \
-        pref = sel4i(O, RF.x, RF.y, RF.z, RF.w);                                        \
-        ptn = sel4f(O, t0, t1, t2, t3);                                                 \
+#define PX_VISIT(K)                                                                      \
+    if (p##K) {                                                                           \
+        if (pend) push(pref, ptn);                                                        \
+        pend = true;                                                                      \
+        pref = r##K;                                                                      \
+        ptn = t##K;                                                                       \
     }
-                PX_VISIT(o3)
-                PX_VISIT(o2)
-                PX_VISIT(o1)
-                PX_VISIT(o0)
+                PX_VISIT(3)
+                PX_VISIT(2)
+                PX_VISIT(1)
+                PX_VISIT(0)
 #undef PX_VISIT
                 have_cur = pend;
                 cur = pref;
