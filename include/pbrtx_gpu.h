@@ -101,13 +101,14 @@ int pbrtx_intersect(pbrtx_scene* scene, pbrtx_ray* d_rays, uint32_t n, pbrtx_hit
 int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, int32_t path, void* stream);
 
-/* Launch shape of a path's closest-hit (kind 0) or occlusion (kind 1)
+/* Launch shape of a path's closest-hit (kind 0), occlusion (kind 1) or
+   renderer camera-ray (kind 2: closest hit of the coherent camera rays)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
    (batched path); refill_pct = idle-lane refill threshold (batched path);
    scheme 0 = a traversal stack per ray over the compressed records (the
    exact ones when a scene has none), 1 = one stack per wave of 64
-   consecutive rays (batched occlusion only), 2 = a stack per ray over the
-   exact records.  Process-wide setting. */
+   consecutive rays (batched path; equal-sign groups for closest hit), 2 = a
+   stack per ray over the exact records.  Process-wide setting. */
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme);
 

@@ -387,15 +387,15 @@ class Aggregate:
         return BVHAccelerator(prims, max_prims, sm)
 
 
-KINDS = {"closest": 0, "any": 1}
+KINDS = {"closest": 0, "any": 1, "camera": 2}
 
 
 SCHEMES = {"ray": 0, "packet": 1, "ray_exact": 2}
 
 
 def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray"):
-    """Launch shape of the closest-hit ("closest") or occlusion ("any")
-    kernel of a path, and its traversal scheme: "ray" (a stack per ray, the
+    """Launch shape of the closest-hit ("closest"), occlusion ("any") or
+    renderer camera-ray ("camera") kernel of a path, and its traversal scheme: "ray" (a stack per ray, the
     compressed 64-byte records when the scene has them), "ray_exact" (a stack
     per ray on the exact 128-byte records) or "packet" (a stack per wave of 64
     consecutive rays; batched occlusion)."""

@@ -1,8 +1,8 @@
 """Autotuner for the batched traversal kernels' launch shape.
 
-For the closest-hit and the occlusion kernel separately it sweeps threads per
-block, persistent blocks per CU, the idle-lane refill threshold and (for
-occlusion) the traversal scheme -- a stack per ray or one per wave of 64
+For the renderer's camera-ray (closest-hit) and occlusion kernels separately
+it sweeps threads per block, persistent blocks per CU, the idle-lane refill
+threshold and the traversal scheme -- a stack per ray or one per wave of 64
 consecutive rays -- on a real frame of a renderer, keeps the fastest
 configuration of each (timed with HIP events on the render stream), and
 stores both as JSON so later runs (bench, serving) start from them:
@@ -25,7 +25,9 @@ DEFAULT_SPACE = {
 }
 # the packet scheme has no refill threshold
 PACKET_SPACE = {"block": [64, 128, 256], "blocks_per_cu": [0, 2, 4, 8, 16]}
-STAGE = {"closest": "ms_trace_closest", "any": "ms_trace_any"}
+# on a direct-lighting frame the closest-hit stage traces the camera rays
+STAGE = {"camera": "ms_trace_closest", "any": "ms_trace_any"}
+KINDS = ("camera", "any")
 
 
 def sweep(renderer, space=None, repeats=2, log=None):
@@ -33,12 +35,11 @@ def sweep(renderer, space=None, repeats=2, log=None):
     {kind: (results, best)} and leaves the best shapes applied."""
     space = space or DEFAULT_SPACE
     out = {}
-    for kind in ("closest", "any"):
+    for kind in KINDS:
         cands = [(b, c, f, "ray") for b, c, f in
                  itertools.product(space["block"], space["blocks_per_cu"], space["refill_pct"])]
-        if kind == "any":
-            cands += [(b, c, 0, "packet") for b, c in
-                      itertools.product(PACKET_SPACE["block"], PACKET_SPACE["blocks_per_cu"])]
+        cands += [(b, c, 0, "packet") for b, c in
+                  itertools.product(PACKET_SPACE["block"], PACKET_SPACE["blocks_per_cu"])]
         results = []
         for block, bpc, refill, scheme in cands:
             try:
@@ -59,7 +60,7 @@ def sweep(renderer, space=None, repeats=2, log=None):
 
 
 def apply(cfg):
-    for kind in ("closest", "any"):
+    for kind in ("closest", "any", "camera"):
         b = cfg.get(kind)
         if b:
             api.set_tuning("batched", kind, b["block"], b["blocks_per_cu"], b["refill_pct"], b.get("scheme", "ray"))

@@ -30,8 +30,11 @@ int hip_fail(hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? PBRTX_ERR_OOM : PBRTX_ERR_HIP;
 }
 
-TraceTuning g_tuning[2][2] = {{TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}},
-                              {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 10, SCHEME_RAY}}};
+// [path][kind]: closest hit, occlusion, camera rays of the renderer (coherent:
+// the batched path walks them as packets)
+TraceTuning g_tuning[2][3] = {
+    {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}},
+    {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 10, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_PACKET}}};
 
 namespace {
 std::mutex g_scr_mu;
@@ -270,14 +273,13 @@ int pbrtx_intersect_p(pbrtx_scene* s, const pbrtx_ray* d_rays, uint32_t n, uint3
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme) {
     if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");
-    if (kind != 0 && kind != 1) return fail(PBRTX_ERR_ARG, "kind must be 0 (closest hit) or 1 (occlusion)");
+    if (kind < 0 || kind > 2) return fail(PBRTX_ERR_ARG, "kind must be 0 (closest hit), 1 (occlusion) or 2 (camera)");
     if (block != 64 && block != 128 && block != 256) return fail(PBRTX_ERR_ARG, "block must be 64, 128 or 256");
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
     if (refill_pct < 0 || refill_pct > 100) return fail(PBRTX_ERR_ARG, "refill_pct out of range");
     if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET && scheme != SCHEME_RAY_EXACT)
         return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
-    if (scheme == SCHEME_PACKET && (path != 1 || kind != 1))
-        return fail(PBRTX_ERR_ARG, "the packet scheme exists for batched occlusion only");
+    if (scheme == SCHEME_PACKET && path != 1) return fail(PBRTX_ERR_ARG, "the packet scheme exists for the batched path only");
     g_tuning[path][kind] = TraceTuning{block, blocks_per_cu, refill_pct, scheme};
     return PBRTX_OK;
 }

@@ -936,11 +936,13 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     const bool batched = path == PATH_BATCHED;
     const TraceTuning& tunC = g_tuning[path][0];
     const TraceTuning& tunA = g_tuning[path][1];
+    const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
     TraceScratch* scr;
     {
         const size_t a = trace_spill_bytes(R->scene.view, path, tunC);
         const size_t b = trace_spill_bytes(R->scene.view, path, tunA);
-        ICHK(stream_scratch(st, a > b ? a : b, &scr));
+        const size_t c = trace_spill_bytes(R->scene.view, path, tunCam);
+        ICHK(stream_scratch(st, std::max(a, std::max(b, c)), &scr));
     }
     R->events.used = 0;
     Timing T;
@@ -969,7 +971,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             T.other.push_back({a, b});
             a = ev(); b = ev();
             RCHK(hipEventRecord(a, st));
-            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunC, scr, st));
+            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunCam, scr, st));
             RCHK(hipEventRecord(b, st));
             T.closest.push_back({a, b});
             a = ev(); b = ev();
@@ -1040,10 +1042,11 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
+                const TraceTuning& tb = bounce == 0 ? tunCam : tunC;  // bounce 0: camera rays
                 if (batched)
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tunC, scr, st, cin));
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st, cin));
                 else
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tunC, scr, st));
+                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st));
                 RCHK(hipEventRecord(b, st));
                 T.closest.push_back({a, b});
                 a = ev(); b = ev();

@@ -684,6 +684,160 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_occlude_packet(SceneView sv, cons
     }
 }
 
+// Batched closest hit, wave-coherent ("packet") form, for coherent rays such
+// as a camera's.  A wave takes 64 consecutive rays and splits them into
+// groups of equal direction signs; the reference's visiting order
+// (bvh.rs:409-415) depends on the ray only through those signs, so one group
+// walks the records in exactly each member's order with ONE stack of (record,
+// lane mask) entries, every record and prim record read once per wave from a
+// wave-uniform address.  Each lane tests boxes and prims against its own
+// tmax, so a lane reaches and tests exactly what the reference's traversal of
+// its ray would, in the same order (a later equal-t hit replaces an earlier
+// one, mesh.rs:71).  The pop-time test of an interior record is left out: a
+// record popped after the lane's tmax has dropped below its entry distance
+// only tests its children again, and they fail (a child box lies inside its
+// parent's, and the rounded slab distances are monotone in the bounds), so
+// that visit changes nothing.  A leaf's pop-time test is kept: each lane
+// tests the leaf's exact box (carried by the 64-byte prim records) against
+// its current tmax before the leaf's prims.  Interior records are the exact
+// 128-byte ones.
+__global__ __launch_bounds__(MAX_BLOCK) void k_closest_packet(SceneView sv, Ray* rays, uint32_t n_host,
+                                                              const uint32_t* n_dev, Hit* hits, uint32_t* counter,
+                                                              uint32_t* err) {
+    const uint32_t n = n_dev ? *n_dev : n_host;
+    __shared__ int32_t s_ref[PWAVES][PSTK];
+    __shared__ unsigned long long s_msk[PWAVES][PSTK];
+    const uint32_t lane = __lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counter, 64u);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        RayState r{};
+        bool alive = false;
+        uint32_t negm = 0;
+        if (valid) {
+            load_ray(rays, i, r);
+            negm = (r.ix < 0.0f ? 1u : 0u) | (r.iy < 0.0f ? 2u : 0u) | (r.iz < 0.0f ? 4u : 0u);
+            float tn;
+            alive = slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2],
+                         sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &tn);
+        }
+        uint32_t hprim = NO_HIT;
+        float hb1 = 0.0f, hb2 = 0.0f;
+        const unsigned long long alive_m = __ballot(alive);
+        unsigned long long todo = __ballot(valid);
+        while (todo) {
+            // the next group: lanes with the signs of the first remaining lane
+            const uint32_t lead = (uint32_t)__builtin_ctzll(todo);
+            const uint32_t g = __builtin_amdgcn_readfirstlane(__shfl(negm, (int)lead));
+            const unsigned long long grp = __ballot(valid && negm == g) & todo;
+            todo &= ~grp;
+            const unsigned long long live = grp & alive_m;
+            if (!live) continue;
+            int sp = 0;
+            int32_t cur = sv.qroot;
+            unsigned long long cm = live;  // lanes that reached the current entry
+            while (true) {
+                const bool act = (cm >> lane) & 1ull;
+                if (cur >= 0) {
+                    const QNode* q = sv.qnodes + cur;
+                    const float4 LX = cload4(q, 0), HX = cload4(q, 1), LY = cload4(q, 2), HY = cload4(q, 3),
+                                 LZ = cload4(q, 4), HZ = cload4(q, 5), R6 = cload4(q, 6);
+                    const uint32_t meta = as_u(cload4(q, 7).x);
+                    int32_t r0 = (int32_t)as_u(R6.x), r1 = (int32_t)as_u(R6.y), r2 = (int32_t)as_u(R6.z),
+                            r3 = (int32_t)as_u(R6.w);
+                    float t;
+                    bool p0 = act && r0 != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
+                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    bool p1 = act && r1 != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
+                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    bool p2 = act && r2 != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
+                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    bool p3 = act && r3 != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
+                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    unsigned long long m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2), m3 = __ballot(p3);
+                    // the group's visiting order: near half first, near child first
+                    const bool nA = (g >> (meta & 3u)) & 1u;
+                    const bool nB = (g >> ((meta >> 2) & 3u)) & 1u;
+                    const bool nC = (g >> ((meta >> 4) & 3u)) & 1u;
+#define PX_SWAP2(C, I, J)                                                                \
+    if (C) {                                                                              \
+        const unsigned long long mb = m##I; m##I = m##J; m##J = mb;                       \
+        const int32_t rb = r##I; r##I = r##J; r##J = rb;                                  \
+    }
+                    PX_SWAP2(nB, 0, 1)
+                    PX_SWAP2(nC, 2, 3)
+                    PX_SWAP2(nA, 0, 2)
+                    PX_SWAP2(nA, 1, 3)
+#undef PX_SWAP2
+                    // nearest passing slot next, the others pushed farthest first
+                    bool pend = false;
+                    int32_t pref = 0;
+                    unsigned long long pm = 0;
+#define PX_VISIT2(K)                                                                     \
+    if (m##K) {                                                                           \
+        if (pend) {                                                                       \
+            if (sp < PSTK) { s_ref[w][sp] = pref; s_msk[w][sp] = pm; ++sp; }              \
+            else atomicOr(err, 1u);                                                       \
+        }                                                                                 \
+        pend = true; pref = r##K; pm = m##K;                                              \
+    }
+                    PX_VISIT2(3)
+                    PX_VISIT2(2)
+                    PX_VISIT2(1)
+                    PX_VISIT2(0)
+#undef PX_VISIT2
+                    if (pend) {
+                        cur = __builtin_amdgcn_readfirstlane(pref);
+                        cm = pm;
+                        continue;
+                    }
+                } else {
+                    // leaf: the reference's pop-time box test at each lane's
+                    // tmax, then its prims in order until the record flagged last
+                    uint32_t pos = (uint32_t)(~cur);
+                    const PrimRec64* pr = sv.prims64 + pos;
+                    float4 c = cload4(pr, 2), d = cload4(pr, 3);
+                    float tn;
+                    const bool go = act && slab(c.y, d.x, c.z, d.y, c.w, d.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz,
+                                                r.tmin, r.tmax, &tn);
+                    if (__ballot(go)) {
+                        while (true) {
+                            const float4 a = cload4(pr, 0), b = cload4(pr, 1);
+                            const uint32_t kind = as_u(d.w);
+                            if (go) {
+                                float tt, u = 0.0f, v = 0.0f;
+                                bool h;
+                                if ((kind & 0xFFu) == 0u) h = tri_test(a, b, c, r, &tt, &u, &v);
+                                else h = sphere_test(sv.spheres[as_u(a.x)], r, &tt);
+                                if (h) { r.tmax = tt; hprim = pos; hb1 = u; hb2 = v; }
+                            }
+                            if (kind & 0x100u) break;
+                            ++pos;
+                            ++pr;
+                            c = cload4(pr, 2);
+                            d = cload4(pr, 3);
+                        }
+                    }
+                }
+                if (sp == 0) break;
+                --sp;
+                cur = __builtin_amdgcn_readfirstlane(s_ref[w][sp]);
+                cm = s_msk[w][sp];
+            }
+        }
+        if (valid) {
+            if (hprim != NO_HIT) hprim = sv.order[hprim];  // position -> prim id
+            hits[i] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
+            if (hprim != NO_HIT) rays[i].tmax = r.tmax;
+        }
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -815,6 +969,13 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_occlude_packet, dim3(wide_grid(k_occlude_packet, tun)), dim3(block), 0, st, s,
                            rays, n, n_dev, occ, scr->counter, err);
+        return hipGetLastError();
+    }
+    if (!ANY && tun.scheme == SCHEME_PACKET && packet_fits(s) && s.prims64) {
+        hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_closest_packet, dim3(wide_grid(k_closest_packet, tun)), dim3(block), 0, st, s,
+                           rays, n, n_dev, hits, scr->counter, err);
         return hipGetLastError();
     }
     const bool cq = use_compressed(s, tun);

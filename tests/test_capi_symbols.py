@@ -49,9 +49,8 @@ def test_errors_are_codes_not_aborts():
     i = np.array([0, 1, 2], np.uint32)
     assert L.pbrtx_bvh_build(v.ctypes.data, 3, i.ctypes.data, 1, None, 0, 7, 1, ctypes.byref(h)) < 0
     assert L.pbrtx_set_tuning(0, 0, 100, 0, 50, 0) < 0
-    assert L.pbrtx_set_tuning(1, 2, 256, 0, 50, 0) < 0
+    assert L.pbrtx_set_tuning(1, 3, 256, 0, 50, 0) < 0
     assert L.pbrtx_set_tuning(1, 1, 256, 0, 101, 0) < 0
-    assert L.pbrtx_set_tuning(1, 0, 256, 0, 10, 1) < 0   # packet: occlusion only
     assert L.pbrtx_set_tuning(0, 1, 256, 0, 10, 1) < 0   # packet: batched only
     assert L.pbrtx_set_tuning(1, 1, 256, 0, 10, 3) < 0
     assert L.pbrtx_intersect(None, None, 1, None, 0, None) < 0

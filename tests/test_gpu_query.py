@@ -252,3 +252,26 @@ def test_exact_record_scheme_matches_oracle(exact_records):
     test_deep_tree_uses_spill_stack("batched")
     test_mixed_triangles_and_spheres("batched")
     test_eight_spheres_prim_ids("batched", "middle")
+
+
+@pytest.fixture
+def packet_closest():
+    px.set_tuning("batched", "closest", 256, 0, 0, "packet")
+    yield
+    px.set_tuning("batched", "closest", 256, 0, 0, "ray")
+
+
+def test_packet_closest_scheme_matches_oracle(packet_closest):
+    # one stack per group of equal direction signs in a wave of 64 rays:
+    # coherent camera rays, incoherent rays (several sign groups per wave),
+    # zero and subnormal components, spheres, the maxt write-back, and a tree
+    # too deep for the per-wave stack (falls back to the per-ray kernel)
+    test_ray_query_camera_parity("batched")
+    test_random_scene_parity("batched", "sah")
+    test_random_scene_parity("batched", "middle")
+    test_edge_cases("batched")
+    test_mixed_finite_and_infinite_inverse_directions("batched")
+    test_mixed_triangles_and_spheres("batched")
+    test_eight_spheres_prim_ids("batched", "equal")
+    test_tetrahedron_maxt_side_effect("batched")
+    test_deep_tree_uses_spill_stack("batched")

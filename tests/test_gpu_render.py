@@ -108,3 +108,18 @@ def test_film_boundary_samples():
     ref = oracle_render(scene, cam, integ)
     img, samples, _ = gpu_render(scene, cam, integ, "batched")
     assert_same(ref, img, samples)
+
+
+def test_camera_ray_schemes_give_the_same_frame():
+    # the batched path walks camera rays as packets by default; the per-ray
+    # kernel (a tuning choice) must give the same samples
+    scene = scenes.direct_lighting_scene(level=4)
+    cam = scenes.pinhole(64, 48)
+    ref = oracle_render(scene, cam, DIRECT)
+    try:
+        for scheme in ("ray", "ray_exact", "packet"):
+            px.set_tuning("batched", "camera", 256, 0, 0, scheme)
+            img, samples, st = gpu_render(scene, cam, DIRECT, "batched")
+            assert_same(ref, img, samples)
+    finally:
+        px.set_tuning("batched", "camera", 256, 0, 0, "packet")
