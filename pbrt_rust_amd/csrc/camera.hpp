@@ -1,0 +1,63 @@
+// camera.hpp -- camera-ray generation shared by the renderer's kernels and the
+// traversal kernel that generates its camera rays itself: both evaluate the
+// same operations, so the rays are bit-identical wherever they are made.
+#pragma once
+#include "pbrtx_common.hpp"
+
+namespace px {
+
+// What camera-ray generation needs (a subset of the renderer's parameters).
+struct CamParams {
+    float r2c[16], c2w[16];  // raster->camera (with /w), camera->world
+    uint32_t xres, nx, spp, seed;
+    float dx, dy;            // 1/nx, 1/ny
+    uint64_t g0;             // global index of the launch's first sample
+};
+
+// ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
+__device__ __forceinline__ f3 xf_point(const float* m, f3 p) {
+    float xt = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    float yt = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    float zt = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    if (w != 1.0f) return mk3(xt / w, yt / w, zt / w);
+    return mk3(xt, yt, zt);
+}
+__device__ __forceinline__ f3 xf_vector(const float* m, f3 v) {
+    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+               m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+
+// StratifiedSampler image sample of pixel (px,py), sample s (stratified.rs:60-83,
+// montecarlo.rs:119-128) with the counter-based jitter.  P: any parameter
+// block with seed, xres, nx, dx, dy.
+template <class P>
+__device__ __forceinline__ void sample_pos(const P& p, uint32_t px_, uint32_t py, uint32_t s, float* ix, float* iy) {
+    float u[4];
+    rng4(p.seed, py * p.xres + px_, s, 0, u);
+    uint32_t sx = s % p.nx, sy = s / p.nx;
+    *ix = (((float)sx + u[0]) * p.dx) + (float)px_;
+    *iy = (((float)sy + u[1]) * p.dy) + (float)py;
+}
+
+// Camera::generate_ray_differential's main ray (camera/mod.rs:168-195,
+// 226-236, 269-270) with a static cam_to_world.
+template <class P>
+__device__ __forceinline__ void camera_ray(const P& p, float ix, float iy, Ray* r) {
+    f3 pc = xf_point(p.r2c, mk3(ix, iy, 0.0f));
+    f3 d = normalize3(pc);
+    f3 o = xf_point(p.c2w, mk3(0.0f, 0.0f, 0.0f));
+    f3 dw = xf_vector(p.c2w, d);
+    *r = Ray{o.x, o.y, o.z, 0.0f, dw.x, dw.y, dw.z, FLT_MAX};
+}
+
+// The camera ray of global sample g (pixel g / spp, sample g % spp).
+template <class P>
+__device__ __forceinline__ void camera_ray_of(const P& p, uint64_t g, Ray* r) {
+    const uint32_t pixel = (uint32_t)(g / p.spp), s = (uint32_t)(g % p.spp);
+    float ix, iy;
+    sample_pos(p, pixel % p.xres, pixel / p.xres, s, &ix, &iy);
+    camera_ray(p, ix, iy, r);
+}
+
+}  // namespace px

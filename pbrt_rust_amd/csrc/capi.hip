@@ -34,7 +34,7 @@ int hip_fail(hipError_t e, const char* what) {
 // the batched path walks them as packets)
 TraceTuning g_tuning[2][3] = {
     {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}},
-    {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 10, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_PACKET}}};
+    {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 10, SCHEME_RAY}, TraceTuning{128, 8, 0, SCHEME_PACKET}}};
 
 namespace {
 std::mutex g_scr_mu;

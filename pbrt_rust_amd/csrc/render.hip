@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/pbrtx_gpu.h"
+#include "camera.hpp"
 #include "capi_internal.hpp"
 
 namespace px {
@@ -67,41 +68,6 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr int RB = 256;  // render kernel block
 
 __device__ __forceinline__ f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
-
-// ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
-__device__ __forceinline__ f3 xf_point(const float* m, f3 p) {
-    float xt = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
-    float yt = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
-    float zt = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
-    float w = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
-    if (w != 1.0f) return mk3(xt / w, yt / w, zt / w);
-    return mk3(xt, yt, zt);
-}
-__device__ __forceinline__ f3 xf_vector(const float* m, f3 v) {
-    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
-               m[8] * v.x + m[9] * v.y + m[10] * v.z);
-}
-
-// StratifiedSampler image sample of pixel (px,py), sample s (stratified.rs:60-83,
-// montecarlo.rs:119-128) with the counter-based jitter.
-__device__ __forceinline__ void sample_pos(const RenderParams& P, uint32_t px_, uint32_t py, uint32_t s,
-                                           float* ix, float* iy) {
-    float u[4];
-    rng4(P.seed, py * P.xres + px_, s, 0, u);
-    uint32_t sx = s % P.nx, sy = s / P.nx;
-    *ix = (((float)sx + u[0]) * P.dx) + (float)px_;
-    *iy = (((float)sy + u[1]) * P.dy) + (float)py;
-}
-
-// Camera::generate_ray_differential's main ray (camera/mod.rs:168-195,
-// 226-236, 269-270) with a static cam_to_world.
-__device__ __forceinline__ void camera_ray(const RenderParams& P, float ix, float iy, Ray* r) {
-    f3 pc = xf_point(P.r2c, mk3(ix, iy, 0.0f));
-    f3 d = normalize3(pc);
-    f3 o = xf_point(P.c2w, mk3(0.0f, 0.0f, 0.0f));
-    f3 dw = xf_vector(P.c2w, d);
-    *r = Ray{o.x, o.y, o.z, 0.0f, dw.x, dw.y, dw.z, FLT_MAX};
-}
 
 __device__ __forceinline__ bool black3(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 __device__ __forceinline__ f3 mulc(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
@@ -184,12 +150,8 @@ __device__ __forceinline__ void set_dead(Ray* r) { *r = Ray{0, 0, 0, 1.0f, 0, 0,
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(RB) void k_camera(RenderParams P, uint64_t g0, uint32_t n, Ray* rays) {
     for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
-        uint64_t g = g0 + i;
-        uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
-        float ix, iy;
-        sample_pos(P, pixel % P.xres, pixel / P.xres, s, &ix, &iy);
         Ray r;
-        camera_ray(P, ix, iy, &r);
+        camera_ray_of(P, g0 + i, &r);
         rays[i] = r;
     }
 }
@@ -277,7 +239,10 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         MatRec mat{1u, 0, 0, 0};
         uint32_t pixel = 0, s = 0;
         if (hit) {
-            const Ray r = rays[i];
+            // the camera ray: read, or made again as the traversal made it
+            Ray r;
+            if (rays) r = rays[i];
+            else camera_ray_of(P, g0 + i, &r);
             p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
             const FaceShade fs = P.faces[h.prim];
             nn = ld3(fs.nn);
@@ -963,15 +928,31 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             ICHK(R->qidx.ensure(slots ? slots : 1));
             ICHK(R->lbase.ensure(n));
             ICHK(R->occ.ensure((slots + 31) / 32 + 1));
-            hipEvent_t a = ev(), b = ev();
-            RCHK(hipEventRecord(a, st));
-            hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p);
-            RCHK(hipGetLastError());
-            RCHK(hipEventRecord(b, st));
-            T.other.push_back({a, b});
+            hipEvent_t a, b;
+            // batched: the packet kernel makes the camera rays itself and the
+            // shade kernel makes them again (rays = null): no ray array
+            const bool fused = batched && camera_fusable(sv, tunCam);
+            Ray* crays = fused ? nullptr : R->rays.p;
+            if (!fused) {
+                a = ev(); b = ev();
+                RCHK(hipEventRecord(a, st));
+                hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p);
+                RCHK(hipGetLastError());
+                RCHK(hipEventRecord(b, st));
+                T.other.push_back({a, b});
+            }
             a = ev(); b = ev();
             RCHK(hipEventRecord(a, st));
-            RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunCam, scr, st));
+            if (fused) {
+                CamParams cp;
+                memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
+                memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
+                cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
+                cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
+                RCHK(trace_camera(sv, cp, n, R->hits.p, tunCam, scr, st));
+            } else {
+                RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunCam, scr, st));
+            }
             RCHK(hipEventRecord(b, st));
             T.closest.push_back({a, b});
             a = ev(); b = ev();
@@ -982,7 +963,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 ICHK(R->bcounts.ensure(2 * (size_t)nb));
                 ICHK(R->boffsets.ensure(2 * (size_t)nb));
                 if (batched) {
-                    hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
+                    hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p,
                                        R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
                                        (const uint32_t*)R->boffsets.p, (int)SHADE_COUNT);
                     RCHK(hipGetLastError());
@@ -990,7 +971,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                                        cnt + 5);
                     RCHK(hipGetLastError());
                 }
-                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, R->rays.p, R->hits.p,
+                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p,
                                    R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
                                    (const uint32_t*)R->boffsets.p, (int)(batched ? SHADE_WRITE : SHADE_PLAIN));
                 RCHK(hipGetLastError());

@@ -21,6 +21,7 @@
 //              its children's boxes, and the rounded slab distances are
 //              monotone in the box bounds, so whenever a grandchild passes,
 //              its parent passes too (at the same or a larger tmax).
+#include "camera.hpp"
 #include "trace.hpp"
 
 namespace px {
@@ -158,6 +159,11 @@ __device__ __forceinline__ void load_ray(const Ray* rays, uint32_t i, RayState& 
     r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
     r.dx = b.x; r.dy = b.y; r.dz = b.z; r.tmax = b.w;
     // bvh.rs:382-383 (and the per-axis 1/d of bbox.rs:191: the same value)
+    r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
+}
+__device__ __forceinline__ void set_ray(const Ray& a, RayState& r) {
+    r.ox = a.ox; r.oy = a.oy; r.oz = a.oz; r.tmin = a.tmin;
+    r.dx = a.dx; r.dy = a.dy; r.dz = a.dz; r.tmax = a.tmax;
     r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
 }
 
@@ -700,10 +706,12 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_occlude_packet(SceneView sv, cons
 // that visit changes nothing.  A leaf's pop-time test is kept: each lane
 // tests the leaf's exact box (carried by the 64-byte prim records) against
 // its current tmax before the leaf's prims.  Interior records are the exact
-// 128-byte ones.
+// 128-byte ones.  GEN: ray i is the camera ray of sample cam.g0 + i, made
+// here instead of read (no ray array; only the hits are written).
+template <bool GEN>
 __global__ __launch_bounds__(MAX_BLOCK) void k_closest_packet(SceneView sv, Ray* rays, uint32_t n_host,
                                                               const uint32_t* n_dev, Hit* hits, uint32_t* counter,
-                                                              uint32_t* err) {
+                                                              uint32_t* err, CamParams cam) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[PWAVES][PSTK];
     __shared__ unsigned long long s_msk[PWAVES][PSTK];
@@ -720,7 +728,13 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_closest_packet(SceneView sv, Ray*
         bool alive = false;
         uint32_t negm = 0;
         if (valid) {
-            load_ray(rays, i, r);
+            if constexpr (GEN) {
+                Ray g;
+                camera_ray_of(cam, cam.g0 + i, &g);
+                set_ray(g, r);
+            } else {
+                load_ray(rays, i, r);
+            }
             negm = (r.ix < 0.0f ? 1u : 0u) | (r.iy < 0.0f ? 2u : 0u) | (r.iz < 0.0f ? 4u : 0u);
             float tn;
             alive = slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2],
@@ -833,7 +847,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_closest_packet(SceneView sv, Ray*
         if (valid) {
             if (hprim != NO_HIT) hprim = sv.order[hprim];  // position -> prim id
             hits[i] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
-            if (hprim != NO_HIT) rays[i].tmax = r.tmax;
+            if (!GEN && hprim != NO_HIT) rays[i].tmax = r.tmax;
         }
     }
 }
@@ -920,6 +934,9 @@ static uint32_t wide_levels(const SceneView& s) {
 // the packet kernel's stack holds at most three pending slots per record
 // level on the current path plus one
 static bool packet_fits(const SceneView& s) { return 3 * s.qdepth + 1 <= (uint32_t)PSTK; }
+bool camera_fusable(const SceneView& s, const TraceTuning& tun) {
+    return tun.scheme == SCHEME_PACKET && packet_fits(s) && s.prims64 != nullptr;
+}
 static bool use_compressed(const SceneView& s, const TraceTuning& tun) {
     return s.cnodes != nullptr && tun.scheme != SCHEME_RAY_EXACT;
 }
@@ -974,8 +991,8 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     if (!ANY && tun.scheme == SCHEME_PACKET && packet_fits(s) && s.prims64) {
         hipError_t e = fill_u32(scr->counter, 0u, 1, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_closest_packet, dim3(wide_grid(k_closest_packet, tun)), dim3(block), 0, st, s,
-                           rays, n, n_dev, hits, scr->counter, err);
+        hipLaunchKernelGGL(k_closest_packet<false>, dim3(wide_grid(k_closest_packet<false>, tun)), dim3(block), 0,
+                           st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{});
         return hipGetLastError();
     }
     const bool cq = use_compressed(s, tun);
@@ -1001,6 +1018,19 @@ hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, i
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
     return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st, n_dev);
+}
+
+hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
+                        TraceScratch* scr, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (!camera_fusable(s, tun)) return hipErrorNotSupported;
+    const int block = tun.block;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_closest_packet<true>, dim3(wide_grid(k_closest_packet<true>, tun)), dim3(block), 0, st, s,
+                       (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter, scr->counter + 1, cam);
+    return hipGetLastError();
 }
 
 hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, uint32_t nseg, uint32_t total,

@@ -58,6 +58,14 @@ hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, i
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
                      const uint32_t* n_dev = nullptr);
+struct CamParams;
+// Closest hit of the camera rays of samples [cam.g0, cam.g0 + n), generated
+// inside the (packet) traversal kernel; only the hits are written.
+// hipErrorNotSupported unless camera_fusable(s, tun).
+hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
+                        TraceScratch* scr, hipStream_t st);
+bool camera_fusable(const SceneView& s, const TraceTuning& tun);
+
 // One request chunk of a served launch (see trace_segments).
 struct TraceSeg {
     Ray* rays;       // the chunk's rays (tmax updated on a closest hit)

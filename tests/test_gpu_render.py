@@ -122,4 +122,4 @@ def test_camera_ray_schemes_give_the_same_frame():
             img, samples, st = gpu_render(scene, cam, DIRECT, "batched")
             assert_same(ref, img, samples)
     finally:
-        px.set_tuning("batched", "camera", 256, 0, 0, "packet")
+        px.set_tuning("batched", "camera", 128, 8, 0, "packet")
