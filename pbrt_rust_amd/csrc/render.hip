@@ -215,107 +215,182 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_counts(const uint32_t* counts, 
     if (t == 0) *total = tot;
 }
 
+// Decoupled look-back (D. Merrill and M. Garland, "Single-pass Parallel
+// Prefix Scan with Decoupled Look-back", NVIDIA technical report
+// NVR-2016-002), run by the first wave of a block: publishes the block's total,
+// returns the sum of the totals of the blocks before it, then publishes its
+// inclusive prefix.  tiles[b] = (flag << 32) | value, flag 0 not yet
+// published, 1 block total, 2 inclusive prefix.  Block ids are taken in start
+// order, so every block looked at has started, and it publishes its total
+// without waiting on any other block: each wait ends.
+__device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t tot) {
+    const uint32_t lane = __lane_id();
+    if (lane == 0)
+        __hip_atomic_store(tiles + bid, ((unsigned long long)(bid == 0 ? 2u : 1u) << 32) | tot, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0;
+    if (bid > 0) {
+        long long top = (long long)bid - 1;
+        while (true) {
+            const long long j = top - (long long)lane;
+            unsigned long long v = 2ull << 32;  // before block 0: an inclusive 0
+            if (j >= 0) {
+                do {
+                    v = __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((v >> 32) == 0) __builtin_amdgcn_s_sleep(1);
+                } while ((v >> 32) == 0);
+            }
+            // blocks from the nearest one back to the nearest inclusive prefix
+            const unsigned long long incl = __ballot((v >> 32) == 2);
+            const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+            uint32_t val = lane <= stop ? (uint32_t)v : 0u;
+            for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off);
+            excl += val;
+            if (incl) break;
+            top -= 64;
+        }
+        if (lane == 0)
+            __hip_atomic_store(tiles + bid, (2ull << 32) | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return excl;
+}
+
 // Shade modes: SHADE_PLAIN writes every slot in place (dead slots carry an
 // empty interval) and counts live rays with one atomic per block;
-// SHADE_COUNT only counts live rays per block; SHADE_WRITE writes the live
-// rays densely at the scanned block offsets, in sample order.  The count and
-// write passes evaluate the same pure function of (hit, sample id).
-enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_COUNT = 1, SHADE_WRITE = 2 };
+// SHADE_QUEUE writes the live rays densely, in sample order, at offsets from
+// a single-pass scan over the blocks (decoupled look-back), and the last
+// block writes their number.  With at most SH_CACHE slots per sample the
+// light samples are kept in registers between the count and the write.
+enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_QUEUE = 1 };
+// The path tracer's vertex kernel fills two queues (shadow and next rays):
+// a count pass, a scan of the per-block counts, then a write pass.
+enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_COUNT = 1, PSHADE_WRITE = 2 };
+constexpr uint32_t SH_CACHE = 4;
 
 __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
                                                      const Hit* hits, Ray* sh, float4* contrib,
                                                      uint32_t* qidx, float4* lbase, uint32_t* sh_count,
-                                                     uint32_t* block_counts, const uint32_t* block_offsets,
-                                                     int mode) {
+                                                     unsigned long long* tiles, uint32_t* ticket, int mode) {
     const uint32_t S = P.S;
-    {
-        const uint32_t base = blockIdx.x * RB;
-        const uint32_t i = base + threadIdx.x;
-        const bool live = i < n;
-        Hit h = live ? hits[i] : Hit{NO_HIT, 0, 0, 0};
-        const bool hit = live && h.prim != NO_HIT;
-        f3 p = mk3(0, 0, 0), nn = mk3(0, 0, 0), wo = mk3(0, 0, 0);
-        float eps = 0.0f;
-        MatRec mat{1u, 0, 0, 0};
-        uint32_t pixel = 0, s = 0;
-        if (hit) {
-            // the camera ray: read, or made again as the traversal made it
-            Ray r;
-            if (rays) r = rays[i];
-            else camera_ray_of(P, g0 + i, &r);
-            p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
-            const FaceShade fs = P.faces[h.prim];
-            nn = ld3(fs.nn);
-            wo = neg3(mk3(r.dx, r.dy, r.dz));
-            eps = h.t * 5e-4f;  // mesh.rs:262
-            mat = P.mats[fs.mat];
-            f3 le = mk3(0, 0, 0);
-            if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
-            if (mode != SHADE_COUNT) lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
-            uint64_t g = g0 + i;
-            pixel = (uint32_t)(g / P.spp); s = (uint32_t)(g % P.spp);
-        } else if (live && mode != SHADE_COUNT) {
-            lbase[i] = make_float4(0, 0, 0, 0);
-        }
-        auto slot_sample = [&](uint32_t k, Ray* sr, f3* c) -> bool {
-            if (!hit) return false;
-            uint32_t li, j;
-            if (k < P.n_point) { li = k; j = 0; }
-            else { li = P.n_point + (k - P.n_point) / P.light_samples; j = (k - P.n_point) % P.light_samples; }
-            float u[4] = {0, 0, 0, 0};
-            if (li >= P.n_point) rng4(P.seed, pixel, s, 1 + (li - P.n_point) * P.light_samples + j, u);
-            return sample_light(P, li, p, eps, nn, wo, mat, u[0], u[1], u[2], sr, c);
-        };
-        if (mode == SHADE_COUNT) {
-            uint32_t cnt = 0;
-            for (uint32_t k = 0; k < S; ++k) {
-                Ray sr;
-                f3 c;
-                cnt += slot_sample(k, &sr, &c) ? 1u : 0u;
-            }
-            uint32_t tot;
-            block_exclusive_scan(cnt, &tot);
-            if (threadIdx.x == 0) block_counts[blockIdx.x] = tot;
-            return;
-        }
-        uint32_t cnt = 0, pos = 0;
-        if (mode == SHADE_WRITE) {
-            for (uint32_t k = 0; k < S; ++k) {
-                Ray sr;
-                f3 c;
-                cnt += slot_sample(k, &sr, &c) ? 1u : 0u;
-            }
-            uint32_t tot;
-            pos = block_offsets[blockIdx.x] + block_exclusive_scan(cnt, &tot);
-        }
-        // The queue index of slot k of sample i is stored light-sample major
-        // (at k*n + i) so that a wave's stores are contiguous; the
-        // contribution of a traced ray sits at its queue position.  Rays stay
-        // sample major in both forms (the light samples of one pixel are
-        // neighbours in the queue, which keeps waves coherent).
+    __shared__ uint32_t s_bid, s_excl;
+    uint32_t bid = blockIdx.x;
+    if (mode == SHADE_QUEUE) {
+        // blocks take their sample ranges in start order (the look-back's
+        // progress guarantee)
+        if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
+        __syncthreads();
+        bid = s_bid;
+    }
+    const uint32_t i = bid * RB + threadIdx.x;
+    const bool live = i < n;
+    Hit h = live ? hits[i] : Hit{NO_HIT, 0, 0, 0};
+    const bool hit = live && h.prim != NO_HIT;
+    f3 p = mk3(0, 0, 0), nn = mk3(0, 0, 0), wo = mk3(0, 0, 0);
+    float eps = 0.0f;
+    MatRec mat{1u, 0, 0, 0};
+    uint32_t pixel = 0, s = 0;
+    if (hit) {
+        // the camera ray: read, or made again as the traversal made it
+        Ray r;
+        if (rays) r = rays[i];
+        else camera_ray_of(P, g0 + i, &r);
+        p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
+        const FaceShade fs = P.faces[h.prim];
+        nn = ld3(fs.nn);
+        wo = neg3(mk3(r.dx, r.dy, r.dz));
+        eps = h.t * 5e-4f;  // mesh.rs:262
+        mat = P.mats[fs.mat];
+        f3 le = mk3(0, 0, 0);
+        if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+        lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
+        uint64_t g = g0 + i;
+        pixel = (uint32_t)(g / P.spp); s = (uint32_t)(g % P.spp);
+    } else if (live) {
+        lbase[i] = make_float4(0, 0, 0, 0);
+    }
+    auto slot_sample = [&](uint32_t k, Ray* sr, f3* c) -> bool {
+        if (!hit) return false;
+        uint32_t li, j;
+        if (k < P.n_point) { li = k; j = 0; }
+        else { li = P.n_point + (k - P.n_point) / P.light_samples; j = (k - P.n_point) % P.light_samples; }
+        float u[4] = {0, 0, 0, 0};
+        if (li >= P.n_point) rng4(P.seed, pixel, s, 1 + (li - P.n_point) * P.light_samples + j, u);
+        return sample_light(P, li, p, eps, nn, wo, mat, u[0], u[1], u[2], sr, c);
+    };
+    if (mode == SHADE_PLAIN) {
         uint32_t live_rays = 0;
         for (uint32_t k = 0; k < S; ++k) {
             Ray sr;
             f3 c = mk3(0, 0, 0);
             const bool valid = slot_sample(k, &sr, &c);
-            uint32_t q = NONE;
-            if (mode == SHADE_WRITE) {
-                if (valid) { q = pos++; sh[q] = sr; }
-            } else {
-                const size_t slot = (size_t)i * S + k;
-                if (!valid) set_dead(&sr);
-                if (live) sh[slot] = sr;
-                if (valid) q = (uint32_t)slot;
-                live_rays += valid ? 1u : 0u;
+            const size_t slot = (size_t)i * S + k;
+            if (!valid) set_dead(&sr);
+            if (live) {
+                sh[slot] = sr;
+                qidx[(size_t)k * n + i] = valid ? (uint32_t)slot : NONE;
             }
-            if (valid) contrib[q] = make_float4(c.x, c.y, c.z, 0.0f);
-            if (live) qidx[(size_t)k * n + i] = q;
+            if (valid) contrib[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+            live_rays += valid ? 1u : 0u;
         }
-        if (mode == SHADE_PLAIN) {
-            uint32_t tot;
-            block_exclusive_scan(live_rays, &tot);
-            if (threadIdx.x == 0 && tot) atomicAdd(sh_count, tot);
+        uint32_t tot;
+        block_exclusive_scan(live_rays, &tot);
+        if (threadIdx.x == 0 && tot) atomicAdd(sh_count, tot);
+        return;
+    }
+    // SHADE_QUEUE: count (keeping the samples when they fit), scan, write.
+    // The queue index of slot k of sample i is stored light-sample major
+    // (at k*n + i) so that a wave's stores are contiguous; the contribution
+    // of a traced ray sits at its queue position.  Rays are sample major (the
+    // light samples of one pixel are neighbours in the queue, which keeps
+    // waves coherent).
+    const bool cached = S <= SH_CACHE;
+    Ray cr[SH_CACHE];
+    f3 cc[SH_CACHE];
+    uint32_t vm = 0, cnt = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+        Ray sr;
+        f3 c = mk3(0, 0, 0);
+        const bool valid = slot_sample(k, &sr, &c);
+        if (valid) {
+            ++cnt;
+            vm |= 1u << (k & 31u);
         }
+        if (cached) {
+#pragma unroll
+            for (uint32_t kk = 0; kk < SH_CACHE; ++kk)
+                if (kk == k) { cr[kk] = sr; cc[kk] = c; }
+        }
+    }
+    uint32_t tot;
+    const uint32_t local = block_exclusive_scan(cnt, &tot);
+    if (threadIdx.x < 64) {
+        const uint32_t e = lookback(tiles, bid, tot);
+        if (threadIdx.x == 0) {
+            s_excl = e;
+            if ((bid + 1) * RB >= n) *sh_count = e + tot;  // the last block
+        }
+    }
+    __syncthreads();
+    uint32_t pos = s_excl + local;
+    for (uint32_t k = 0; k < S; ++k) {
+        Ray sr;
+        f3 c = mk3(0, 0, 0);
+        bool valid;
+        if (cached) {
+            valid = (vm >> k) & 1u;
+#pragma unroll
+            for (uint32_t kk = 0; kk < SH_CACHE; ++kk)
+                if (kk == k) { sr = cr[kk]; c = cc[kk]; }
+        } else {
+            valid = slot_sample(k, &sr, &c);
+        }
+        uint32_t q = NONE;
+        if (valid) {
+            q = pos++;
+            sh[q] = sr;
+            contrib[q] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+        if (live) qidx[(size_t)k * n + i] = q;
     }
 }
 
@@ -478,7 +553,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
             want_next = cont;
             finish = !cont;
         }
-        if (mode == SHADE_COUNT) {
+        if (mode == PSHADE_COUNT) {
             uint32_t ts, tn;
             block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
             block_exclusive_scan(want_next ? 1u : 0u, &tn);
@@ -498,7 +573,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         uint32_t ts, tn;
         const uint32_t rs = block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
         const uint32_t rn = block_exclusive_scan(want_next ? 1u : 0u, &tn);
-        if (mode == SHADE_WRITE) {
+        if (mode == PSHADE_WRITE) {
             const uint32_t qs = bo[blockIdx.x] + rs, qn = bo[nblocks + blockIdx.x] + rn;
             if (want_shadow) { sh[qs] = sray; sh_pid[qs] = pid; shc[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f); }
             if (want_next) { rays_out[qn] = nray; pid_out[qn] = pid; }
@@ -960,20 +1035,15 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             RCHK(fill_u32(cnt + 5, 0u, 1, st));
             {
                 const uint32_t nb = (n + RB - 1) / RB;
-                ICHK(R->bcounts.ensure(2 * (size_t)nb));
-                ICHK(R->boffsets.ensure(2 * (size_t)nb));
                 if (batched) {
-                    hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p,
-                                       R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
-                                       (const uint32_t*)R->boffsets.p, (int)SHADE_COUNT);
-                    RCHK(hipGetLastError());
-                    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p, nb, R->boffsets.p,
-                                       cnt + 5);
-                    RCHK(hipGetLastError());
+                    // look-back tiles (two words per block) and the block ticket
+                    ICHK(R->bcounts.ensure(2 * (size_t)nb + 2));
+                    RCHK(fill_u32(R->bcounts.p, 0u, 2 * (size_t)nb + 2, st));
                 }
-                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p,
-                                   R->sh.p, R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5, R->bcounts.p,
-                                   (const uint32_t*)R->boffsets.p, (int)(batched ? SHADE_WRITE : SHADE_PLAIN));
+                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p, R->sh.p,
+                                   R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5,
+                                   (unsigned long long*)R->bcounts.p, R->bcounts.p + 2 * (size_t)nb,
+                                   (int)(batched ? SHADE_QUEUE : SHADE_PLAIN));
                 RCHK(hipGetLastError());
             }
             RCHK(hipEventRecord(b, st));
@@ -1042,7 +1112,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                         hipLaunchKernelGGL(k_path_shade, dim3(nb), dim3(RB), 0, st, P, g0, bounce, n,
                                            (const uint32_t*)cin, (const uint32_t*)pin, rin, R->hits.p, R->st.p, rout,
                                            pout, cout, R->sh.p, R->shpid.p, R->contrib.p, cnt + 5, R->sample_l.p,
-                                           R->bcounts.p, (const uint32_t*)R->boffsets.p, nb, (int)SHADE_COUNT);
+                                           R->bcounts.p, (const uint32_t*)R->boffsets.p, nb, (int)PSHADE_COUNT);
                         RCHK(hipGetLastError());
                         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p, nb,
                                            R->boffsets.p, cnt + 5);
@@ -1056,7 +1126,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                                        R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
                                        batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p,
                                        R->bcounts.p, (const uint32_t*)R->boffsets.p, nb,
-                                       (int)(batched ? SHADE_WRITE : SHADE_PLAIN));
+                                       (int)(batched ? PSHADE_WRITE : PSHADE_PLAIN));
                     RCHK(hipGetLastError());
                 }
                 RCHK(hipEventRecord(b, st));
