@@ -338,11 +338,12 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         return;
     }
     // SHADE_QUEUE: count (keeping the samples when they fit), scan, write.
-    // The queue index of slot k of sample i is stored light-sample major
-    // (at k*n + i) so that a wave's stores are contiguous; the contribution
-    // of a traced ray sits at its queue position.  Rays are sample major (the
-    // light samples of one pixel are neighbours in the queue, which keeps
-    // waves coherent).
+    // Within a block's part of the queue the rays are ordered by (light
+    // sample k, sample), so the stores of one k are one contiguous run across
+    // the lanes; consecutive rays still start in neighbouring pixels.  The
+    // queue index of slot k of sample i is stored at k*n + i (contiguous per
+    // wave as well); the contribution of a traced ray sits at its queue
+    // position.
     const bool cached = S <= SH_CACHE;
     Ray cr[SH_CACHE];
     f3 cc[SH_CACHE];
@@ -361,8 +362,36 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
                 if (kk == k) { cr[kk] = sr; cc[kk] = c; }
         }
     }
+    // rank of each lane's slot k among the block's valid slots k, and the
+    // block's count of valid slots k (wave ballots, then the four waves)
+    __shared__ uint32_t s_cnt[SH_CACHE][RB / 64];
+    const uint32_t wid = threadIdx.x / 64;
+    const unsigned long long ltm = lt_mask();
+    uint32_t before[SH_CACHE];
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < SH_CACHE; ++k) {
+            const unsigned long long m = __ballot(k < S && ((vm >> k) & 1u));
+            before[k] = (uint32_t)__popcll(m & ltm);
+            if (__lane_id() == 0) s_cnt[k][wid] = (uint32_t)__popcll(m);
+        }
+    }
     uint32_t tot;
-    const uint32_t local = block_exclusive_scan(cnt, &tot);
+    const uint32_t local = block_exclusive_scan(cnt, &tot);  // also orders the s_cnt writes
+    uint32_t run = 0;  // valid slots of earlier k in this block
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < SH_CACHE; ++k) {
+            uint32_t wb = 0, bt = 0;
+            for (uint32_t w2 = 0; w2 < RB / 64; ++w2) {
+                const uint32_t c = s_cnt[k][w2];
+                if (w2 < wid) wb += c;
+                bt += c;
+            }
+            before[k] += wb + run;
+            run += bt;
+        }
+    }
     if (threadIdx.x < 64) {
         const uint32_t e = lookback(tiles, bid, tot);
         if (threadIdx.x == 0) {
@@ -371,7 +400,7 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         }
     }
     __syncthreads();
-    uint32_t pos = s_excl + local;
+    uint32_t pos = s_excl + local;  // sample-major order when the slots are not kept
     for (uint32_t k = 0; k < S; ++k) {
         Ray sr;
         f3 c = mk3(0, 0, 0);
@@ -386,7 +415,15 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         }
         uint32_t q = NONE;
         if (valid) {
-            q = pos++;
+            if (cached) {
+                uint32_t b = 0;
+#pragma unroll
+                for (uint32_t kk = 0; kk < SH_CACHE; ++kk)
+                    if (kk == k) b = before[kk];
+                q = s_excl + b;
+            } else {
+                q = pos++;
+            }
             sh[q] = sr;
             contrib[q] = make_float4(c.x, c.y, c.z, 0.0f);
         }
