@@ -182,38 +182,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return before + x - v;
 }
 
-// Offsets of per-block counts: one block of SCAN_T threads, each summing a
-// contiguous run, then a block-wide scan of the run sums.  offsets[b] =
-// sum of counts[0..b); *total = sum of all.
-constexpr int SCAN_T = 1024;
-__global__ __launch_bounds__(SCAN_T) void k_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets,
-                                                        uint32_t* total) {
-    __shared__ uint32_t wsum[SCAN_T / 64];
-    const uint32_t t = threadIdx.x, lane = __lane_id(), wid = t / 64;
-    const uint32_t per = (nb + SCAN_T - 1) / SCAN_T;
-    const uint32_t lo = t * per, hi = (lo + per < nb) ? lo + per : nb;
-    uint32_t s = 0;
-    for (uint32_t b = lo; b < hi; ++b) s += counts[b];
-    uint32_t x = s;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-    for (uint32_t w = 0; w < SCAN_T / 64; ++w) {
-        const uint32_t v = wsum[w];
-        if (w < wid) before += v;
-        tot += v;
-    }
-    uint32_t run = before + x - s;
-    for (uint32_t b = lo; b < hi; ++b) {
-        offsets[b] = run;
-        run += counts[b];
-    }
-    if (t == 0) *total = tot;
-}
 
 // Decoupled look-back (D. Merrill and M. Garland, "Single-pass Parallel
 // Prefix Scan with Decoupled Look-back", NVIDIA technical report
@@ -262,9 +230,10 @@ __device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t t
 // block writes their number.  With at most SH_CACHE slots per sample the
 // light samples are kept in registers between the count and the write.
 enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_QUEUE = 1 };
-// The path tracer's vertex kernel fills two queues (shadow and next rays):
-// a count pass, a scan of the per-block counts, then a write pass.
-enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_COUNT = 1, PSHADE_WRITE = 2 };
+// The path tracer's vertex kernel: PSHADE_PLAIN writes every path's slot,
+// PSHADE_QUEUE fills two dense queues (shadow and next rays) in queue order
+// with a decoupled look-back per queue.
+enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_QUEUE = 1 };
 constexpr uint32_t SH_CACHE = 4;
 
 __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
@@ -489,14 +458,28 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                                                    Ray* rays_out, uint32_t* pid_out, uint32_t* out_count,
                                                    Ray* sh, uint32_t* sh_pid, float4* shc,
                                                    uint32_t* sh_count, float4* sample_l,
-                                                   uint32_t* bc, const uint32_t* bo, uint32_t nblocks,
-                                                   int mode) {
-    // bc / bo: per-block counts / offsets, shadow rays at [0, nblocks), next
-    // rays at [nblocks, 2*nblocks)
+                                                   unsigned long long* tiles_s, unsigned long long* tiles_n,
+                                                   uint32_t* ticket, int mode) {
     const uint32_t n = q_count ? *q_count : cap;
     const bool queued = pid_q != nullptr;
-    {
-        const uint32_t base = blockIdx.x * RB;
+    __shared__ uint32_t s_bid, s_es, s_en;
+    // PSHADE_QUEUE: persistent blocks take the tiles of RB paths that the
+    // queue's count covers, in ticket order (the look-back's progress
+    // guarantee); PSHADE_PLAIN: one tile per block.
+    const uint32_t ntiles = (n + RB - 1) / RB;
+    for (uint32_t round = 0;; ++round) {
+        uint32_t bid;
+        if (mode == PSHADE_QUEUE) {
+            if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
+            __syncthreads();
+            bid = s_bid;
+            __syncthreads();  // every thread has read s_bid before the next round
+            if (bid >= ntiles) break;
+        } else {
+            if (round > 0) break;
+            bid = blockIdx.x;
+        }
+        const uint32_t base = bid * RB;
         const uint32_t i = base + threadIdx.x;
         const bool live = i < n;
         uint32_t pid = live ? (queued ? pid_q[i] : i) : 0u;
@@ -590,13 +573,6 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
             want_next = cont;
             finish = !cont;
         }
-        if (mode == PSHADE_COUNT) {
-            uint32_t ts, tn;
-            block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
-            block_exclusive_scan(want_next ? 1u : 0u, &tn);
-            if (threadIdx.x == 0) { bc[blockIdx.x] = ts; bc[nblocks + blockIdx.x] = tn; }
-            return;
-        }
         // paths that end with no shadow ray pending are final now
         if (live && active) {
             if (finish && !want_shadow) {
@@ -610,8 +586,18 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         uint32_t ts, tn;
         const uint32_t rs = block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
         const uint32_t rn = block_exclusive_scan(want_next ? 1u : 0u, &tn);
-        if (mode == PSHADE_WRITE) {
-            const uint32_t qs = bo[blockIdx.x] + rs, qn = bo[nblocks + blockIdx.x] + rn;
+        if (mode == PSHADE_QUEUE) {
+            if (threadIdx.x < 64) {
+                const uint32_t es = lookback(tiles_s, bid, ts);
+                const uint32_t en = lookback(tiles_n, bid, tn);
+                if (threadIdx.x == 0) {
+                    s_es = es;
+                    s_en = en;
+                    if (bid == ntiles - 1) { *sh_count = es + ts; *out_count = en + tn; }  // the last tile
+                }
+            }
+            __syncthreads();
+            const uint32_t qs = s_es + rs, qn = s_en + rn;
             if (want_shadow) { sh[qs] = sray; sh_pid[qs] = pid; shc[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f); }
             if (want_next) { rays_out[qn] = nray; pid_out[qn] = pid; }
         } else {
@@ -888,7 +874,7 @@ struct Renderer {
     DevArr<Ray> rays, rays2, sh;
     DevArr<Hit> hits;
     DevArr<float4> contrib, lbase, sample_l;
-    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters, bcounts, boffsets;
+    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters, bcounts;
     DevArr<PathState> st;
     DevArr<float> rgb;
     hipStream_t stream = nullptr;
@@ -1143,27 +1129,20 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 RCHK(fill_u32(cnt + 5, 0u, 1, st));
                 {
                     const uint32_t nb = (n + RB - 1) / RB;
-                    ICHK(R->bcounts.ensure(2 * (size_t)nb));
-                    ICHK(R->boffsets.ensure(2 * (size_t)nb));
-                    if (batched) {
-                        hipLaunchKernelGGL(k_path_shade, dim3(nb), dim3(RB), 0, st, P, g0, bounce, n,
-                                           (const uint32_t*)cin, (const uint32_t*)pin, rin, R->hits.p, R->st.p, rout,
-                                           pout, cout, R->sh.p, R->shpid.p, R->contrib.p, cnt + 5, R->sample_l.p,
-                                           R->bcounts.p, (const uint32_t*)R->boffsets.p, nb, (int)PSHADE_COUNT);
-                        RCHK(hipGetLastError());
-                        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p, nb,
-                                           R->boffsets.p, cnt + 5);
-                        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_T), 0, st, R->bcounts.p + nb, nb,
-                                           R->boffsets.p + nb, cout);
-                        RCHK(hipGetLastError());
-                    }
-                    hipLaunchKernelGGL(k_path_shade, dim3(nb), dim3(RB), 0, st, P, g0, bounce, n,
+                    // look-back tiles of the two queues (two words per block
+                    // each) and the block ticket
+                    ICHK(R->bcounts.ensure(4 * (size_t)nb + 2));
+                    if (batched) RCHK(fill_u32(R->bcounts.p, 0u, 4 * (size_t)nb + 2, st));
+                    unsigned long long* tiles = (unsigned long long*)R->bcounts.p;
+                    // queue form: persistent blocks over the live paths' tiles
+                    const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
+                    hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(RB), 0, st, P, g0, bounce, n,
                                        batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
                                        batched ? (const uint32_t*)pin : (const uint32_t*)nullptr, rin, R->hits.p,
                                        R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
-                                       batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p,
-                                       R->bcounts.p, (const uint32_t*)R->boffsets.p, nb,
-                                       (int)(batched ? PSHADE_WRITE : PSHADE_PLAIN));
+                                       batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p, tiles,
+                                       tiles + nb, R->bcounts.p + 4 * (size_t)nb,
+                                       (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
                     RCHK(hipGetLastError());
                 }
                 RCHK(hipEventRecord(b, st));
@@ -1303,7 +1282,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.rays.release(); R.rays2.release(); R.sh.release(); R.hits.release(); R.contrib.release();
     R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
     R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
-    R.bcounts.release(); R.boffsets.release();
+    R.bcounts.release();
     delete h;
     return PBRTX_OK;
 }
