@@ -461,7 +461,11 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                                                    unsigned long long* tiles_s, unsigned long long* tiles_n,
                                                    uint32_t* ticket, int mode) {
     const uint32_t n = q_count ? *q_count : cap;
-    const bool queued = pid_q != nullptr;
+    const bool queued = mode == PSHADE_QUEUE;
+    // fresh (rays_in null): the first vertex of new paths whose camera rays
+    // the traversal made itself -- path i, initial state, the camera ray made
+    // again here
+    const bool fresh = rays_in == nullptr;
     __shared__ uint32_t s_bid, s_es, s_en;
     // PSHADE_QUEUE: persistent blocks take the tiles of RB paths that the
     // queue's count covers, in ticket order (the look-back's progress
@@ -482,11 +486,12 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         const uint32_t base = bid * RB;
         const uint32_t i = base + threadIdx.x;
         const bool live = i < n;
-        uint32_t pid = live ? (queued ? pid_q[i] : i) : 0u;
+        uint32_t pid = live ? ((queued && !fresh) ? pid_q[i] : i) : 0u;
         PathState ps{};
         bool active = false;
         if (live) {
-            ps = st[pid];
+            if (fresh) ps = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
+            else ps = st[pid];
             active = queued || !(ps.flags & 4u);  // plain form: bit 4 marks a finished path
         }
         Hit h = active ? hits[i] : Hit{NO_HIT, 0, 0, 0};
@@ -496,7 +501,9 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         if (active && h.prim == NO_HIT) {
             finish = true;  // a miss ends the path (no infinite lights)
         } else if (active) {
-            const Ray r = rays_in[i];
+            Ray r;
+            if (fresh) camera_ray_of(P, g0 + i, &r);
+            else r = rays_in[i];
             const f3 d = mk3(r.dx, r.dy, r.dz);
             const f3 p = add3(mk3(r.ox, r.oy, r.oz), mul3(d, h.t));
             const FaceShade fs = P.faces[h.prim];
@@ -1101,10 +1108,17 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             ICHK(R->pid2.ensure(cap));
             ICHK(R->shpid.ensure(cap));
             hipEvent_t a = ev(), b = ev();
+            // batched: the packet kernel makes the camera rays and the first
+            // vertex makes them again and starts the paths (no init pass)
+            const bool fused = batched && camera_fusable(sv, tunCam);
             RCHK(hipEventRecord(a, st));
-            hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->st.p,
-                               batched ? R->pid.p : nullptr, batched ? cnt + 1 : nullptr);
-            RCHK(hipGetLastError());
+            if (fused) {
+                RCHK(fill_u32(cnt + 1, n, 1, st));
+            } else {
+                hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->st.p,
+                                   batched ? R->pid.p : nullptr, batched ? cnt + 1 : nullptr);
+                RCHK(hipGetLastError());
+            }
             RCHK(hipEventRecord(b, st));
             T.other.push_back({a, b});
             Ray* rin = R->rays.p;
@@ -1117,10 +1131,19 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 a = ev(); b = ev();
                 RCHK(hipEventRecord(a, st));
                 const TraceTuning& tb = bounce == 0 ? tunCam : tunC;  // bounce 0: camera rays
-                if (batched)
+                const bool gen = fused && bounce == 0;
+                if (gen) {
+                    CamParams cp;
+                    memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
+                    memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
+                    cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
+                    cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
+                    RCHK(trace_camera(sv, cp, n, R->hits.p, tb, scr, st));
+                } else if (batched) {
                     RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st, cin));
-                else
+                } else {
                     RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st));
+                }
                 RCHK(hipEventRecord(b, st));
                 T.closest.push_back({a, b});
                 a = ev(); b = ev();
@@ -1138,7 +1161,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                     const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
                     hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(RB), 0, st, P, g0, bounce, n,
                                        batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
-                                       batched ? (const uint32_t*)pin : (const uint32_t*)nullptr, rin, R->hits.p,
+                                       batched ? (const uint32_t*)pin : (const uint32_t*)nullptr,
+                                       (fused && bounce == 0) ? (const Ray*)nullptr : rin, R->hits.p,
                                        R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
                                        batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p, tiles,
                                        tiles + nb, R->bcounts.p + 4 * (size_t)nb,
