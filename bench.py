@@ -45,6 +45,9 @@ def parse():
                     help="bounded CPU baseline sample (also the source of the per-ray work figures)")
     ap.add_argument("--tuning", default=None, help="autotuner JSON to apply (pbrt_rust_amd.autotune)")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving measurement")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the ray-query and path-tracing measurements")
+    ap.add_argument("--pt-spp-side", type=int, default=8, help="path-tracing workload: sqrt(spp)")
     ap.add_argument("--serve-pool", type=int, default=48, help="distinct requests in the serving pool")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests in the fixed-rate run")
     ap.add_argument("--serve-batch", type=int, default=1 << 22, help="rays per serving step")
@@ -124,6 +127,9 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serving(args, px, scenes, lambda m: log(args, m))
+    workloads = None
+    if not args.no_workloads:
+        workloads = other_workloads(args, px, scenes, lambda m: log(args, m))
 
     # ---- roofline of the dominant kernel (per-launch averages, HIP events on
     # the render stream) ----
@@ -196,11 +202,62 @@ def main():
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "serving": serve,
+        "workloads": workloads,
         "cpu_baseline": cpu,
         "device": px.device_name(),
     }
     print(json.dumps(out), flush=True)
     r.close()
+
+
+def frame_rate(px, r, path, steps, warmup):
+    """ms per frame (HIP events on the render stream) and rays per frame"""
+    for _ in range(warmup):
+        r.render(path, want_image=False)
+    sts = [r.render(path, want_image=False)[1] for _ in range(steps)]
+    st = sts[-1]
+    rays = st["camera_rays"] + st["shadow_rays"] + st["extension_rays"]
+    ms = float(np.mean([s["ms_total"] for s in sts]))
+    return ms, rays, st
+
+
+def other_workloads(args, px, scenes, log_fn):
+    """The ray-query workload (16,384 random triangles, 512x512 pinhole, one
+    jittered sample, closest hit then one shadow segment to the point light
+    at (2,2,-2)) and the path-tracing workload (64 level-6 icospheres, 5.2M
+    triangles, 70% Lambertian / 30% mirror, quad light, 1920x1080, 64 spp,
+    depth 5, roulette after bounce 3), batched and plain."""
+    out = {}
+    sc = scenes.ray_query_scene(16384)
+    cam = scenes.pinhole(512, 512)
+    r = px.SamplerRenderer(sc, cam, px.StratifiedSampler(1, 1, 0xC0), px.DirectLightingIntegrator(1), split="sah")
+    msb, rays, st = frame_rate(px, r, "batched", 20, 3)
+    msp, _, _ = frame_rate(px, r, "plain", 20, 3)
+    r.close()
+    out["ray_queries"] = {"workload": "16,384 random triangles, 512x512, 1 spp, closest hit + point-light shadow rays",
+                          "rays_per_frame": int(rays), "camera_rays": int(st["camera_rays"]),
+                          "shadow_rays": int(st["shadow_rays"]),
+                          "batched_ms": round(msb, 4), "plain_ms": round(msp, 4),
+                          "batched_mrays_per_s": round(rays / msb / 1e3, 1),
+                          "plain_mrays_per_s": round(rays / msp / 1e3, 1), "batched_over_plain": round(msp / msb, 3)}
+    log_fn("ray queries %s" % json.dumps(out["ray_queries"]))
+    side = args.pt_spp_side
+    sc = scenes.spheres_scene(n_spheres=64, level=6, seed=2)
+    cam = scenes.pinhole(1920, 1080, fov=60.0, cam_pos=(0.0, 0.0, -25.0))
+    n = 1920 * 1080 * side * side
+    r = px.SamplerRenderer(sc, cam, px.StratifiedSampler(side, side, 2), px.PathIntegrator(5, 3), chunk=n)
+    msb, rays, st = frame_rate(px, r, "batched", 2, 1)
+    msp, _, _ = frame_rate(px, r, "plain", 1, 1)
+    r.close()
+    out["path_tracing"] = {"workload": "64 level-6 icospheres (%d tris), 1920x1080, %d spp, depth 5, roulette after "
+                                       "bounce 3" % (int(sc["idx"].shape[0]), side * side),
+                           "rays_per_frame": int(rays), "camera_rays": int(st["camera_rays"]),
+                           "shadow_rays": int(st["shadow_rays"]), "extension_rays": int(st["extension_rays"]),
+                           "batched_ms": round(msb, 3), "plain_ms": round(msp, 3),
+                           "batched_mrays_per_s": round(rays / msb / 1e3, 1),
+                           "plain_mrays_per_s": round(rays / msp / 1e3, 1), "batched_over_plain": round(msp / msb, 3)}
+    log_fn("path tracing %s" % json.dumps(out["path_tracing"]))
+    return out
 
 
 def serving(args, px, scenes, log_fn):
