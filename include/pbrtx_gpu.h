@@ -152,6 +152,11 @@ typedef struct pbrtx_renderer pbrtx_renderer;
 int pbrtx_renderer_create(const pbrtx_render_desc* desc, pbrtx_renderer** out);
 /* samples per work chunk (whole pixels; bounds the per-chunk buffers) */
 int pbrtx_renderer_set_chunk(pbrtx_renderer* r, uint32_t samples_per_chunk);
+/* graph mode (enable != 0): after one ordinary frame of a path, a frame's
+   device work is captured once as a hipGraph and replayed (captured again
+   after a tuning change); per-stage times are then not measured (ms_total
+   is).  Cuts the host launch cost of small frames. */
+int pbrtx_renderer_set_graph(pbrtx_renderer* r, int32_t enable);
 /* renders one frame with the given traversal path; rgb_out (host, may be
    NULL) receives xres*yres*3 floats: the film's final rgb */
 int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);

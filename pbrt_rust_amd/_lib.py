@@ -123,6 +123,7 @@ def lib():
         L.pbrtx_renderer_destroy.argtypes = [vp]
         L.pbrtx_renderer_sample_radiance.argtypes = [vp, vp, u64]
         L.pbrtx_renderer_set_chunk.argtypes = [vp, u32]
+        L.pbrtx_renderer_set_graph.argtypes = [vp, i32]
     _lib = L
     return L
 

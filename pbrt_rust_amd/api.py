@@ -484,6 +484,11 @@ class SamplerRenderer:
         if chunk:
             check(lib().pbrtx_renderer_set_chunk(self._h, int(chunk)))
 
+    def set_graph(self, enable=True):
+        """Graph mode: after one ordinary frame of a path, frames are replayed
+        from a captured hipGraph (per-stage times are then not measured)."""
+        check(lib().pbrtx_renderer_set_graph(self._h, 1 if enable else 0))
+
     def render(self, path="batched", want_image=True):
         from ._lib import RenderStats
         st = RenderStats()

@@ -30,6 +30,7 @@ int hip_fail(hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? PBRTX_ERR_OOM : PBRTX_ERR_HIP;
 }
 
+uint32_t g_tuning_gen = 1;
 // [path][kind]: closest hit, occlusion, camera rays of the renderer (coherent:
 // the batched path walks them as packets)
 TraceTuning g_tuning[2][3] = {
@@ -281,6 +282,7 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
         return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
     if (scheme == SCHEME_PACKET && path != 1) return fail(PBRTX_ERR_ARG, "the packet scheme exists for the batched path only");
     g_tuning[path][kind] = TraceTuning{block, blocks_per_cu, refill_pct, scheme};
+    ++g_tuning_gen;  // captured frames are captured again
     return PBRTX_OK;
 }
 

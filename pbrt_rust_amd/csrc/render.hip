@@ -888,6 +888,23 @@ struct Renderer {
     EventPool events;
     int integrator = 0;
     double build_ms = 0.0;
+    // graph mode, per path: warm after one eager frame; the captured frame
+    // and the tuning generation it was captured under
+    bool use_graph = false;
+    bool warm[2] = {false, false};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint32_t gen[2] = {0, 0};
+    uint64_t gcamera[2] = {0, 0};
+    void drop_graphs() {
+        for (int k = 0; k < 2; ++k) {
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            gexec[k] = nullptr;
+            graph[k] = nullptr;
+            warm[k] = false;
+        }
+    }
 };
 
 static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
@@ -901,6 +918,12 @@ static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
     std::string err;
     if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, nullptr, 0, d->split, d->max_prims, &R->bvh, &err))
         return fail(PBRTX_ERR_BUILD, err);
+    // the renderer's own stream (default priority; a frame ends by waiting
+    // for its last event, so host reads after a frame see its results)
+    {
+        hipError_t e = hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    }
     R->build_ms = R->bvh.build_ms;
     int rc = scene_from_host(R->bvh, &R->scene);
     if (rc) return rc;
@@ -990,7 +1013,10 @@ struct Timing {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, any, other;
 };
 
-static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
+// Enqueues one frame's device work on the renderer's stream: every kernel and
+// fill from the counter reset to the film, nothing that waits on the host, so
+// it can be captured as a graph.  timed: record per-stage events into T.
+static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bool timed, uint64_t* camera_rays) {
     RenderParams& P = R->P;
     hipStream_t st = R->stream;
     const uint32_t spp = P.spp;
@@ -1007,20 +1033,21 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     const TraceTuning& tunC = g_tuning[path][0];
     const TraceTuning& tunA = g_tuning[path][1];
     const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
-    TraceScratch* scr;
-    {
-        const size_t a = trace_spill_bytes(R->scene.view, path, tunC);
-        const size_t b = trace_spill_bytes(R->scene.view, path, tunA);
-        const size_t c = trace_spill_bytes(R->scene.view, path, tunCam);
-        ICHK(stream_scratch(st, std::max(a, std::max(b, c)), &scr));
-    }
-    R->events.used = 0;
-    Timing T;
-    auto ev = [&]() { return R->events.next(); };
+    auto stage_begin = [&]() -> hipEvent_t {
+        if (!timed) return nullptr;
+        hipEvent_t e = R->events.next();
+        return (e && hipEventRecord(e, st) == hipSuccess) ? e : nullptr;
+    };
+    auto stage_end = [&](hipEvent_t a, std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) -> hipError_t {
+        if (!timed || !a) return hipSuccess;
+        hipEvent_t b = R->events.next();
+        if (!b) return hipErrorOutOfMemory;
+        hipError_t e = hipEventRecord(b, st);
+        if (e == hipSuccess) v.push_back({a, b});
+        return e;
+    };
     uint32_t* cnt = R->counters.p;  // [0] shadow, [1] ext/next A, [2] next B, [3] camera, [4..] totals
     RCHK(fill_u32(cnt, 0u, 16, st));
-    hipEvent_t e_start = ev();
-    RCHK(hipEventRecord(e_start, st));
     const SceneView& sv = R->scene.view;
     uint64_t total_camera = 0;
     for (uint64_t g0 = 0; g0 < R->n_samples; g0 += per) {
@@ -1033,21 +1060,18 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             ICHK(R->qidx.ensure(slots ? slots : 1));
             ICHK(R->lbase.ensure(n));
             ICHK(R->occ.ensure((slots + 31) / 32 + 1));
-            hipEvent_t a, b;
+            hipEvent_t a;
             // batched: the packet kernel makes the camera rays itself and the
             // shade kernel makes them again (rays = null): no ray array
             const bool fused = batched && camera_fusable(sv, tunCam);
             Ray* crays = fused ? nullptr : R->rays.p;
             if (!fused) {
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                a = stage_begin();
                 hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p);
                 RCHK(hipGetLastError());
-                RCHK(hipEventRecord(b, st));
-                T.other.push_back({a, b});
+                RCHK(stage_end(a, T.other));
             }
-            a = ev(); b = ev();
-            RCHK(hipEventRecord(a, st));
+            a = stage_begin();
             if (fused) {
                 CamParams cp;
                 memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
@@ -1058,10 +1082,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             } else {
                 RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunCam, scr, st));
             }
-            RCHK(hipEventRecord(b, st));
-            T.closest.push_back({a, b});
-            a = ev(); b = ev();
-            RCHK(hipEventRecord(a, st));
+            RCHK(stage_end(a, T.closest));
+            a = stage_begin();
             RCHK(fill_u32(cnt + 5, 0u, 1, st));
             {
                 const uint32_t nb = (n + RB - 1) / RB;
@@ -1076,28 +1098,23 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                                    (int)(batched ? SHADE_QUEUE : SHADE_PLAIN));
                 RCHK(hipGetLastError());
             }
-            RCHK(hipEventRecord(b, st));
-            T.other.push_back({a, b});
+            RCHK(stage_end(a, T.other));
             if (slots) {
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                a = stage_begin();
                 if (batched)
                     RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st, cnt + 5));
                 else
                     RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st));
-                RCHK(hipEventRecord(b, st));
-                T.any.push_back({a, b});
+                RCHK(stage_end(a, T.any));
             }
-            a = ev(); b = ev();
-            RCHK(hipEventRecord(a, st));
+            a = stage_begin();
             hipLaunchKernelGGL(k_accum_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->lbase.p,
                                R->contrib.p, R->qidx.p, R->occ.p, R->sample_l.p);
             RCHK(hipGetLastError());
             // running total of shadow rays
             hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
             RCHK(hipGetLastError());
-            RCHK(hipEventRecord(b, st));
-            T.other.push_back({a, b});
+            RCHK(stage_end(a, T.other));
         } else {
             ICHK(R->st.ensure(cap));
             ICHK(R->rays2.ensure(cap));
@@ -1107,11 +1124,10 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             ICHK(R->pid.ensure(cap));
             ICHK(R->pid2.ensure(cap));
             ICHK(R->shpid.ensure(cap));
-            hipEvent_t a = ev(), b = ev();
             // batched: the packet kernel makes the camera rays and the first
             // vertex makes them again and starts the paths (no init pass)
             const bool fused = batched && camera_fusable(sv, tunCam);
-            RCHK(hipEventRecord(a, st));
+            hipEvent_t a = stage_begin();
             if (fused) {
                 RCHK(fill_u32(cnt + 1, n, 1, st));
             } else {
@@ -1119,8 +1135,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                                    batched ? R->pid.p : nullptr, batched ? cnt + 1 : nullptr);
                 RCHK(hipGetLastError());
             }
-            RCHK(hipEventRecord(b, st));
-            T.other.push_back({a, b});
+            RCHK(stage_end(a, T.other));
             Ray* rin = R->rays.p;
             Ray* rout = R->rays2.p;
             uint32_t* pin = R->pid.p;
@@ -1128,8 +1143,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
             uint32_t* cin = cnt + 1;
             uint32_t* cout = cnt + 2;
             for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                a = stage_begin();
                 const TraceTuning& tb = bounce == 0 ? tunCam : tunC;  // bounce 0: camera rays
                 const bool gen = fused && bounce == 0;
                 if (gen) {
@@ -1144,10 +1158,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 } else {
                     RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st));
                 }
-                RCHK(hipEventRecord(b, st));
-                T.closest.push_back({a, b});
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                RCHK(stage_end(a, T.closest));
+                a = stage_begin();
                 RCHK(fill_u32(cout, 0u, 1, st));
                 RCHK(fill_u32(cnt + 5, 0u, 1, st));
                 {
@@ -1169,18 +1181,14 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                                        (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
                     RCHK(hipGetLastError());
                 }
-                RCHK(hipEventRecord(b, st));
-                T.other.push_back({a, b});
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                RCHK(stage_end(a, T.other));
+                a = stage_begin();
                 if (batched)
                     RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st, cnt + 5));
                 else
                     RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st));
-                RCHK(hipEventRecord(b, st));
-                T.any.push_back({a, b});
-                a = ev(); b = ev();
-                RCHK(hipEventRecord(a, st));
+                RCHK(stage_end(a, T.any));
+                a = stage_begin();
                 hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
                                    batched ? cnt + 5 : (const uint32_t*)nullptr,
                                    batched ? R->shpid.p : (const uint32_t*)nullptr, R->contrib.p, R->occ.p, R->st.p,
@@ -1190,8 +1198,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
                 if (bounce < P.max_depth)
                     hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
                 RCHK(hipGetLastError());
-                RCHK(hipEventRecord(b, st));
-                T.other.push_back({a, b});
+                RCHK(stage_end(a, T.other));
                 std::swap(rin, rout);
                 std::swap(pin, pout);
                 std::swap(cin, cout);
@@ -1200,17 +1207,65 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     }
     ICHK(R->rgb.ensure((size_t)P.xres * P.yres * 3));
     {
-        hipEvent_t a = ev(), b = ev();
-        RCHK(hipEventRecord(a, st));
+        hipEvent_t a = stage_begin();
         hipLaunchKernelGGL(k_film, dim3(grid_for((size_t)P.xres * P.yres)), dim3(RB), 0, st, P, R->sample_l.p,
                            R->rgb.p);
         RCHK(hipGetLastError());
-        RCHK(hipEventRecord(b, st));
-        T.other.push_back({a, b});
+        RCHK(stage_end(a, T.other));
     }
-    hipEvent_t e_end = ev();
+    *camera_rays = total_camera;
+    return PBRTX_OK;
+}
+
+// One frame: eagerly with per-stage timing, or -- graph mode, after one eager
+// frame of that path has sized every buffer -- as a captured graph replayed
+// (captured again when the launch tuning changed since).
+static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
+    hipStream_t st = R->stream;
+    const TraceTuning& tunC = g_tuning[path][0];
+    const TraceTuning& tunA = g_tuning[path][1];
+    const TraceTuning& tunCam = g_tuning[path][2];
+    TraceScratch* scr;
+    {
+        const size_t a = trace_spill_bytes(R->scene.view, path, tunC);
+        const size_t b = trace_spill_bytes(R->scene.view, path, tunA);
+        const size_t c = trace_spill_bytes(R->scene.view, path, tunCam);
+        ICHK(stream_scratch(st, std::max(a, std::max(b, c)), &scr));
+    }
+    R->events.used = 0;
+    Timing T;
+    uint64_t total_camera = 0;
+    hipEvent_t e_start = R->events.next(), e_end = R->events.next();
+    if (!e_start || !e_end) return fail(PBRTX_ERR_OOM, "events");
+    const bool replay = R->use_graph && R->warm[path];
+    if (replay && (!R->gexec[path] || R->gen[path] != g_tuning_gen)) {
+        if (R->gexec[path]) { (void)hipGraphExecDestroy(R->gexec[path]); R->gexec[path] = nullptr; }
+        if (R->graph[path]) { (void)hipGraphDestroy(R->graph[path]); R->graph[path] = nullptr; }
+        RCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        Timing none;
+        const int rc = enqueue_frame(R, path, scr, none, false, &R->gcamera[path]);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        RCHK(ec);
+        R->graph[path] = g;
+        RCHK(hipGraphInstantiate(&R->gexec[path], g, nullptr, nullptr, 0));
+        R->gen[path] = g_tuning_gen;
+    }
+    RCHK(hipEventRecord(e_start, st));
+    if (replay) {
+        RCHK(hipGraphLaunch(R->gexec[path], st));
+        total_camera = R->gcamera[path];
+    } else {
+        ICHK(enqueue_frame(R, path, scr, T, true, &total_camera));
+        R->warm[path] = true;
+    }
     RCHK(hipEventRecord(e_end, st));
     RCHK(hipEventSynchronize(e_end));
+    uint32_t* cnt = R->counters.p;
     if (stats) {
         float ms = 0.0f;
         RCHK(hipEventElapsedTime(&ms, e_start, e_end));
@@ -1267,6 +1322,14 @@ int pbrtx_renderer_create(const pbrtx_render_desc* desc, pbrtx_renderer** out) {
 int pbrtx_renderer_set_chunk(pbrtx_renderer* h, uint32_t samples_per_chunk) {
     if (!h || samples_per_chunk == 0) return fail(PBRTX_ERR_ARG, "bad chunk");
     h->r.chunk = samples_per_chunk;
+    h->r.drop_graphs();  // the frame's shape changed
+    return PBRTX_OK;
+}
+
+int pbrtx_renderer_set_graph(pbrtx_renderer* h, int32_t enable) {
+    if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
+    h->r.use_graph = enable != 0;
+    if (!enable) h->r.drop_graphs();
     return PBRTX_OK;
 }
 
@@ -1300,6 +1363,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     if (!h) return PBRTX_OK;
     Renderer& R = h->r;
     (void)hipDeviceSynchronize();
+    R.drop_graphs();
     scene_free(&R.scene);
     R.faces.release(); R.mats.release(); R.atris.release(); R.a_first.release(); R.a_count.release();
     R.a_total.release(); R.a_le.release(); R.points.release();
@@ -1307,6 +1371,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
     R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
     R.bcounts.release();
+    if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;
 }

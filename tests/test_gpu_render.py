@@ -123,3 +123,43 @@ def test_camera_ray_schemes_give_the_same_frame():
             assert_same(ref, img, samples)
     finally:
         px.set_tuning("batched", "camera", 128, 8, 0, "packet")
+
+
+@pytest.mark.parametrize("integ", [DIRECT, PATH], ids=["direct", "path"])
+def test_graph_replays_match_eager_frames(integ):
+    # graph mode: every replay, with other eager work and synchronizations in
+    # between and a tuning change (recapture) halfway, gives the eager frame
+    if integ["kind"] == "path":
+        scene = scenes.spheres_scene(n_spheres=8, level=3, seed=5)
+        cam = scenes.pinhole(48, 32, fov=60.0, cam_pos=(0.0, 0.0, -25.0))
+    else:
+        scene = scenes.direct_lighting_scene(level=4)
+        cam = scenes.pinhole(64, 48)
+    ref = oracle_render(scene, cam, integ)
+    other = px.SamplerRenderer(scenes.direct_lighting_scene(level=3), scenes.pinhole(32, 24),
+                               px.StratifiedSampler(2, 2, 3), px.DirectLightingIntegrator(2))
+    sampler = px.StratifiedSampler(integ["nx"], integ["ny"], integ["seed"])
+    it = px.PathIntegrator(integ["max_depth"], integ["rr_start"]) if integ["kind"] == "path" \
+        else px.DirectLightingIntegrator(integ["light_samples"])
+    r = px.SamplerRenderer(scene, cam, sampler, it)
+    spp = integ["nx"] * integ["ny"]
+    try:
+        for path in ("batched", "plain"):
+            eager, _ = r.render(path)
+            r.set_graph(True)
+            for k in range(4):
+                if k == 2:
+                    px.set_tuning("batched", "any", 128, 0, 10, "ray")
+                img, st = r.render(path)
+                samples = r.sample_radiance().reshape(cam["yres"], cam["xres"], spp, 3)
+                assert np.array_equal(img.view(np.uint32), eager.view(np.uint32)), (path, k)
+                assert_same(ref, img, samples)
+                assert st["ms_total"] > 0
+                other.render("batched")
+                px.synchronize()
+            r.set_graph(False)
+            px.set_tuning("batched", "any", 256, 0, 10, "ray")
+    finally:
+        px.set_tuning("batched", "any", 256, 0, 10, "ray")
+        r.close()
+        other.close()
