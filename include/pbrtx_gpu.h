@@ -157,6 +157,12 @@ int pbrtx_renderer_set_chunk(pbrtx_renderer* r, uint32_t samples_per_chunk);
    after a tuning change); per-stage times are then not measured (ms_total
    is).  Cuts the host launch cost of small frames. */
 int pbrtx_renderer_set_graph(pbrtx_renderer* r, int32_t enable);
+/* path tracer, batched path (enable != 0): the extension-ray queue of every
+   bounce is octant-sorted (stable, within tiles of 4096 queue entries) before
+   it is traced.  Images are unchanged: a ray's result does not depend on
+   its queue position.  No counterpart in the reference (its rays are traced
+   one at a time); off by default. */
+int pbrtx_renderer_set_octant_sort(pbrtx_renderer* r, int32_t enable);
 /* renders one frame with the given traversal path; rgb_out (host, may be
    NULL) receives xres*yres*3 floats: the film's final rgb */
 int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);

@@ -124,6 +124,7 @@ def lib():
         L.pbrtx_renderer_sample_radiance.argtypes = [vp, vp, u64]
         L.pbrtx_renderer_set_chunk.argtypes = [vp, u32]
         L.pbrtx_renderer_set_graph.argtypes = [vp, i32]
+        L.pbrtx_renderer_set_octant_sort.argtypes = [vp, i32]
     _lib = L
     return L
 

@@ -484,10 +484,20 @@ class SamplerRenderer:
         if chunk:
             check(lib().pbrtx_renderer_set_chunk(self._h, int(chunk)))
 
+    def set_chunk(self, samples):
+        """Samples per work chunk (whole pixels): the capacity of the ray
+        queues and work buffers of one pass over the frame."""
+        check(lib().pbrtx_renderer_set_chunk(self._h, int(samples)))
+
     def set_graph(self, enable=True):
         """Graph mode: after one ordinary frame of a path, frames are replayed
         from a captured hipGraph (per-stage times are then not measured)."""
         check(lib().pbrtx_renderer_set_graph(self._h, 1 if enable else 0))
+
+    def set_octant_sort(self, enable=True):
+        """Path tracer, batched path: octant-sort each bounce's extension-ray
+        queue (stable within tiles of 4096 entries) before tracing it."""
+        check(lib().pbrtx_renderer_set_octant_sort(self._h, 1 if enable else 0))
 
     def render(self, path="batched", want_image=True):
         from ._lib import RenderStats

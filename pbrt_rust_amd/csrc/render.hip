@@ -652,6 +652,83 @@ __global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0,
 // them in place) or finished in the plain form: nothing to do -- every path
 // writes its radiance when it ends.
 
+// Octant sort of the extension-ray queue: a stable partition by direction
+// octant within tiles of OS_TILE consecutive queue entries.  The rays and
+// their path ids are copied to the output queue in (tile, octant, position)
+// order, so a wave of the next traversal takes rays of one octant -- one
+// near-first visiting order, the same direction signs -- wherever a tile
+// holds 64 or more of them, while rays stay near their queue neighbours
+// (nearby pixels).  Octant bit a is the sign bit of d[a] (the sign of 1/d
+// that picks the traversal's near child, -0 included).  Positions depend
+// only on the input order, so the queue order is deterministic.
+constexpr uint32_t OS_BLOCK = 1024, OS_PER = 4, OS_TILE = OS_BLOCK * OS_PER;
+__device__ __forceinline__ uint32_t octant_of(const Ray& r) {
+    return (__float_as_uint(r.dx) >> 31) | ((__float_as_uint(r.dy) >> 31) << 1) | ((__float_as_uint(r.dz) >> 31) << 2);
+}
+__global__ __launch_bounds__(OS_BLOCK) void k_octant_sort(const uint32_t* count, const Ray* rin, const uint32_t* pin,
+                                                          Ray* rout, uint32_t* pout) {
+    const uint32_t n = *count;
+    constexpr uint32_t NW = OS_BLOCK / 64;
+    __shared__ uint32_t s_cnt[OS_PER][NW][8];  // per (round, wave, octant): count, then exclusive offset
+    __shared__ uint32_t s_base[8];
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+    const unsigned long long lt = lt_mask();
+    for (uint32_t tile = blockIdx.x; (uint64_t)tile * OS_TILE < n; tile += gridDim.x) {
+        const uint32_t t0 = tile * OS_TILE;
+        Ray r[OS_PER];
+        uint32_t p[OS_PER], oc[OS_PER], rank[OS_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < OS_PER; ++k) {
+            const uint32_t i = t0 + k * OS_BLOCK + threadIdx.x;
+            oc[k] = 8u;  // past the end: no octant
+            rank[k] = 0u;
+            if (i < n) {
+                r[k] = rin[i];
+                p[k] = pin[i];
+                oc[k] = octant_of(r[k]);
+            }
+#pragma unroll
+            for (uint32_t o = 0; o < 8; ++o) {
+                const unsigned long long m = __ballot(oc[k] == o);
+                if (oc[k] == o) rank[k] = (uint32_t)__popcll(m & lt);
+                if (lane == 0) s_cnt[k][wid][o] = (uint32_t)__popcll(m);
+            }
+        }
+        __syncthreads();
+        // per octant: exclusive offsets over (round, wave), which is input order
+        if (threadIdx.x < 8) {
+            const uint32_t o = threadIdx.x;
+            uint32_t acc = 0;
+            for (uint32_t k = 0; k < OS_PER; ++k)
+                for (uint32_t w = 0; w < NW; ++w) {
+                    const uint32_t c = s_cnt[k][w][o];
+                    s_cnt[k][w][o] = acc;
+                    acc += c;
+                }
+            s_base[o] = acc;  // the octant's total, turned into its base below
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (uint32_t o = 0; o < 8; ++o) {
+                const uint32_t c = s_base[o];
+                s_base[o] = acc;
+                acc += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < OS_PER; ++k) {
+            if (oc[k] < 8u) {
+                const uint32_t dst = t0 + s_base[oc[k]] + s_cnt[k][wid][oc[k]] + rank[k];
+                rout[dst] = r[k];
+                pout[dst] = p[k];
+            }
+        }
+        __syncthreads();  // the tables are reused by the next tile
+    }
+}
+
 // Film: box filter of half-width 0.5 (Filter::mean, filter.rs:89;
 // Film::add_sample film.rs:192-249), gathered per pixel from the samples of
 // the 3x3 neighbourhood in row-major order; write_image's conversion with the
@@ -878,16 +955,17 @@ struct Renderer {
     DevArr<uint32_t> a_first, a_count;
     DevArr<float> a_total, a_le, points;
     // per-chunk work buffers
-    DevArr<Ray> rays, rays2, sh;
+    DevArr<Ray> rays, rays2, rays3, sh;
     DevArr<Hit> hits;
     DevArr<float4> contrib, lbase, sample_l;
-    DevArr<uint32_t> qidx, occ, pid, pid2, shpid, counters, bcounts;
+    DevArr<uint32_t> qidx, occ, pid, pid2, pid3, shpid, counters, bcounts;
     DevArr<PathState> st;
     DevArr<float> rgb;
     hipStream_t stream = nullptr;
     EventPool events;
     int integrator = 0;
     double build_ms = 0.0;
+    bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
     // graph mode, per path: warm after one eager frame; the captured frame
     // and the tuning generation it was captured under
     bool use_graph = false;
@@ -1124,6 +1202,11 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bo
             ICHK(R->pid.ensure(cap));
             ICHK(R->pid2.ensure(cap));
             ICHK(R->shpid.ensure(cap));
+            const bool osort = batched && R->octant_sort;
+            if (osort) {
+                ICHK(R->rays3.ensure(cap));
+                ICHK(R->pid3.ensure(cap));
+            }
             // batched: the packet kernel makes the camera rays and the first
             // vertex makes them again and starts the paths (no init pass)
             const bool fused = batched && camera_fusable(sv, tunCam);
@@ -1140,6 +1223,8 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bo
             Ray* rout = R->rays2.p;
             uint32_t* pin = R->pid.p;
             uint32_t* pout = R->pid2.p;
+            Ray* rspare = R->rays3.p;
+            uint32_t* pspare = R->pid3.p;
             uint32_t* cin = cnt + 1;
             uint32_t* cout = cnt + 2;
             for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
@@ -1198,6 +1283,15 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bo
                 if (bounce < P.max_depth)
                     hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
                 RCHK(hipGetLastError());
+                if (osort && bounce < P.max_depth) {
+                    const uint32_t tiles = (n + OS_TILE - 1) / OS_TILE;
+                    const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)device_cu_count() * 2u);
+                    hipLaunchKernelGGL(k_octant_sort, dim3(grid), dim3(OS_BLOCK), 0, st, (const uint32_t*)cout,
+                                       (const Ray*)rout, (const uint32_t*)pout, rspare, pspare);
+                    RCHK(hipGetLastError());
+                    std::swap(rout, rspare);
+                    std::swap(pout, pspare);
+                }
                 RCHK(stage_end(a, T.other));
                 std::swap(rin, rout);
                 std::swap(pin, pout);
@@ -1326,6 +1420,13 @@ int pbrtx_renderer_set_chunk(pbrtx_renderer* h, uint32_t samples_per_chunk) {
     return PBRTX_OK;
 }
 
+int pbrtx_renderer_set_octant_sort(pbrtx_renderer* h, int32_t enable) {
+    if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
+    h->r.octant_sort = enable != 0;
+    h->r.drop_graphs();  // the frame's kernels changed
+    return PBRTX_OK;
+}
+
 int pbrtx_renderer_set_graph(pbrtx_renderer* h, int32_t enable) {
     if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
     h->r.use_graph = enable != 0;
@@ -1367,7 +1468,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     scene_free(&R.scene);
     R.faces.release(); R.mats.release(); R.atris.release(); R.a_first.release(); R.a_count.release();
     R.a_total.release(); R.a_le.release(); R.points.release();
-    R.rays.release(); R.rays2.release(); R.sh.release(); R.hits.release(); R.contrib.release();
+    R.rays.release(); R.rays2.release(); R.rays3.release(); R.pid3.release(); R.sh.release(); R.hits.release(); R.contrib.release();
     R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
     R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
     R.bcounts.release();

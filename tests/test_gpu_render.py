@@ -82,6 +82,27 @@ def test_path_tracing_matches_oracle(path):
     assert st["extension_rays"] == ref["stats"][2]
 
 
+@pytest.mark.parametrize("chunk", [None, 5000])
+def test_octant_sorted_queues_match_oracle(chunk):
+    # the extension queues sorted by direction octant (tiles of 4096 entries:
+    # several tiles, a ragged last one) change no sample, also in graph mode
+    scene = scenes.spheres_scene(n_spheres=40, level=3, seed=8)
+    cam = scenes.pinhole(96, 64, fov=60.0, cam_pos=(0.0, 0.0, 0.0))
+    ref = oracle_render(scene, cam, PATH)
+    r = px.SamplerRenderer(scene, cam, px.StratifiedSampler(2, 2, PATH["seed"]), px.PathIntegrator(5, 3), chunk=chunk)
+    try:
+        r.set_octant_sort(True)
+        for graph in (False, True, True):
+            r.set_graph(graph)
+            img, st = r.render("batched")
+            samples = r.sample_radiance().reshape(cam["yres"], cam["xres"], 4, 3)
+            assert_same(ref, img, samples)
+            assert st["extension_rays"] == ref["stats"][2]
+        assert ref["stats"][2] > 3 * 4096  # the queues span several tiles
+    finally:
+        r.close()
+
+
 def test_chunking_and_repeat_frames_are_identical():
     scene = scenes.spheres_scene(n_spheres=6, level=3, seed=6)
     cam = scenes.pinhole(40, 30, cam_pos=(0.0, 0.0, -25.0))
