@@ -185,12 +185,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 
 // Decoupled look-back (D. Merrill and M. Garland, "Single-pass Parallel
 // Prefix Scan with Decoupled Look-back", NVIDIA technical report
-// NVR-2016-002), run by the first wave of a block: publishes the block's total,
+// NVR-2016-002), run by one wave of a block: publishes the block's total,
 // returns the sum of the totals of the blocks before it, then publishes its
 // inclusive prefix.  tiles[b] = (flag << 32) | value, flag 0 not yet
-// published, 1 block total, 2 inclusive prefix.  Block ids are taken in start
-// order, so every block looked at has started, and it publishes its total
-// without waiting on any other block: each wait ends.
+// published, 1 block total, 2 inclusive prefix.  Each poll reads LB_PER
+// predecessors per lane (64 * LB_PER per poll); windows of 256 and 512
+// tiles per poll measured no faster than 64 on the path tracer's queues, so
+// one per lane.
+// Block ids are taken in start order, so every block looked at has started,
+// and it publishes its total without waiting on any other block: each wait
+// ends.
+constexpr int LB_PER = 1;
 __device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t tot) {
     const uint32_t lane = __lane_id();
     if (lane == 0)
@@ -200,22 +205,47 @@ __device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t t
     if (bid > 0) {
         long long top = (long long)bid - 1;
         while (true) {
-            const long long j = top - (long long)lane;
-            unsigned long long v = 2ull << 32;  // before block 0: an inclusive 0
-            if (j >= 0) {
-                do {
-                    v = __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((v >> 32) == 0) __builtin_amdgcn_s_sleep(1);
-                } while ((v >> 32) == 0);
+            // lane l looks at tiles top - l - 64q, q = 0..LB_PER-1: distance
+            // order is (q, lane), and the nearest 64 -- the ones most likely
+            // still unpublished -- are waited for by 64 lanes at once.
+            // Before tile 0: an inclusive 0.
+            unsigned long long v[LB_PER];
+#pragma unroll
+            for (int q = 0; q < LB_PER; ++q) {
+                const long long j = top - (long long)lane - 64ll * q;
+                v[q] = j >= 0 ? __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ull << 32);
             }
-            // blocks from the nearest one back to the nearest inclusive prefix
-            const unsigned long long incl = __ballot((v >> 32) == 2);
+            while (true) {
+                bool pending = false;
+#pragma unroll
+                for (int q = 0; q < LB_PER; ++q) pending |= (v[q] >> 32) == 0;
+                if (!pending) break;
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int q = 0; q < LB_PER; ++q) {
+                    const long long j = top - (long long)lane - 64ll * q;
+                    if ((v[q] >> 32) == 0)  // not published yet (so j >= 0)
+                        v[q] = __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            // the nearest inclusive prefix: the first q with one, its first lane
+            int qs = LB_PER;
+            unsigned long long incl = 0;
+#pragma unroll
+            for (int q = LB_PER - 1; q >= 0; --q) {
+                const unsigned long long m = __ballot((v[q] >> 32) == 2);
+                if (m) { qs = q; incl = m; }
+            }
             const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-            uint32_t val = lane <= stop ? (uint32_t)v : 0u;
+            // the tiles from the nearest one back to the nearest inclusive prefix
+            uint32_t val = 0;
+#pragma unroll
+            for (int q = 0; q < LB_PER; ++q)
+                if (q < qs || (q == qs && lane <= stop)) val += (uint32_t)v[q];
             for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off);
             excl += val;
             if (incl) break;
-            top -= 64;
+            top -= 64 * LB_PER;
         }
         if (lane == 0)
             __hip_atomic_store(tiles + bid, (2ull << 32) | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -234,6 +264,7 @@ enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_QUEUE = 1 };
 // PSHADE_QUEUE fills two dense queues (shadow and next rays) in queue order
 // with a decoupled look-back per queue.
 enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_QUEUE = 1 };
+static_assert(RB >= 128, "the path vertex kernel runs its two look-backs on two waves");
 constexpr uint32_t SH_CACHE = 4;
 
 __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
@@ -582,25 +613,30 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         }
         // paths that end with no shadow ray pending are final now
         if (live && active) {
+            bool keep = true;
             if (finish && !want_shadow) {
                 sample_l[g0 + ps.gid] = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
                 ps.flags |= 4u;
+                keep = !queued;  // queue form: a finished path is in no queue, its state is not read again
             } else if (finish) {
                 ps.flags |= 2u;
             }
-            st[pid] = ps;
+            if (keep) st[pid] = ps;
         }
         uint32_t ts, tn;
         const uint32_t rs = block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
         const uint32_t rn = block_exclusive_scan(want_next ? 1u : 0u, &tn);
         if (mode == PSHADE_QUEUE) {
-            if (threadIdx.x < 64) {
-                const uint32_t es = lookback(tiles_s, bid, ts);
-                const uint32_t en = lookback(tiles_n, bid, tn);
-                if (threadIdx.x == 0) {
-                    s_es = es;
-                    s_en = en;
-                    if (bid == ntiles - 1) { *sh_count = es + ts; *out_count = en + tn; }  // the last tile
+            // the two look-backs side by side: wave 0 the shadow queue, wave 1
+            // the next-ray queue
+            if (threadIdx.x < 128) {
+                const bool shq = threadIdx.x < 64;
+                const uint32_t tot = shq ? ts : tn;
+                const uint32_t e = lookback(shq ? tiles_s : tiles_n, bid, tot);
+                if (__lane_id() == 0) {
+                    if (shq) s_es = e;
+                    else s_en = e;
+                    if (bid == ntiles - 1) *(shq ? sh_count : out_count) = e + tot;  // the last tile
                 }
             }
             __syncthreads();

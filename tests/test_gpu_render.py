@@ -103,6 +103,32 @@ def test_octant_sorted_queues_match_oracle(chunk):
         r.close()
 
 
+@pytest.mark.parametrize("kind", ["direct", "path"])
+def test_large_frames_batched_equals_plain(kind):
+    # thousands of look-back tiles in flight (queue offsets found many tiles
+    # back): the batched frame must still equal the plain one bit for bit
+    if kind == "path":
+        scene = scenes.spheres_scene(n_spheres=40, level=3, seed=8)
+        cam = scenes.pinhole(512, 512, fov=60.0, cam_pos=(0.0, 0.0, 0.0))
+        it = px.PathIntegrator(5, 3)
+    else:
+        scene = scenes.direct_lighting_scene(level=4)
+        cam = scenes.pinhole(512, 512)
+        it = px.DirectLightingIntegrator(4)
+    r = px.SamplerRenderer(scene, cam, px.StratifiedSampler(2, 2, 11), it)
+    try:
+        a, sa = r.render("batched")
+        b, sb = r.render("plain")
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert sa["shadow_rays"] == sb["shadow_rays"] and sa["extension_rays"] == sb["extension_rays"]
+        if kind == "path":
+            r.set_octant_sort(True)
+            c, _ = r.render("batched")
+            assert np.array_equal(c.view(np.uint32), b.view(np.uint32))
+    finally:
+        r.close()
+
+
 def test_chunking_and_repeat_frames_are_identical():
     scene = scenes.spheres_scene(n_spheres=6, level=3, seed=6)
     cam = scenes.pinhole(40, 30, cam_pos=(0.0, 0.0, -25.0))
