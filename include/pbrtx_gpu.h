@@ -221,6 +221,61 @@ int pbrtx_server_stats(pbrtx_server* srv, pbrtx_server_counters* stats);
 /* finishes the requests already submitted, then stops */
 int pbrtx_server_destroy(pbrtx_server* srv);
 
+/* ---- render-request server ----------------------------------------------- */
+/* A server of render requests (Renderer::render, renderer.rs:8-26, with the
+   path integrator) against many device-resident scenes; the reference has no
+   server.  One worker thread owns the GPU work.  PBRTX_SERVE_BATCHED:
+   requests are admitted in arrival order while their device state (16 B per
+   sample + 12 B per pixel) fits arena_bytes; each step gives every admitted
+   request of a scene a share of up to batch_samples paths (progressive tiles:
+   its next samples in pixel order) and runs them as one wavefront batch whose traversal
+   launches all requests share; batches of different scenes run side by side
+   on their own streams.  PBRTX_SERVE_PLAIN: one request at a time on the
+   plain path.  An image does not depend on the other requests in flight. */
+typedef struct pbrtx_render_server pbrtx_render_server;
+typedef struct pbrtx_render_server_config {
+    uint64_t arena_bytes;    /* HBM for admitted requests (admission budget) */
+    uint32_t batch_samples;  /* paths per batch (>= 4096) */
+    int32_t mode;            /* PBRTX_SERVE_* */
+} pbrtx_render_server_config;
+/* One render: camera (as pbrtx_render_desc), image size and stratified
+   sampler (nx x ny samples per pixel, seed). */
+typedef struct pbrtx_render_request {
+    uint32_t scene;
+    float cam_to_world[16]; float fov_deg; float screen[4];
+    uint32_t xres, yres, nx, ny, seed;
+} pbrtx_render_request;
+/* per-request record; times in ms since the server was created */
+typedef struct pbrtx_render_request_stats {
+    uint64_t n_samples;
+    int32_t status;          /* 0 or the error code of a failed request */
+    uint32_t batches;        /* batches that carried part of it */
+    double submit_ms, admit_ms, first_launch_ms, done_ms;
+} pbrtx_render_request_stats;
+typedef struct pbrtx_render_server_counters {
+    uint64_t samples_done, requests_done, steps, batches;
+    uint64_t camera_rays, shadow_rays, extension_rays; /* rays traced */
+    double busy_ms;          /* summed batch times (HIP events) */
+    double elapsed_ms;       /* since creation */
+    uint64_t arena_bytes, scene_bytes;
+} pbrtx_render_server_counters;
+int pbrtx_render_server_create(const pbrtx_render_server_config* cfg, pbrtx_render_server** out);
+/* registers a scene: the geometry, material and light fields of desc (the
+   camera, image and sampler fields are ignored); desc->integrator must be 1
+   (path), and its max_depth / rr_start apply to every request of the scene.
+   Builds the BVH on the host and uploads it. */
+int pbrtx_render_server_add_scene(pbrtx_render_server* srv, const pbrtx_render_desc* desc, uint32_t* scene_id);
+/* rgb_out: host memory for xres*yres*3 floats (the film's final rgb, as
+   pbrtx_renderer_render); it must stay valid until the wait returns */
+int pbrtx_render_server_submit(pbrtx_render_server* srv, const pbrtx_render_request* req, float* rgb_out,
+                               uint64_t* ticket);
+/* blocks until the request is done; fills *stats (may be NULL) and retires
+   the ticket */
+int pbrtx_render_server_wait(pbrtx_render_server* srv, uint64_t ticket, pbrtx_render_request_stats* stats);
+int pbrtx_render_server_stats(pbrtx_render_server* srv, pbrtx_render_server_counters* stats);
+/* finishes the requests already submitted, then stops */
+int pbrtx_render_server_destroy(pbrtx_render_server* srv);
+
 /* ---- device memory and timing plumbing ----------------------------------- */
 int pbrtx_device_malloc(void** d_ptr, uint64_t bytes);
 int pbrtx_device_free(void* d_ptr);

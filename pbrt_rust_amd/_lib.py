@@ -74,6 +74,33 @@ class ServerCounters(ctypes.Structure):
                 ("arena_bytes", ctypes.c_uint64), ("scene_bytes", ctypes.c_uint64)]
 
 
+class RenderServerConfig(ctypes.Structure):
+    """pbrtx_render_server_config (include/pbrtx_gpu.h)."""
+    _fields_ = [("arena_bytes", ctypes.c_uint64), ("batch_samples", ctypes.c_uint32), ("mode", ctypes.c_int32)]
+
+
+class RenderRequest(ctypes.Structure):
+    """pbrtx_render_request (include/pbrtx_gpu.h)."""
+    _fields_ = [("scene", ctypes.c_uint32), ("cam_to_world", ctypes.c_float * 16), ("fov_deg", ctypes.c_float),
+                ("screen", ctypes.c_float * 4), ("xres", ctypes.c_uint32), ("yres", ctypes.c_uint32),
+                ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("seed", ctypes.c_uint32)]
+
+
+class RenderRequestStats(ctypes.Structure):
+    """pbrtx_render_request_stats (include/pbrtx_gpu.h)."""
+    _fields_ = [("n_samples", ctypes.c_uint64), ("status", ctypes.c_int32), ("batches", ctypes.c_uint32),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("first_launch_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
+
+
+class RenderServerCounters(ctypes.Structure):
+    """pbrtx_render_server_counters (include/pbrtx_gpu.h)."""
+    _fields_ = [("samples_done", ctypes.c_uint64), ("requests_done", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("camera_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("extension_rays", ctypes.c_uint64), ("busy_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
+                ("arena_bytes", ctypes.c_uint64), ("scene_bytes", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -117,6 +144,12 @@ def lib():
     L.pbrtx_server_wait.argtypes = [vp, u64, c.POINTER(RequestStats)]
     L.pbrtx_server_stats.argtypes = [vp, c.POINTER(ServerCounters)]
     L.pbrtx_server_destroy.argtypes = [vp]
+    L.pbrtx_render_server_create.argtypes = [c.POINTER(RenderServerConfig), c.POINTER(vp)]
+    L.pbrtx_render_server_add_scene.argtypes = [vp, c.POINTER(RenderDesc), c.POINTER(u32)]
+    L.pbrtx_render_server_submit.argtypes = [vp, c.POINTER(RenderRequest), vp, c.POINTER(u64)]
+    L.pbrtx_render_server_wait.argtypes = [vp, u64, c.POINTER(RenderRequestStats)]
+    L.pbrtx_render_server_stats.argtypes = [vp, c.POINTER(RenderServerCounters)]
+    L.pbrtx_render_server_destroy.argtypes = [vp]
     if hasattr(L, "pbrtx_renderer_create"):
         L.pbrtx_renderer_create.argtypes = [c.POINTER(RenderDesc), c.POINTER(vp)]
         L.pbrtx_renderer_render.argtypes = [vp, i32, vp, c.POINTER(RenderStats)]

@@ -379,6 +379,78 @@ class RayServer:
             pass
 
 
+class RenderServer:
+    """Render-request server over device-resident scenes
+    (pbrtx_render_server_*): requests of Renderer::render with the path
+    integrator -- a camera, an image size and a stratified sampler -- against
+    registered scenes, admitted by an HBM budget and rendered in progressive
+    tiles batched across requests ("batched") or one at a time on the plain
+    path ("plain").  submit() returns a ticket; wait(ticket) blocks and
+    returns the image (yres, xres, 3) and the request's timings (ms since the
+    server started)."""
+
+    MODES = {"batched": 0, "plain": 1}
+
+    def __init__(self, arena_bytes=8 << 30, batch_samples=1 << 22, mode="batched"):
+        cfg = _lib.RenderServerConfig(int(arena_bytes), int(batch_samples), self.MODES[mode])
+        h = ctypes.c_void_p()
+        check(lib().pbrtx_render_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h.value
+        self._pending = {}
+
+    def add_scene(self, scene, integrator, split="sah"):
+        """scene: a scene dict (see scenes.py); integrator: a PathIntegrator
+        (its depth and roulette start apply to every request of the scene)."""
+        d, keep = _render_desc(scene, None, None, integrator, split)
+        sid = ctypes.c_uint32()
+        check(lib().pbrtx_render_server_add_scene(self._h, ctypes.byref(d), ctypes.byref(sid)))
+        return sid.value
+
+    def submit(self, scene_id, camera, sampler, out=None):
+        """camera: scenes.pinhole dict; sampler: StratifiedSampler; out
+        (optional): a float32 (yres, xres, 3) buffer, page-locked arrays
+        (PinnedArray) let the copy overlap device work.  Kept until wait()."""
+        req = _lib.RenderRequest()
+        req.scene = int(scene_id)
+        _set_camera(req, camera)
+        req.nx, req.ny, req.seed = sampler.xs, sampler.ys, sampler.seed
+        shape = (camera["yres"], camera["xres"], 3)
+        if out is None:
+            out = np.zeros(shape, F32)
+        elif out.dtype != F32 or out.size < np.prod(shape) or not out.flags.c_contiguous:
+            raise ValueError("image buffer too small, not float32 or not contiguous")
+        t = ctypes.c_uint64()
+        check(lib().pbrtx_render_server_submit(self._h, ctypes.byref(req), out.ctypes.data, ctypes.byref(t)))
+        self._pending[t.value] = (out, shape)
+        return t.value
+
+    def wait(self, ticket):
+        st = _lib.RenderRequestStats()
+        rc = lib().pbrtx_render_server_wait(self._h, int(ticket), ctypes.byref(st))
+        out, shape = self._pending.pop(ticket)
+        check(rc)
+        img = out.reshape(-1)[:int(np.prod(shape))].reshape(shape)
+        return img, dict(n_samples=st.n_samples, batches=st.batches, submit_ms=st.submit_ms, admit_ms=st.admit_ms,
+                         first_launch_ms=st.first_launch_ms, done_ms=st.done_ms,
+                         latency_ms=st.done_ms - st.submit_ms)
+
+    def counters(self):
+        c = _lib.RenderServerCounters()
+        check(lib().pbrtx_render_server_stats(self._h, ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbrtx_render_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Aggregate:
     """Aggregate::bvh (aggregates/mod.rs:27-29)."""
 
@@ -439,43 +511,57 @@ class PathIntegrator:
         self.max_depth, self.rr_start = int(max_depth), int(rr_start)
 
 
+def _render_desc(scene, camera, sampler, integrator, split="sah"):
+    """pbrtx_render_desc of a scene dict (see scenes.py), a camera dict
+    (scenes.pinhole, or None: scene only), a sampler and an integrator; also
+    returns the arrays it points into (keep them alive)."""
+    from ._lib import RenderDesc
+    keep = dict(
+        verts=np.ascontiguousarray(scene["verts"], F32).reshape(-1),
+        idx=np.ascontiguousarray(scene["idx"], np.uint32).reshape(-1),
+        face_mat=np.ascontiguousarray(scene["face_mat"], np.uint32),
+        mats=np.ascontiguousarray(scene["mats"], F32).reshape(-1),
+        face_light=np.ascontiguousarray(scene["face_light"], np.int32),
+        area_le=np.ascontiguousarray(scene["area_le"], F32).reshape(-1),
+        point_lights=np.ascontiguousarray(scene["point_lights"], F32).reshape(-1),
+    )
+    k = keep
+    d = RenderDesc()
+    d.verts, d.nverts = k["verts"].ctypes.data, k["verts"].size // 3
+    d.idx, d.nfaces = k["idx"].ctypes.data, k["idx"].size // 3
+    d.face_mat, d.mats, d.nmats = k["face_mat"].ctypes.data, k["mats"].ctypes.data, k["mats"].size // 4
+    d.face_light = k["face_light"].ctypes.data
+    d.area_le, d.n_area = (k["area_le"].ctypes.data if k["area_le"].size else None), k["area_le"].size // 3
+    d.point_lights = k["point_lights"].ctypes.data if k["point_lights"].size else None
+    d.n_point = k["point_lights"].size // 6
+    if camera is not None:
+        _set_camera(d, camera)
+    if sampler is not None:
+        d.nx, d.ny, d.seed = sampler.xs, sampler.ys, sampler.seed
+    d.integrator = integrator.kind
+    d.light_samples = getattr(integrator, "light_samples", 1)
+    d.max_depth = getattr(integrator, "max_depth", 5)
+    d.rr_start = getattr(integrator, "rr_start", 3)
+    d.split = {"sah": 0, "middle": 1, "equal": 2}[split]
+    d.max_prims = 1
+    return d, keep
+
+
+def _set_camera(d, camera):
+    for i, v in enumerate(np.asarray(camera["cam_to_world"], F32).reshape(16)):
+        d.cam_to_world[i] = float(v)
+    d.fov_deg = camera["fov"]
+    for i, v in enumerate(camera["screen"]):
+        d.screen[i] = float(v)
+    d.xres, d.yres = camera["xres"], camera["yres"]
+
+
 class SamplerRenderer:
     """Renderer::render for a generated scene dict (see scenes.py), a camera
     dict (scenes.pinhole) and an integrator; the frame runs on the GPU."""
 
     def __init__(self, scene, camera, sampler, integrator, split="sah", chunk=None):
-        from ._lib import RenderDesc
-        self._keep = dict(
-            verts=np.ascontiguousarray(scene["verts"], F32).reshape(-1),
-            idx=np.ascontiguousarray(scene["idx"], np.uint32).reshape(-1),
-            face_mat=np.ascontiguousarray(scene["face_mat"], np.uint32),
-            mats=np.ascontiguousarray(scene["mats"], F32).reshape(-1),
-            face_light=np.ascontiguousarray(scene["face_light"], np.int32),
-            area_le=np.ascontiguousarray(scene["area_le"], F32).reshape(-1),
-            point_lights=np.ascontiguousarray(scene["point_lights"], F32).reshape(-1),
-        )
-        k = self._keep
-        d = RenderDesc()
-        d.verts, d.nverts = k["verts"].ctypes.data, k["verts"].size // 3
-        d.idx, d.nfaces = k["idx"].ctypes.data, k["idx"].size // 3
-        d.face_mat, d.mats, d.nmats = k["face_mat"].ctypes.data, k["mats"].ctypes.data, k["mats"].size // 4
-        d.face_light = k["face_light"].ctypes.data
-        d.area_le, d.n_area = (k["area_le"].ctypes.data if k["area_le"].size else None), k["area_le"].size // 3
-        d.point_lights = k["point_lights"].ctypes.data if k["point_lights"].size else None
-        d.n_point = k["point_lights"].size // 6
-        for i, v in enumerate(np.asarray(camera["cam_to_world"], F32).reshape(16)):
-            d.cam_to_world[i] = float(v)
-        d.fov_deg = camera["fov"]
-        for i, v in enumerate(camera["screen"]):
-            d.screen[i] = float(v)
-        d.xres, d.yres = camera["xres"], camera["yres"]
-        d.nx, d.ny, d.seed = sampler.xs, sampler.ys, sampler.seed
-        d.integrator = integrator.kind
-        d.light_samples = getattr(integrator, "light_samples", 1)
-        d.max_depth = getattr(integrator, "max_depth", 5)
-        d.rr_start = getattr(integrator, "rr_start", 3)
-        d.split = {"sah": 0, "middle": 1, "equal": 2}[split]
-        d.max_prims = 1
+        d, self._keep = _render_desc(scene, camera, sampler, integrator, split)
         self.desc = d
         self.xres, self.yres, self.spp = d.xres, d.yres, d.nx * d.ny
         h = ctypes.c_void_p()

@@ -21,48 +21,9 @@
 #include <chrono>
 #include <vector>
 
-#include "../../include/pbrtx_gpu.h"
-#include "camera.hpp"
-#include "capi_internal.hpp"
+#include "render.hpp"
 
 namespace px {
-
-struct FaceShade {  // per face, precomputed with the hit-time formulas
-    float nn[3];    // dg.nn = normalize(dpdu x dpdv)  (mesh.rs:224-254, diff_geom.rs:52-60)
-    float tn[3];    // normalize(dpdu)                  (bsdf/mod.rs:70-73)
-    uint32_t mat;
-    int32_t light;
-};
-struct AreaTriRec {
-    float p1[3], p2[3], p3[3], nn[3];
-    float area, pad0, pad1, pad2;
-};
-struct MatRec {
-    uint32_t kind;
-    float r, g, b;
-};
-
-struct RenderParams {
-    float r2c[16], c2w[16];
-    uint32_t xres, yres, nx, ny, spp, seed;
-    uint32_t n_point, n_area, nlights, light_samples, max_depth, rr_start, S;
-    float dx, dy;
-    const FaceShade* faces;
-    const MatRec* mats;
-    const AreaTriRec* atris;
-    const uint32_t* a_first;
-    const uint32_t* a_count;
-    const float* a_total;
-    const float* a_le;     // 3 per area light
-    const float* points;   // 6 per point light
-};
-
-struct PathState {
-    float beta[3];
-    float L[3];
-    uint32_t gid;
-    uint32_t flags;  // 1 specular, 2 done after this bounce's shadow ray
-};
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr int RB = 256;  // render kernel block
@@ -463,17 +424,49 @@ __global__ void k_add_counter(uint32_t* lo, const uint32_t* add, uint32_t* hi) {
 }
 
 // ---- path tracing ------------------------------------------------------------
+// Where a path's sample radiance goes and what keys its random numbers:
+// sample g0 + gid of the frame, or -- in a served batch (P.segs) -- the
+// sample of the request that the path's segment (flags bits 8-15) maps
+// batch index gid to.
+struct SampleRef {
+    uint64_t g;
+    uint32_t spp, seed;
+    float4* out;
+};
+__device__ __forceinline__ SampleRef sample_ref(const RenderParams& P, uint64_t g0, const PathState& ps,
+                                                float4* sample_l) {
+    if (P.segs) {
+        const RenderSeg sg = P.segs[ps.flags >> 8];
+        const ReqView* rq = P.reqs + sg.req;
+        const uint64_t g = sg.g0 + (ps.gid - sg.start);
+        return SampleRef{g, rq->cam.spp, rq->cam.seed, rq->sample_l + g};
+    }
+    const uint64_t g = g0 + ps.gid;
+    return SampleRef{g, P.spp, P.seed, sample_l + g};
+}
+
+// Camera rays and initial path state of the batch's paths.
 __global__ __launch_bounds__(RB) void k_path_init(RenderParams P, uint64_t g0, uint32_t n, Ray* rays,
                                                   PathState* st, uint32_t* pid_q, uint32_t* q_count) {
     for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
-        uint64_t g = g0 + i;
-        uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
-        float ix, iy;
-        sample_pos(P, pixel % P.xres, pixel / P.xres, s, &ix, &iy);
         Ray r;
-        camera_ray(P, ix, iy, &r);
+        uint32_t flags = 0;
+        if (P.segs) {
+            // the segment holding batch index i: the last one starting at or before it
+            uint32_t lo = 0, hi = P.nsegs - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (P.segs[mid].start <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            const RenderSeg sg = P.segs[lo];
+            camera_ray_of(P.reqs[sg.req].cam, sg.g0 + (i - sg.start), &r);
+            flags = lo << 8;
+        } else {
+            camera_ray_of(P, g0 + i, &r);
+        }
         rays[i] = r;
-        st[i] = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
+        st[i] = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, flags};
         if (pid_q) pid_q[i] = i;
     }
     if (q_count && blockIdx.x == 0 && threadIdx.x == 0) *q_count = n;
@@ -549,11 +542,11 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                 L = add3(L, mulc(beta, le));
             }
             const MatRec mat = P.mats[fs.mat];
-            const uint64_t g = g0 + ps.gid;
-            const uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
+            const SampleRef sr = sample_ref(P, g0, ps, sample_l);
+            const uint32_t pixel = (uint32_t)(sr.g / sr.spp), s = (uint32_t)(sr.g % sr.spp);
             float ua[4], ub[4];
-            rng4(P.seed, pixel, s, 1 + 2 * bounce, ua);
-            rng4(P.seed, pixel, s, 2 + 2 * bounce, ub);
+            rng4(sr.seed, pixel, s, 1 + 2 * bounce, ua);
+            rng4(sr.seed, pixel, s, 2 + 2 * bounce, ub);
             if (P.nlights > 0 && mat.kind == 0u) {
                 uint32_t ln = (uint32_t)(ua[0] * (float)P.nlights);
                 if (ln > P.nlights - 1) ln = P.nlights - 1;
@@ -615,7 +608,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         if (live && active) {
             bool keep = true;
             if (finish && !want_shadow) {
-                sample_l[g0 + ps.gid] = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
+                *sample_ref(P, g0, ps, sample_l).out = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
                 ps.flags |= 4u;
                 keep = !queued;  // queue form: a finished path is in no queue, its state is not read again
             } else if (finish) {
@@ -677,7 +670,7 @@ __global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0,
             ps.L[0] = ps.L[0] + c.x; ps.L[1] = ps.L[1] + c.y; ps.L[2] = ps.L[2] + c.z;
         }
         if (ps.flags & 2u) {
-            sample_l[g0 + ps.gid] = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
+            *sample_ref(P, g0, ps, sample_l).out = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
             ps.flags = (ps.flags & ~2u) | 4u;
         }
         st[pid] = ps;
@@ -923,35 +916,6 @@ bool perspective_r2c(uint32_t xres, uint32_t yres, const float screen[4], float 
     return true;
 }
 
-template <typename T>
-struct DevArr {
-    T* p = nullptr;
-    size_t n = 0;
-    int ensure(size_t count) {
-        if (count <= n) return PBRTX_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T) + 16);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(render buffer)");
-        n = count;
-        return PBRTX_OK;
-    }
-    int upload(const std::vector<T>& v) {
-        int rc = ensure(v.size() ? v.size() : 1);
-        if (rc) return rc;
-        if (v.empty()) return PBRTX_OK;
-        hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(render upload)");
-        return PBRTX_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
 int grid_for(size_t n) {
     size_t g = (n + RB - 1) / RB;
     size_t cap = (size_t)device_cu_count() * 16;
@@ -960,47 +924,270 @@ int grid_for(size_t n) {
     return (int)g;
 }
 
-struct EventPool {
-    std::vector<hipEvent_t> ev;
-    size_t used = 0;
-    hipEvent_t next() {
-        if (used == ev.size()) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            ev.push_back(e);
-        }
-        return ev[used++];
-    }
-    ~EventPool() {
-        for (auto e : ev) (void)hipEventDestroy(e);
-    }
-};
-
 }  // namespace
 
+#define RCHK(x)                                  \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, #x); \
+    } while (0)
+#define ICHK(x)              \
+    do {                     \
+        int r_ = (x);        \
+        if (r_) return r_;   \
+    } while (0)
+
+// ---- scene and camera set-up -------------------------------------------------
+void RenderScene::release() {
+    scene_free(&scene);
+    faces.release(); mats.release(); atris.release(); a_first.release(); a_count.release();
+    a_total.release(); a_le.release(); points.release();
+}
+
+void WorkBufs::release() {
+    rays.release(); rays2.release(); rays3.release(); sh.release(); hits.release(); contrib.release();
+    lbase.release(); qidx.release(); occ.release(); pid.release(); pid2.release(); pid3.release();
+    shpid.release(); bcounts.release(); st.release();
+}
+
+int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
+    if (!d->verts || !d->idx || !d->face_mat || !d->mats) return fail(PBRTX_ERR_ARG, "null scene arrays");
+    if (d->integrator != 0 && d->integrator != 1) return fail(PBRTX_ERR_ARG, "unknown integrator");
+    if (d->integrator == 0 && d->n_area > 0 && d->light_samples == 0)
+        return fail(PBRTX_ERR_ARG, "light_samples must be >= 1");
+    for (uint32_t f = 0; f < d->nfaces; ++f)
+        if (d->face_mat[f] >= d->nmats) return fail(PBRTX_ERR_ARG, "face material out of range");
+    std::string err;
+    if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, nullptr, 0, d->split, d->max_prims, &S->bvh, &err))
+        return fail(PBRTX_ERR_BUILD, err);
+    S->build_ms = S->bvh.build_ms;
+    int rc = scene_from_host(S->bvh, &S->scene);
+    if (rc) return rc;
+    S->nfaces = d->nfaces;
+    auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
+    std::vector<FaceShade> fsh(d->nfaces);
+    std::vector<std::vector<AreaTriRec>> per_light(d->n_area);
+    for (uint32_t f = 0; f < d->nfaces; ++f) {
+        // Triangle::intersect dg with the default uvs (mesh.rs:212-254)
+        const f3 p1 = V(d->idx[3 * f + 2]), p2 = V(d->idx[3 * f + 1]), p3 = V(d->idx[3 * f]);
+        const float du1 = 0.0f - 1.0f, du2 = 1.0f - 1.0f, dv1 = 0.0f - 1.0f, dv2 = 0.0f - 1.0f;
+        const f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
+        const float det = du1 * dv2 - dv1 * du2;
+        const float inv_det = 1.0f / det;
+        const f3 dpdu = mul3(sub3(mul3(dp1, dv2), mul3(dp2, dv1)), inv_det);
+        const f3 dpdv = mul3(add3(mul3(dp1, -du2), mul3(dp2, du1)), inv_det);
+        const f3 nn = normalize3(cross3(dpdu, dpdv));
+        const f3 tn = normalize3(dpdu);
+        FaceShade& sh = fsh[f];
+        sh.nn[0] = nn.x; sh.nn[1] = nn.y; sh.nn[2] = nn.z;
+        sh.tn[0] = tn.x; sh.tn[1] = tn.y; sh.tn[2] = tn.z;
+        sh.mat = d->face_mat[f];
+        sh.light = d->face_light ? d->face_light[f] : -1;
+        if (sh.light >= 0) {
+            if ((uint32_t)sh.light >= d->n_area) return fail(PBRTX_ERR_ARG, "face light out of range");
+            AreaTriRec t{};
+            t.p1[0] = p1.x; t.p1[1] = p1.y; t.p1[2] = p1.z;
+            t.p2[0] = p2.x; t.p2[1] = p2.y; t.p2[2] = p2.z;
+            t.p3[0] = p3.x; t.p3[1] = p3.y; t.p3[2] = p3.z;
+            t.nn[0] = nn.x; t.nn[1] = nn.y; t.nn[2] = nn.z;
+            t.area = 0.5f * sqrtf(lensq3(cross3(sub3(p2, p1), sub3(p3, p1))));  // mesh.rs:100-103
+            per_light[(size_t)sh.light].push_back(t);
+        }
+    }
+    std::vector<AreaTriRec> at;
+    std::vector<uint32_t> afirst, acount;
+    std::vector<float> atotal, ale;
+    for (uint32_t l = 0; l < d->n_area; ++l) {
+        if (per_light[l].empty()) return fail(PBRTX_ERR_ARG, "area light without triangles");
+        afirst.push_back((uint32_t)at.size());
+        acount.push_back((uint32_t)per_light[l].size());
+        float total = 0.0f;
+        for (auto& t : per_light[l]) { total = total + t.area; at.push_back(t); }
+        atotal.push_back(total);
+        for (int c = 0; c < 3; ++c) ale.push_back(d->area_le[3 * l + c]);
+    }
+    std::vector<MatRec> mats(d->nmats);
+    for (uint32_t m = 0; m < d->nmats; ++m)
+        mats[m] = MatRec{(uint32_t)d->mats[4 * m], d->mats[4 * m + 1], d->mats[4 * m + 2], d->mats[4 * m + 3]};
+    std::vector<float> pts(d->point_lights ? d->point_lights : (const float*)nullptr,
+                           d->point_lights ? d->point_lights + 6 * d->n_point : (const float*)nullptr);
+    if ((rc = S->faces.upload(fsh)) || (rc = S->mats.upload(mats)) || (rc = S->atris.upload(at)) ||
+        (rc = S->a_first.upload(afirst)) || (rc = S->a_count.upload(acount)) || (rc = S->a_total.upload(atotal)) ||
+        (rc = S->a_le.upload(ale)) || (rc = S->points.upload(pts)))
+        return rc;
+    RenderParams& P = S->base;
+    P.n_point = d->n_point; P.n_area = d->n_area; P.nlights = d->n_point + d->n_area;
+    P.light_samples = d->light_samples ? d->light_samples : 1;
+    P.max_depth = d->max_depth; P.rr_start = d->rr_start;
+    P.S = P.n_point + P.n_area * P.light_samples;
+    P.faces = S->faces.p; P.mats = S->mats.p; P.atris = S->atris.p;
+    P.a_first = S->a_first.p; P.a_count = S->a_count.p; P.a_total = S->a_total.p; P.a_le = S->a_le.p;
+    P.points = S->points.p;
+    P.segs = nullptr; P.reqs = nullptr; P.nsegs = 0;
+    S->integrator = d->integrator;
+    return PBRTX_OK;
+}
+
+int camera_setup(RenderParams* P, const float* cam_to_world, float fov_deg, const float* screen, uint32_t xres,
+                 uint32_t yres, uint32_t nx, uint32_t ny, uint32_t seed) {
+    if (nx == 0 || ny == 0 || xres == 0 || yres == 0) return fail(PBRTX_ERR_ARG, "empty image");
+    if ((uint64_t)xres * yres * nx * ny >= (1ull << 40)) return fail(PBRTX_ERR_ARG, "image too large");
+    if (!perspective_r2c(xres, yres, screen, fov_deg, P->r2c)) return fail(PBRTX_ERR_ARG, "singular projection");
+    memcpy(P->c2w, cam_to_world, sizeof(P->c2w));
+    P->xres = xres; P->yres = yres; P->nx = nx; P->ny = ny; P->spp = nx * ny; P->seed = seed;
+    P->dx = 1.0f / (float)nx; P->dy = 1.0f / (float)ny;
+    return PBRTX_OK;
+}
+
+CamParams cam_params(const RenderParams& P, uint64_t g0) {
+    CamParams cp;
+    memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
+    memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
+    cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
+    cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
+    return cp;
+}
+
+size_t render_spill_bytes(const SceneView& sv, int path) {
+    const size_t a = trace_spill_bytes(sv, path, g_tuning[path][0]);
+    const size_t b = trace_spill_bytes(sv, path, g_tuning[path][1]);
+    const size_t c = trace_spill_bytes(sv, path, g_tuning[path][2]);
+    return std::max(a, std::max(b, c));
+}
+
+hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st) {
+    hipLaunchKernelGGL(k_film, dim3(grid_for((size_t)P.xres * P.yres)), dim3(RB), 0, st, P, sample_l, rgb);
+    return hipGetLastError();
+}
+
+// ---- the path tracer's launch sequence over one batch ---------------------------
+int reserve_path_work(WorkBufs& W, uint32_t cap, bool osort) {
+    ICHK(W.rays.ensure(cap));
+    ICHK(W.hits.ensure(cap));
+    ICHK(W.st.ensure(cap));
+    ICHK(W.rays2.ensure(cap));
+    ICHK(W.sh.ensure(cap));
+    ICHK(W.contrib.ensure(cap));
+    ICHK(W.occ.ensure((size_t)cap / 32 + 2));
+    ICHK(W.pid.ensure(cap));
+    ICHK(W.pid2.ensure(cap));
+    ICHK(W.shpid.ensure(cap));
+    ICHK(W.bcounts.ensure(4 * (size_t)((cap + RB - 1) / RB) + 2));
+    if (osort) {
+        ICHK(W.rays3.ensure(cap));
+        ICHK(W.pid3.ensure(cap));
+    }
+    return PBRTX_OK;
+}
+
+int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, uint32_t* cnt, uint64_t g0,
+                       uint32_t n, int path, bool osort_req, float4* sample_l, TraceScratch* scr, hipStream_t st,
+                       Stager& sg) {
+    if (n == 0) return PBRTX_OK;
+    const bool batched = path == PATH_BATCHED;
+    const TraceTuning& tunC = g_tuning[path][0];
+    const TraceTuning& tunA = g_tuning[path][1];
+    const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
+    const bool osort = batched && osort_req;
+    ICHK(reserve_path_work(W, n, osort));
+    // batched frames: the packet kernel makes the camera rays and the first
+    // vertex makes them again and starts the paths (no init pass); served
+    // batches (several cameras) start from an init pass
+    const bool fused = batched && !P.segs && camera_fusable(sv, tunCam);
+    hipEvent_t a = sg.begin();
+    if (fused) {
+        RCHK(fill_u32(cnt + 1, n, 1, st));
+    } else {
+        hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.rays.p, W.st.p,
+                           batched ? W.pid.p : nullptr, batched ? cnt + 1 : nullptr);
+        RCHK(hipGetLastError());
+    }
+    RCHK(sg.end(a, &Timing::other));
+    Ray* rin = W.rays.p;
+    Ray* rout = W.rays2.p;
+    Ray* rspare = W.rays3.p;
+    uint32_t* pin = W.pid.p;
+    uint32_t* pout = W.pid2.p;
+    uint32_t* pspare = W.pid3.p;
+    uint32_t* cin = cnt + 1;
+    uint32_t* cout = cnt + 2;
+    for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
+        a = sg.begin();
+        const TraceTuning& tb = (bounce == 0 && !P.segs) ? tunCam : tunC;  // frames' bounce 0: camera rays
+        if (fused && bounce == 0) {
+            RCHK(trace_camera(sv, cam_params(P, g0), n, W.hits.p, tb, scr, st));
+        } else if (batched) {
+            RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st, cin));
+        } else {
+            RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st));
+        }
+        RCHK(sg.end(a, &Timing::closest));
+        a = sg.begin();
+        RCHK(fill_u32(cout, 0u, 1, st));
+        RCHK(fill_u32(cnt + 5, 0u, 1, st));
+        {
+            const uint32_t nb = (n + RB - 1) / RB;
+            // look-back tiles of the two queues (two words per block each)
+            // and the block ticket
+            ICHK(W.bcounts.ensure(4 * (size_t)nb + 2));
+            if (batched) RCHK(fill_u32(W.bcounts.p, 0u, 4 * (size_t)nb + 2, st));
+            unsigned long long* tiles = (unsigned long long*)W.bcounts.p;
+            // queue form: persistent blocks over the live paths' tiles
+            const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
+            hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(RB), 0, st, P, g0, bounce, n,
+                               batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
+                               batched ? (const uint32_t*)pin : (const uint32_t*)nullptr,
+                               (fused && bounce == 0) ? (const Ray*)nullptr : rin, W.hits.p, W.st.p, rout,
+                               batched ? pout : nullptr, cout, W.sh.p, batched ? W.shpid.p : nullptr, W.contrib.p,
+                               cnt + 5, sample_l, tiles, tiles + nb, W.bcounts.p + 4 * (size_t)nb,
+                               (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
+            RCHK(hipGetLastError());
+        }
+        RCHK(sg.end(a, &Timing::other));
+        a = sg.begin();
+        if (batched)
+            RCHK(trace_any(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st, cnt + 5));
+        else
+            RCHK(trace_any(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st));
+        RCHK(sg.end(a, &Timing::any));
+        a = sg.begin();
+        hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
+                           batched ? cnt + 5 : (const uint32_t*)nullptr,
+                           batched ? W.shpid.p : (const uint32_t*)nullptr, W.contrib.p, W.occ.p, W.st.p, sample_l);
+        RCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+        if (bounce < P.max_depth)
+            hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
+        RCHK(hipGetLastError());
+        if (osort && bounce < P.max_depth) {
+            const uint32_t tiles = (n + OS_TILE - 1) / OS_TILE;
+            const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)device_cu_count() * 2u);
+            hipLaunchKernelGGL(k_octant_sort, dim3(grid), dim3(OS_BLOCK), 0, st, (const uint32_t*)cout,
+                               (const Ray*)rout, (const uint32_t*)pout, rspare, pspare);
+            RCHK(hipGetLastError());
+            std::swap(rout, rspare);
+            std::swap(pout, pspare);
+        }
+        RCHK(sg.end(a, &Timing::other));
+        std::swap(rin, rout);
+        std::swap(pin, pout);
+        std::swap(cin, cout);
+    }
+    return PBRTX_OK;
+}
+
+// ---- the frame renderer ----------------------------------------------------------
 struct Renderer {
+    RenderScene sc;
     RenderParams P{};
-    HostBvh bvh;
-    DeviceScene scene{};
-    uint32_t nfaces = 0;
     uint64_t n_samples = 0;
     uint32_t chunk = 1u << 22;
-    DevArr<FaceShade> faces;
-    DevArr<MatRec> mats;
-    DevArr<AreaTriRec> atris;
-    DevArr<uint32_t> a_first, a_count;
-    DevArr<float> a_total, a_le, points;
-    // per-chunk work buffers
-    DevArr<Ray> rays, rays2, rays3, sh;
-    DevArr<Hit> hits;
-    DevArr<float4> contrib, lbase, sample_l;
-    DevArr<uint32_t> qidx, occ, pid, pid2, pid3, shpid, counters, bcounts;
-    DevArr<PathState> st;
+    WorkBufs W;
+    DevArr<float4> sample_l;
+    DevArr<uint32_t> counters;
     DevArr<float> rgb;
     hipStream_t stream = nullptr;
     EventPool events;
-    int integrator = 0;
-    double build_ms = 0.0;
     bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
     // graph mode, per path: warm after one eager frame; the captured frame
     // and the tuning generation it was captured under
@@ -1024,115 +1211,26 @@ struct Renderer {
 static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
     if (!d->verts || !d->idx || !d->face_mat || !d->mats) return fail(PBRTX_ERR_ARG, "null scene arrays");
     if (d->nx == 0 || d->ny == 0 || d->xres == 0 || d->yres == 0) return fail(PBRTX_ERR_ARG, "empty image");
-    if (d->integrator != 0 && d->integrator != 1) return fail(PBRTX_ERR_ARG, "unknown integrator");
-    if (d->integrator == 0 && d->n_area > 0 && d->light_samples == 0)
-        return fail(PBRTX_ERR_ARG, "light_samples must be >= 1");
-    for (uint32_t f = 0; f < d->nfaces; ++f)
-        if (d->face_mat[f] >= d->nmats) return fail(PBRTX_ERR_ARG, "face material out of range");
-    std::string err;
-    if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, nullptr, 0, d->split, d->max_prims, &R->bvh, &err))
-        return fail(PBRTX_ERR_BUILD, err);
+    ICHK(render_scene_setup(&R->sc, d));
     // the renderer's own stream (default priority; a frame ends by waiting
     // for its last event, so host reads after a frame see its results)
     {
         hipError_t e = hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking);
         if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     }
-    R->build_ms = R->bvh.build_ms;
-    int rc = scene_from_host(R->bvh, &R->scene);
-    if (rc) return rc;
-    R->nfaces = d->nfaces;
-    auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
-    std::vector<FaceShade> fsh(d->nfaces);
-    std::vector<std::vector<AreaTriRec>> per_light(d->n_area);
-    for (uint32_t f = 0; f < d->nfaces; ++f) {
-        // Triangle::intersect dg with the default uvs (mesh.rs:212-254)
-        const f3 p1 = V(d->idx[3 * f + 2]), p2 = V(d->idx[3 * f + 1]), p3 = V(d->idx[3 * f]);
-        const float du1 = 0.0f - 1.0f, du2 = 1.0f - 1.0f, dv1 = 0.0f - 1.0f, dv2 = 0.0f - 1.0f;
-        const f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
-        const float det = du1 * dv2 - dv1 * du2;
-        const float inv_det = 1.0f / det;
-        const f3 dpdu = mul3(sub3(mul3(dp1, dv2), mul3(dp2, dv1)), inv_det);
-        const f3 dpdv = mul3(add3(mul3(dp1, -du2), mul3(dp2, du1)), inv_det);
-        const f3 nn = normalize3(cross3(dpdu, dpdv));
-        const f3 tn = normalize3(dpdu);
-        FaceShade& s = fsh[f];
-        s.nn[0] = nn.x; s.nn[1] = nn.y; s.nn[2] = nn.z;
-        s.tn[0] = tn.x; s.tn[1] = tn.y; s.tn[2] = tn.z;
-        s.mat = d->face_mat[f];
-        s.light = d->face_light ? d->face_light[f] : -1;
-        if (s.light >= 0) {
-            if ((uint32_t)s.light >= d->n_area) return fail(PBRTX_ERR_ARG, "face light out of range");
-            AreaTriRec t{};
-            t.p1[0] = p1.x; t.p1[1] = p1.y; t.p1[2] = p1.z;
-            t.p2[0] = p2.x; t.p2[1] = p2.y; t.p2[2] = p2.z;
-            t.p3[0] = p3.x; t.p3[1] = p3.y; t.p3[2] = p3.z;
-            t.nn[0] = nn.x; t.nn[1] = nn.y; t.nn[2] = nn.z;
-            t.area = 0.5f * sqrtf(lensq3(cross3(sub3(p2, p1), sub3(p3, p1))));  // mesh.rs:100-103
-            per_light[(size_t)s.light].push_back(t);
-        }
-    }
-    std::vector<AreaTriRec> at;
-    std::vector<uint32_t> afirst, acount;
-    std::vector<float> atotal, ale;
-    for (uint32_t l = 0; l < d->n_area; ++l) {
-        if (per_light[l].empty()) return fail(PBRTX_ERR_ARG, "area light without triangles");
-        afirst.push_back((uint32_t)at.size());
-        acount.push_back((uint32_t)per_light[l].size());
-        float total = 0.0f;
-        for (auto& t : per_light[l]) { total = total + t.area; at.push_back(t); }
-        atotal.push_back(total);
-        for (int c = 0; c < 3; ++c) ale.push_back(d->area_le[3 * l + c]);
-    }
-    std::vector<MatRec> mats(d->nmats);
-    for (uint32_t m = 0; m < d->nmats; ++m)
-        mats[m] = MatRec{(uint32_t)d->mats[4 * m], d->mats[4 * m + 1], d->mats[4 * m + 2], d->mats[4 * m + 3]};
-    std::vector<float> pts(d->point_lights ? d->point_lights : (const float*)nullptr,
-                           d->point_lights ? d->point_lights + 6 * d->n_point : (const float*)nullptr);
-    if ((rc = R->faces.upload(fsh)) || (rc = R->mats.upload(mats)) || (rc = R->atris.upload(at)) ||
-        (rc = R->a_first.upload(afirst)) || (rc = R->a_count.upload(acount)) || (rc = R->a_total.upload(atotal)) ||
-        (rc = R->a_le.upload(ale)) || (rc = R->points.upload(pts)))
-        return rc;
-    RenderParams& P = R->P;
-    if (!perspective_r2c(d->xres, d->yres, d->screen, d->fov_deg, P.r2c))
-        return fail(PBRTX_ERR_ARG, "singular projection");
-    memcpy(P.c2w, d->cam_to_world, sizeof(P.c2w));
-    P.xres = d->xres; P.yres = d->yres; P.nx = d->nx; P.ny = d->ny; P.spp = d->nx * d->ny; P.seed = d->seed;
-    P.n_point = d->n_point; P.n_area = d->n_area; P.nlights = d->n_point + d->n_area;
-    P.light_samples = d->light_samples ? d->light_samples : 1;
-    P.max_depth = d->max_depth; P.rr_start = d->rr_start;
-    P.S = P.n_point + P.n_area * P.light_samples;
-    P.dx = 1.0f / (float)d->nx; P.dy = 1.0f / (float)d->ny;
-    P.faces = R->faces.p; P.mats = R->mats.p; P.atris = R->atris.p;
-    P.a_first = R->a_first.p; P.a_count = R->a_count.p; P.a_total = R->a_total.p; P.a_le = R->a_le.p;
-    P.points = R->points.p;
-    R->integrator = d->integrator;
-    R->n_samples = (uint64_t)d->xres * d->yres * P.spp;
-    if (R->n_samples >= (1ull << 40)) return fail(PBRTX_ERR_ARG, "image too large");
+    R->P = R->sc.base;
+    ICHK(camera_setup(&R->P, d->cam_to_world, d->fov_deg, d->screen, d->xres, d->yres, d->nx, d->ny, d->seed));
+    R->n_samples = (uint64_t)d->xres * d->yres * R->P.spp;
     return PBRTX_OK;
 }
 
-#define RCHK(x)                                  \
-    do {                                         \
-        hipError_t e_ = (x);                     \
-        if (e_ != hipSuccess) return hip_fail(e_, #x); \
-    } while (0)
-#define ICHK(x)              \
-    do {                     \
-        int r_ = (x);        \
-        if (r_) return r_;   \
-    } while (0)
-
-struct Timing {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> closest, any, other;
-};
-
 // Enqueues one frame's device work on the renderer's stream: every kernel and
 // fill from the counter reset to the film, nothing that waits on the host, so
-// it can be captured as a graph.  timed: record per-stage events into T.
-static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bool timed, uint64_t* camera_rays) {
+// it can be captured as a graph.  sg: per-stage events (or none).
+static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, uint64_t* camera_rays) {
     RenderParams& P = R->P;
     hipStream_t st = R->stream;
+    WorkBufs& W = R->W;
     const uint32_t spp = P.spp;
     // chunk: whole pixels, at most R->chunk samples
     uint64_t per = (uint64_t)R->chunk / spp * spp;
@@ -1140,208 +1238,83 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bo
     if (per > R->n_samples) per = R->n_samples;
     const uint32_t cap = (uint32_t)per;
     ICHK(R->sample_l.ensure(R->n_samples));
-    ICHK(R->rays.ensure(cap));
-    ICHK(R->hits.ensure(cap));
-    ICHK(R->counters.ensure(16));
+    ICHK(W.rays.ensure(cap));
+    ICHK(W.hits.ensure(cap));
+    ICHK(R->counters.ensure(N_COUNTERS));
     const bool batched = path == PATH_BATCHED;
-    const TraceTuning& tunC = g_tuning[path][0];
     const TraceTuning& tunA = g_tuning[path][1];
     const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
-    auto stage_begin = [&]() -> hipEvent_t {
-        if (!timed) return nullptr;
-        hipEvent_t e = R->events.next();
-        return (e && hipEventRecord(e, st) == hipSuccess) ? e : nullptr;
-    };
-    auto stage_end = [&](hipEvent_t a, std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) -> hipError_t {
-        if (!timed || !a) return hipSuccess;
-        hipEvent_t b = R->events.next();
-        if (!b) return hipErrorOutOfMemory;
-        hipError_t e = hipEventRecord(b, st);
-        if (e == hipSuccess) v.push_back({a, b});
-        return e;
-    };
-    uint32_t* cnt = R->counters.p;  // [0] shadow, [1] ext/next A, [2] next B, [3] camera, [4..] totals
-    RCHK(fill_u32(cnt, 0u, 16, st));
-    const SceneView& sv = R->scene.view;
+    uint32_t* cnt = R->counters.p;
+    RCHK(fill_u32(cnt, 0u, N_COUNTERS, st));
+    const SceneView& sv = R->sc.scene.view;
     uint64_t total_camera = 0;
     for (uint64_t g0 = 0; g0 < R->n_samples; g0 += per) {
         const uint32_t n = (uint32_t)((R->n_samples - g0) < per ? (R->n_samples - g0) : per);
         total_camera += n;
-        if (R->integrator == 0) {
-            const size_t slots = (size_t)n * P.S;
-            ICHK(R->sh.ensure(slots ? slots : 1));
-            ICHK(R->contrib.ensure(slots ? slots : 1));
-            ICHK(R->qidx.ensure(slots ? slots : 1));
-            ICHK(R->lbase.ensure(n));
-            ICHK(R->occ.ensure((slots + 31) / 32 + 1));
-            hipEvent_t a;
-            // batched: the packet kernel makes the camera rays itself and the
-            // shade kernel makes them again (rays = null): no ray array
-            const bool fused = batched && camera_fusable(sv, tunCam);
-            Ray* crays = fused ? nullptr : R->rays.p;
-            if (!fused) {
-                a = stage_begin();
-                hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p);
-                RCHK(hipGetLastError());
-                RCHK(stage_end(a, T.other));
-            }
-            a = stage_begin();
-            if (fused) {
-                CamParams cp;
-                memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
-                memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
-                cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
-                cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
-                RCHK(trace_camera(sv, cp, n, R->hits.p, tunCam, scr, st));
-            } else {
-                RCHK(trace_closest(sv, R->rays.p, n, R->hits.p, path, tunCam, scr, st));
-            }
-            RCHK(stage_end(a, T.closest));
-            a = stage_begin();
-            RCHK(fill_u32(cnt + 5, 0u, 1, st));
-            {
-                const uint32_t nb = (n + RB - 1) / RB;
-                if (batched) {
-                    // look-back tiles (two words per block) and the block ticket
-                    ICHK(R->bcounts.ensure(2 * (size_t)nb + 2));
-                    RCHK(fill_u32(R->bcounts.p, 0u, 2 * (size_t)nb + 2, st));
-                }
-                hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, R->hits.p, R->sh.p,
-                                   R->contrib.p, R->qidx.p, R->lbase.p, cnt + 5,
-                                   (unsigned long long*)R->bcounts.p, R->bcounts.p + 2 * (size_t)nb,
-                                   (int)(batched ? SHADE_QUEUE : SHADE_PLAIN));
-                RCHK(hipGetLastError());
-            }
-            RCHK(stage_end(a, T.other));
-            if (slots) {
-                a = stage_begin();
-                if (batched)
-                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st, cnt + 5));
-                else
-                    RCHK(trace_any(sv, R->sh.p, (uint32_t)slots, R->occ.p, path, tunA, scr, st));
-                RCHK(stage_end(a, T.any));
-            }
-            a = stage_begin();
-            hipLaunchKernelGGL(k_accum_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->lbase.p,
-                               R->contrib.p, R->qidx.p, R->occ.p, R->sample_l.p);
-            RCHK(hipGetLastError());
-            // running total of shadow rays
-            hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
-            RCHK(hipGetLastError());
-            RCHK(stage_end(a, T.other));
-        } else {
-            ICHK(R->st.ensure(cap));
-            ICHK(R->rays2.ensure(cap));
-            ICHK(R->sh.ensure(cap));
-            ICHK(R->contrib.ensure(cap));
-            ICHK(R->occ.ensure((size_t)cap / 32 + 2));
-            ICHK(R->pid.ensure(cap));
-            ICHK(R->pid2.ensure(cap));
-            ICHK(R->shpid.ensure(cap));
-            const bool osort = batched && R->octant_sort;
-            if (osort) {
-                ICHK(R->rays3.ensure(cap));
-                ICHK(R->pid3.ensure(cap));
-            }
-            // batched: the packet kernel makes the camera rays and the first
-            // vertex makes them again and starts the paths (no init pass)
-            const bool fused = batched && camera_fusable(sv, tunCam);
-            hipEvent_t a = stage_begin();
-            if (fused) {
-                RCHK(fill_u32(cnt + 1, n, 1, st));
-            } else {
-                hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, R->rays.p, R->st.p,
-                                   batched ? R->pid.p : nullptr, batched ? cnt + 1 : nullptr);
-                RCHK(hipGetLastError());
-            }
-            RCHK(stage_end(a, T.other));
-            Ray* rin = R->rays.p;
-            Ray* rout = R->rays2.p;
-            uint32_t* pin = R->pid.p;
-            uint32_t* pout = R->pid2.p;
-            Ray* rspare = R->rays3.p;
-            uint32_t* pspare = R->pid3.p;
-            uint32_t* cin = cnt + 1;
-            uint32_t* cout = cnt + 2;
-            for (uint32_t bounce = 0; bounce <= P.max_depth; ++bounce) {
-                a = stage_begin();
-                const TraceTuning& tb = bounce == 0 ? tunCam : tunC;  // bounce 0: camera rays
-                const bool gen = fused && bounce == 0;
-                if (gen) {
-                    CamParams cp;
-                    memcpy(cp.r2c, P.r2c, sizeof(cp.r2c));
-                    memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
-                    cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
-                    cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
-                    RCHK(trace_camera(sv, cp, n, R->hits.p, tb, scr, st));
-                } else if (batched) {
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st, cin));
-                } else {
-                    RCHK(trace_closest(sv, rin, n, R->hits.p, path, tb, scr, st));
-                }
-                RCHK(stage_end(a, T.closest));
-                a = stage_begin();
-                RCHK(fill_u32(cout, 0u, 1, st));
-                RCHK(fill_u32(cnt + 5, 0u, 1, st));
-                {
-                    const uint32_t nb = (n + RB - 1) / RB;
-                    // look-back tiles of the two queues (two words per block
-                    // each) and the block ticket
-                    ICHK(R->bcounts.ensure(4 * (size_t)nb + 2));
-                    if (batched) RCHK(fill_u32(R->bcounts.p, 0u, 4 * (size_t)nb + 2, st));
-                    unsigned long long* tiles = (unsigned long long*)R->bcounts.p;
-                    // queue form: persistent blocks over the live paths' tiles
-                    const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
-                    hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(RB), 0, st, P, g0, bounce, n,
-                                       batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
-                                       batched ? (const uint32_t*)pin : (const uint32_t*)nullptr,
-                                       (fused && bounce == 0) ? (const Ray*)nullptr : rin, R->hits.p,
-                                       R->st.p, rout, batched ? pout : nullptr, cout, R->sh.p,
-                                       batched ? R->shpid.p : nullptr, R->contrib.p, cnt + 5, R->sample_l.p, tiles,
-                                       tiles + nb, R->bcounts.p + 4 * (size_t)nb,
-                                       (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
-                    RCHK(hipGetLastError());
-                }
-                RCHK(stage_end(a, T.other));
-                a = stage_begin();
-                if (batched)
-                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st, cnt + 5));
-                else
-                    RCHK(trace_any(sv, R->sh.p, n, R->occ.p, path, tunA, scr, st));
-                RCHK(stage_end(a, T.any));
-                a = stage_begin();
-                hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
-                                   batched ? cnt + 5 : (const uint32_t*)nullptr,
-                                   batched ? R->shpid.p : (const uint32_t*)nullptr, R->contrib.p, R->occ.p, R->st.p,
-                                   R->sample_l.p);
-                RCHK(hipGetLastError());
-                hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
-                if (bounce < P.max_depth)
-                    hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
-                RCHK(hipGetLastError());
-                if (osort && bounce < P.max_depth) {
-                    const uint32_t tiles = (n + OS_TILE - 1) / OS_TILE;
-                    const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)device_cu_count() * 2u);
-                    hipLaunchKernelGGL(k_octant_sort, dim3(grid), dim3(OS_BLOCK), 0, st, (const uint32_t*)cout,
-                                       (const Ray*)rout, (const uint32_t*)pout, rspare, pspare);
-                    RCHK(hipGetLastError());
-                    std::swap(rout, rspare);
-                    std::swap(pout, pspare);
-                }
-                RCHK(stage_end(a, T.other));
-                std::swap(rin, rout);
-                std::swap(pin, pout);
-                std::swap(cin, cout);
-            }
+        if (R->sc.integrator != 0) {
+            ICHK(enqueue_path_batch(P, sv, W, cnt, g0, n, path, R->octant_sort, R->sample_l.p, scr, st, sg));
+            continue;
         }
+        const size_t slots = (size_t)n * P.S;
+        ICHK(W.sh.ensure(slots ? slots : 1));
+        ICHK(W.contrib.ensure(slots ? slots : 1));
+        ICHK(W.qidx.ensure(slots ? slots : 1));
+        ICHK(W.lbase.ensure(n));
+        ICHK(W.occ.ensure((slots + 31) / 32 + 1));
+        hipEvent_t a;
+        // batched: the packet kernel makes the camera rays itself and the
+        // shade kernel makes them again (rays = null): no ray array
+        const bool fused = batched && camera_fusable(sv, tunCam);
+        Ray* crays = fused ? nullptr : W.rays.p;
+        if (!fused) {
+            a = sg.begin();
+            hipLaunchKernelGGL(k_camera, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.rays.p);
+            RCHK(hipGetLastError());
+            RCHK(sg.end(a, &Timing::other));
+        }
+        a = sg.begin();
+        if (fused)
+            RCHK(trace_camera(sv, cam_params(P, g0), n, W.hits.p, tunCam, scr, st));
+        else
+            RCHK(trace_closest(sv, W.rays.p, n, W.hits.p, path, tunCam, scr, st));
+        RCHK(sg.end(a, &Timing::closest));
+        a = sg.begin();
+        RCHK(fill_u32(cnt + 5, 0u, 1, st));
+        {
+            const uint32_t nb = (n + RB - 1) / RB;
+            if (batched) {
+                // look-back tiles (two words per block) and the block ticket
+                ICHK(W.bcounts.ensure(2 * (size_t)nb + 2));
+                RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)nb + 2, st));
+            }
+            hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
+                               W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (unsigned long long*)W.bcounts.p,
+                               W.bcounts.p + 2 * (size_t)nb, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN));
+            RCHK(hipGetLastError());
+        }
+        RCHK(sg.end(a, &Timing::other));
+        if (slots) {
+            a = sg.begin();
+            if (batched)
+                RCHK(trace_any(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st, cnt + 5));
+            else
+                RCHK(trace_any(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st));
+            RCHK(sg.end(a, &Timing::any));
+        }
+        a = sg.begin();
+        hipLaunchKernelGGL(k_accum_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.lbase.p, W.contrib.p,
+                           W.qidx.p, W.occ.p, R->sample_l.p);
+        RCHK(hipGetLastError());
+        // running total of shadow rays
+        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+        RCHK(hipGetLastError());
+        RCHK(sg.end(a, &Timing::other));
     }
     ICHK(R->rgb.ensure((size_t)P.xres * P.yres * 3));
     {
-        hipEvent_t a = stage_begin();
-        hipLaunchKernelGGL(k_film, dim3(grid_for((size_t)P.xres * P.yres)), dim3(RB), 0, st, P, R->sample_l.p,
-                           R->rgb.p);
-        RCHK(hipGetLastError());
-        RCHK(stage_end(a, T.other));
+        hipEvent_t a = sg.begin();
+        RCHK(enqueue_film(P, R->sample_l.p, R->rgb.p, st));
+        RCHK(sg.end(a, &Timing::other));
     }
     *camera_rays = total_camera;
     return PBRTX_OK;
@@ -1352,16 +1325,8 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Timing& T, bo
 // (captured again when the launch tuning changed since).
 static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     hipStream_t st = R->stream;
-    const TraceTuning& tunC = g_tuning[path][0];
-    const TraceTuning& tunA = g_tuning[path][1];
-    const TraceTuning& tunCam = g_tuning[path][2];
     TraceScratch* scr;
-    {
-        const size_t a = trace_spill_bytes(R->scene.view, path, tunC);
-        const size_t b = trace_spill_bytes(R->scene.view, path, tunA);
-        const size_t c = trace_spill_bytes(R->scene.view, path, tunCam);
-        ICHK(stream_scratch(st, std::max(a, std::max(b, c)), &scr));
-    }
+    ICHK(stream_scratch(st, render_spill_bytes(R->sc.scene.view, path), &scr));
     R->events.used = 0;
     Timing T;
     uint64_t total_camera = 0;
@@ -1372,8 +1337,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         if (R->gexec[path]) { (void)hipGraphExecDestroy(R->gexec[path]); R->gexec[path] = nullptr; }
         if (R->graph[path]) { (void)hipGraphDestroy(R->graph[path]); R->graph[path] = nullptr; }
         RCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        Timing none;
-        const int rc = enqueue_frame(R, path, scr, none, false, &R->gcamera[path]);
+        Stager none;
+        const int rc = enqueue_frame(R, path, scr, none, &R->gcamera[path]);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &g);
         if (rc) {
@@ -1390,7 +1355,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         RCHK(hipGraphLaunch(R->gexec[path], st));
         total_camera = R->gcamera[path];
     } else {
-        ICHK(enqueue_frame(R, path, scr, T, true, &total_camera));
+        Stager sg{&R->events, &T, st};
+        ICHK(enqueue_frame(R, path, scr, sg, &total_camera));
         R->warm[path] = true;
     }
     RCHK(hipEventRecord(e_end, st));
@@ -1400,7 +1366,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         float ms = 0.0f;
         RCHK(hipEventElapsedTime(&ms, e_start, e_end));
         stats->ms_total = ms;
-        auto sum = [&](const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double* out) -> int {
+        auto sum = [&](const EventPairs& v, double* out) -> int {
             double t = 0.0;
             for (auto& pr : v) {
                 float x = 0.0f;
@@ -1420,7 +1386,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         stats->camera_rays = total_camera;
         stats->shadow_rays = (uint64_t)c[6] | ((uint64_t)c[7] << 32);
         stats->extension_rays = (uint64_t)c[8] | ((uint64_t)c[9] << 32);
-        stats->build_ms = R->build_ms;
+        stats->build_ms = R->sc.build_ms;
     }
     return PBRTX_OK;
 }
@@ -1501,13 +1467,9 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     Renderer& R = h->r;
     (void)hipDeviceSynchronize();
     R.drop_graphs();
-    scene_free(&R.scene);
-    R.faces.release(); R.mats.release(); R.atris.release(); R.a_first.release(); R.a_count.release();
-    R.a_total.release(); R.a_le.release(); R.points.release();
-    R.rays.release(); R.rays2.release(); R.rays3.release(); R.pid3.release(); R.sh.release(); R.hits.release(); R.contrib.release();
-    R.lbase.release(); R.sample_l.release(); R.qidx.release(); R.occ.release(); R.pid.release();
-    R.pid2.release(); R.shpid.release(); R.counters.release(); R.st.release(); R.rgb.release();
-    R.bcounts.release();
+    R.sc.release();
+    R.W.release();
+    R.sample_l.release(); R.counters.release(); R.rgb.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;
