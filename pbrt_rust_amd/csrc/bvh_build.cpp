@@ -9,7 +9,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 
 namespace px {
 namespace {
@@ -115,16 +117,31 @@ void partition_by(uint32_t* v, size_t nv, const std::vector<f3>& cen, int dim) {
     }
 }
 
-template <typename Pred>
-size_t stable_partition(uint32_t* a, size_t n, Pred pred, uint32_t* tmp) {
-    size_t l = 0, r = 0;
-    for (size_t i = 0; i < n; ++i) {
-        uint32_t x = a[i];
-        if (pred(x)) a[l++] = x;
-        else tmp[r++] = x;
-    }
-    if (r) memcpy(a + l, tmp, r * sizeof(uint32_t));
-    return l;
+// Runs f(k, begin, end) for the chunks k = 0..T-1 of [0, n) (contiguous,
+// equal but the last) on T threads, the calling thread taking chunk 0;
+// T = 1 below 64K items.  Chunk boundaries depend only on (n, threads).
+unsigned g_threads() {
+    static const unsigned t = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    return t;
+}
+size_t chunk_count(size_t n, unsigned threads) {
+    if (threads <= 1 || n < (1u << 16)) return 1;
+    return std::min<size_t>(threads, n / 16384);
+}
+template <typename F>
+void parallel_chunks(size_t n, unsigned threads, F f) {
+    const size_t T = chunk_count(n, threads);
+    if (T == 1) { f((size_t)0, (size_t)0, n); return; }
+    const size_t per = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < T; ++k)
+        th.emplace_back([&f, k, per, n] { f(k, std::min(n, k * per), std::min(n, (k + 1) * per)); });
+    f((size_t)0, (size_t)0, std::min(n, per));
+    for (auto& t : th) t.join();
+}
+template <typename F>
+void parallel_for(size_t n, F f) {
+    parallel_chunks(n, g_threads(), [&f](size_t, size_t b, size_t e) { f(b, e); });
 }
 
 // bucket_for_prim: bvh.rs:118-130 (`as usize` saturates: NaN and negatives -> 0)
@@ -156,7 +173,7 @@ static float cq_decode(float o, uint32_t q, float s) {
 // Quantizes one axis of a record: picks the scale exponent, then for every
 // slot the largest q whose decode is <= lo and the smallest whose decode is
 // >= hi.  Returns false when the union box is not finite or no exponent fits.
-static bool cq_axis(float ulo, float uhi, const float* lo, const float* hi, const bool* used, uint8_t* qlo,
+bool cq_axis(float ulo, float uhi, const float* lo, const float* hi, const bool* used, uint8_t* qlo,
                     uint8_t* qhi, uint32_t* ebyte) {
     if (!(std::fabs(ulo) <= FLT_MAX) || !(std::fabs(uhi) <= FLT_MAX)) return false;
     const double ext = (double)uhi - (double)ulo;
@@ -192,9 +209,11 @@ static bool cq_axis(float ulo, float uhi, const float* lo, const float* hi, cons
 // Builds the compressed layout from the four-wide records and the leaves of
 // the reference tree.  Returns false if some record cannot be represented
 // (non-finite bounds); the exact layout is used then.
-static bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
+bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
     out->cnodes.assign(out->qnodes.size(), CNode{});
-    for (size_t i = 0; i < out->qnodes.size(); ++i) {
+    std::atomic<bool> ok{true};
+    parallel_for(out->qnodes.size(), [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1 && ok.load(std::memory_order_relaxed); ++i) {
         const QNode& q = out->qnodes[i];
         CNode& c = out->cnodes[i];
         bool used[4];
@@ -207,15 +226,21 @@ static bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
             u[3] = std::max(u[3], q.hx[k]); u[4] = std::max(u[4], q.hy[k]); u[5] = std::max(u[5], q.hz[k]);
         }
         uint32_t ex, ey, ez;
-        if (!cq_axis(u[0], u[3], q.lx, q.hx, used, c.q + 0, c.q + 4, &ex)) return false;
-        if (!cq_axis(u[1], u[4], q.ly, q.hy, used, c.q + 8, c.q + 12, &ey)) return false;
-        if (!cq_axis(u[2], u[5], q.lz, q.hz, used, c.q + 16, c.q + 20, &ez)) return false;
+        if (!cq_axis(u[0], u[3], q.lx, q.hx, used, c.q + 0, c.q + 4, &ex) ||
+            !cq_axis(u[1], u[4], q.ly, q.hy, used, c.q + 8, c.q + 12, &ey) ||
+            !cq_axis(u[2], u[5], q.lz, q.hz, used, c.q + 16, c.q + 20, &ez)) {
+            ok.store(false);
+            break;
+        }
         c.ox = u[0]; c.oy = u[1]; c.oz = u[2];
         c.emeta = ex | (ey << 8) | (ez << 16) | ((q.meta & 63u) << 24);
     }
+    });
+    if (!ok.load()) return false;
     // prim records with their leaf's exact box
     out->prims64.assign(out->prims.size(), PrimRec64{});
-    for (size_t i = 0; i < nodes.size(); ++i) {
+    parallel_for(nodes.size(), [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; ++i) {
         if ((nodes[i].meta & 3u) != 3u) continue;
         const uint32_t first = nodes[i].off, cnt = nodes[i].meta >> 2;
         for (uint32_t k = 0; k < cnt; ++k) {
@@ -229,8 +254,265 @@ static bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
             r.w[15] = p.w[11];
         }
     }
+    });
     return true;
 }
+
+namespace {
+
+constexpr uint32_t UNSET = 0xFFFFFFFFu;
+
+// The permutation being partitioned, with each prim's bounds and centroid
+// carried along (sob[i] = bounds of prim ord[i], soc[i] its centroid), so the
+// passes over a node's range read memory in order; tmp* are scratch with the
+// same indexing.
+struct BuildCtx {
+    const Box* pb;                // prim bounds, by prim id
+    const std::vector<f3>* pc;    // prim centroids, by prim id
+    uint32_t* ord;
+    Box* sob;
+    f3* soc;
+    uint32_t* tmp;
+    Box* tmpb;
+    f3* tmpc;
+    int split;
+    uint32_t max_prims;
+    unsigned threads;  // for the passes over one node's range
+};
+
+// Stable partition of positions [b, b+n) by keep(i) (ord, bounds and
+// centroids together); returns the size of the kept front.
+template <typename Keep>
+size_t stable_partition3(const BuildCtx& c, uint32_t b, uint32_t n, Keep keep);
+
+// The same partition over T chunks: kept items in order, then the others in
+// order, through the scratch arrays.
+template <typename Keep>
+size_t stable_partition3_par(const BuildCtx& c, uint32_t b, uint32_t n, Keep keep) {
+    const size_t T = chunk_count(n, c.threads);
+    if (T == 1) return stable_partition3(c, b, n, keep);
+    std::vector<size_t> kept(T, 0);
+    parallel_chunks(n, c.threads, [&](size_t k, size_t i0, size_t i1) {
+        size_t m = 0;
+        for (size_t i = i0; i < i1; ++i) m += keep((uint32_t)i) ? 1 : 0;
+        kept[k] = m;
+    });
+    std::vector<size_t> lo(T), hi(T);
+    size_t tk = 0;
+    for (size_t k = 0; k < T; ++k) { lo[k] = tk; tk += kept[k]; }
+    const size_t per = (n + T - 1) / T;
+    for (size_t k = 0; k < T; ++k) hi[k] = tk + std::min<size_t>(n, k * per) - lo[k];
+    uint32_t* a = c.ord + b;
+    Box* ab = c.sob + b;
+    f3* ac = c.soc + b;
+    uint32_t* t = c.tmp + b;
+    Box* tb = c.tmpb + b;
+    f3* tc = c.tmpc + b;
+    parallel_chunks(n, c.threads, [&](size_t k, size_t i0, size_t i1) {
+        size_t l = lo[k], r = hi[k];
+        for (size_t i = i0; i < i1; ++i) {
+            const size_t d = keep((uint32_t)i) ? l++ : r++;
+            t[d] = a[i]; tb[d] = ab[i]; tc[d] = ac[i];
+        }
+    });
+    parallel_chunks(n, c.threads, [&](size_t, size_t i0, size_t i1) {
+        memcpy(a + i0, t + i0, (i1 - i0) * sizeof(uint32_t));
+        memcpy(ab + i0, tb + i0, (i1 - i0) * sizeof(Box));
+        memcpy(ac + i0, tc + i0, (i1 - i0) * sizeof(f3));
+    });
+    return tk;
+}
+
+template <typename Keep>
+size_t stable_partition3(const BuildCtx& c, uint32_t b, uint32_t n, Keep keep) {
+    uint32_t* a = c.ord + b;
+    Box* ab = c.sob + b;
+    f3* ac = c.soc + b;
+    uint32_t* t = c.tmp + b;
+    Box* tb = c.tmpb + b;
+    f3* tc = c.tmpc + b;
+    size_t l = 0, r = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (keep(i)) {
+            a[l] = a[i]; ab[l] = ab[i]; ac[l] = ac[i];
+            ++l;
+        } else {
+            t[r] = a[i]; tb[r] = ab[i]; tc[r] = ac[i];
+            ++r;
+        }
+    }
+    if (r) {
+        memcpy(a + l, t, r * sizeof(uint32_t));
+        memcpy(ab + l, tb, r * sizeof(Box));
+        memcpy(ac + l, tc, r * sizeof(f3));
+    }
+    return l;
+}
+
+// A pre-order run of the tree: nodes with ids local to the piece (an
+// interior node's off is its second child's local id, a leaf's off the
+// global position of its first prim) and their boxes (leaves only; interior
+// boxes are folded after splicing).
+struct Piece {
+    struct Def {
+        uint32_t at;     // local id the subtree's first node would have taken
+        uint32_t b, e, depth;
+        int64_t parent;  // local id of the parent when it is a second child, else -1
+    };
+    std::vector<RNode> nodes;
+    std::vector<Box> nb;
+    std::vector<Def> deferred;  // set-aside subtrees, in pre-order
+    uint32_t maxdepth = 0;
+    bool ok = true;
+    std::string err;
+};
+
+// recursive_build (bvh.rs:189-258) of the subtree over ord[b, e) at `depth`,
+// iteratively in pre-order.  Subtrees (other than the piece's root) of
+// 4096..defer_max prims are not built but listed in out->deferred.
+bool build_range(const BuildCtx& c, uint32_t b0, uint32_t e0, uint32_t depth0, uint32_t defer_max, Piece* out) {
+    const std::vector<f3>& pc = *c.pc;
+    uint32_t* ord = c.ord;
+    const Box* sob = c.sob;
+    const f3* soc = c.soc;
+    std::vector<RNode>& nodes = out->nodes;
+    std::vector<Box>& nb = out->nb;
+    struct Task {
+        uint32_t b, e;
+        int64_t parent;
+        uint32_t depth;
+    };
+    std::vector<Task> stack;
+    stack.push_back(Task{b0, e0, -1, depth0});
+    std::vector<uint8_t> bk;  // bucket per position of the current range
+    bool root = true;
+    while (!stack.empty()) {
+        Task T = stack.back();
+        stack.pop_back();
+        const uint32_t b = T.b, e = T.e, n = e - b;
+        if (!root && n >= 4096 && n <= defer_max) {
+            out->deferred.push_back(Piece::Def{(uint32_t)nodes.size(), b, e, T.depth, T.parent});
+            continue;
+        }
+        root = false;
+        const uint32_t id = (uint32_t)nodes.size();
+        nodes.push_back(RNode{});
+        nodes.back().off = UNSET;
+        nb.push_back(Box{});
+        if (T.parent >= 0) nodes[(size_t)T.parent].off = id;
+        out->maxdepth = std::max(out->maxdepth, T.depth);
+        // recursive_build bbox: bvh.rs:192-194 (fold in the range's order)
+        // min / max folds: any grouping gives the same box
+        const size_t NC = chunk_count(n, c.threads);
+        Box bb = box_empty();
+        if (NC == 1) {
+            for (uint32_t i = b; i < e; ++i) bb = box_union(bb, sob[i]);
+        } else {
+            std::vector<Box> part(NC, box_empty());
+            parallel_chunks(n, c.threads, [&](size_t k, size_t i0, size_t i1) {
+                Box x = box_empty();
+                for (size_t i = b + i0; i < b + i1; ++i) x = box_union(x, sob[i]);
+                part[k] = x;
+            });
+            for (auto& x : part) bb = box_union(bb, x);
+        }
+        auto make_leaf = [&]() {
+            nodes[id].off = b;
+            nodes[id].meta = 3u | (n << 2);
+            nb[id] = bb;
+        };
+        if (n == 1) { make_leaf(); continue; }
+        Box cb = box_empty();
+        if (NC == 1) {
+            for (uint32_t i = b; i < e; ++i) cb = box_union_p(cb, soc[i]);
+        } else {
+            std::vector<Box> part(NC, box_empty());
+            parallel_chunks(n, c.threads, [&](size_t k, size_t i0, size_t i1) {
+                Box x = box_empty();
+                for (size_t i = b + i0; i < b + i1; ++i) x = box_union_p(x, soc[i]);
+                part[k] = x;
+            });
+            for (auto& x : part) cb = box_union(cb, x);
+        }
+        const int dim = max_extent(cb);
+        const float cmin = comp(cb.lo, dim), cmax = comp(cb.hi, dim);
+        if (cmin == cmax) { make_leaf(); continue; }
+        uint32_t mid;
+        bool sah_leaf = false;
+        if (c.split == 1) {
+            // split_middle: bvh.rs:86-90
+            float pmid = 0.5f * (cmin + cmax);
+            mid = b + (uint32_t)stable_partition3_par(c, b, n, [&](uint32_t i) { return comp(soc[b + i], dim) < pmid; });
+        } else if (c.split == 2 || n <= 4) {
+            partition_by(ord + b, n, pc, dim);
+            for (uint32_t i = b; i < e; ++i) {  // carry bounds and centroids along
+                c.sob[i] = c.pb[ord[i]];
+                c.soc[i] = pc[ord[i]];
+            }
+            mid = b + n / 2;
+        } else {
+            // split_surface_area_heuristic: bvh.rs:105-186
+            uint32_t cnt[12] = {0};
+            Box bbk[12];
+            for (int k = 0; k < 12; ++k) bbk[k] = box_empty();
+            bk.resize(n);
+            // bucket counts and boxes per chunk, merged (exact in any grouping)
+            std::vector<uint32_t> pcnt(NC * 12, 0);
+            std::vector<Box> pbk(NC * 12, box_empty());
+            parallel_chunks(n, c.threads, [&](size_t ck, size_t i0, size_t i1) {
+                uint32_t* cn = pcnt.data() + 12 * ck;
+                Box* bx = pbk.data() + 12 * ck;
+                for (size_t i = i0; i < i1; ++i) {
+                    int k = bucket_for(comp(soc[b + i], dim), cmin, cmax);
+                    bk[i] = (uint8_t)k;
+                    cn[k] += 1;
+                    bx[k] = box_union(bx[k], sob[b + i]);
+                }
+            });
+            for (size_t ck = 0; ck < NC; ++ck)
+                for (int k = 0; k < 12; ++k) {
+                    cnt[k] += pcnt[12 * ck + k];
+                    bbk[k] = box_union(bbk[k], pbk[12 * ck + k]);
+                }
+            const float tsa = surface_area(bb);
+            int best = 0;
+            float mc = FLT_MAX;
+            Box b0 = box_empty();
+            uint32_t c0 = 0;
+            for (int i = 0; i < 11; ++i) {
+                c0 += cnt[i];
+                b0 = box_union(b0, bbk[i]);  // fold over buckets 0..i in order
+                Box b1 = box_empty();
+                uint32_t c1 = 0;
+                for (int k = i + 1; k < 12; ++k) { c1 += cnt[k]; b1 = box_union(b1, bbk[k]); }
+                float b0sa = (float)c0 * surface_area(b0);
+                float b1sa = (float)c1 * surface_area(b1);
+                float cost = 0.125f * (b0sa + b1sa) / tsa;
+                if (cost < mc) { best = i; mc = cost; }
+            }
+            size_t mcu = (mc != mc || mc <= 0.0f) ? 0 : (size_t)mc;
+            if (c.max_prims < n || mcu < n) {
+                // stable partition on the precomputed buckets
+                mid = b + (uint32_t)stable_partition3_par(c, b, n, [&](uint32_t i) { return (int)bk[i] <= best; });
+            } else {
+                sah_leaf = true;
+                mid = b;
+            }
+        }
+        if (sah_leaf) { make_leaf(); continue; }
+        if (mid == b || mid == e) {
+            out->ok = false;
+            out->err = "BVH split produced an empty side";  // bvh.rs:237-238 asserts
+            return false;
+        }
+        nodes[id].meta = (uint32_t)dim;
+        stack.push_back(Task{mid, e, (int64_t)id, T.depth + 1});
+        stack.push_back(Task{b, mid, -1, T.depth + 1});
+    }
+    return true;
+}
+
+}  // namespace
 
 bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
                const SphereIn* sph, uint32_t nsph, int split, uint32_t max_prims, HostBvh* out,
@@ -249,13 +531,15 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
     std::vector<f3> pc(N);
     auto V = [&](uint32_t i) { return mk3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]); };
     // Triangle::world_bound: mesh.rs:196-205 over the refined triple [c,b,a]
-    for (uint32_t f = 0; f < nfaces; ++f) {
-        Box b = box_empty();
-        b = box_union_p(b, V(idx[3 * f + 2]));
-        b = box_union_p(b, V(idx[3 * f + 1]));
-        b = box_union_p(b, V(idx[3 * f + 0]));
-        pb[f] = b;
-    }
+    parallel_for(nfaces, [&](size_t f0, size_t f1) {
+        for (size_t f = f0; f < f1; ++f) {
+            Box b = box_empty();
+            b = box_union_p(b, V(idx[3 * f + 2]));
+            b = box_union_p(b, V(idx[3 * f + 1]));
+            b = box_union_p(b, V(idx[3 * f + 0]));
+            pb[f] = b;
+        }
+    });
     out->spheres.resize(nsph);
     for (uint32_t s = 0; s < nsph; ++s) {
         const SphereIn& in = sph[s];
@@ -285,123 +569,116 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
         b = box_union_p(b, add3(add3(add3(tp, tx), ty), tz));
         pb[nfaces + s] = b;
     }
-    for (uint32_t i = 0; i < N; ++i) pc[i] = mul3(add3(pb[i].lo, pb[i].hi), 0.5f);  // bvh.rs:77
+    parallel_for(N, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) pc[i] = mul3(add3(pb[i].lo, pb[i].hi), 0.5f);  // bvh.rs:77
+    });
 
     std::vector<uint32_t> ord(N), tmp(N);
     for (uint32_t i = 0; i < N; ++i) ord[i] = i;
-    std::vector<RNode>& nodes = out->rnodes;
-    nodes.clear();
-    nodes.reserve(2 * (size_t)N);
-    std::vector<Box> nb;
-    nb.reserve(2 * (size_t)N);
+    std::vector<Box> sob(pb), tmpb(N);
+    std::vector<f3> soc(pc), tmpc(N);
+    const unsigned hw = g_threads();
+    const BuildCtx ctx{pb.data(), &pc, ord.data(), sob.data(), soc.data(), tmp.data(), tmpb.data(), tmpc.data(),
+                       split, max_prims, hw};
+    BuildCtx ctx1 = ctx;  // the set-aside subtrees: one thread each
+    ctx1.threads = 1;
 
-    struct Task {
-        uint32_t b, e;
-        int64_t parent;
-        uint32_t depth;
+    // The top of the tree is built here, subtrees of a middling size are
+    // set aside and built on worker threads (they touch disjoint ranges of
+    // ord and tmp, and each is the same sequence of operations as in a
+    // one-thread build), then all is spliced together in pre-order.
+    const uint32_t defer_max = (N < 200000 || hw == 1) ? 0u : std::max<uint32_t>(N / (8 * hw), 50000u);
+    Piece top;
+    if (!build_range(ctx, 0, N, 0, defer_max, &top)) { *err = top.err; return false; }
+    std::vector<Piece> pieces(top.deferred.size());
+    {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < pieces.size();) {
+                const Piece::Def& d = top.deferred[k];
+                Piece& pc_ = pieces[k];
+                if (!build_range(ctx1, d.b, d.e, d.depth, 0u, &pc_)) continue;
+                // interior boxes of the piece (children follow parents)
+                for (size_t i = pc_.nodes.size(); i-- > 0;)
+                    if ((pc_.nodes[i].meta & 3u) != 3u) pc_.nb[i] = box_union(pc_.nb[i + 1], pc_.nb[pc_.nodes[i].off]);
+            }
+        };
+        std::vector<std::thread> th;
+        const unsigned nt = (unsigned)std::min<size_t>(hw, pieces.size());
+        for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    }
+    for (auto& pc_ : pieces)
+        if (!pc_.ok) { *err = pc_.err; return false; }
+    // final ids: top node i goes after the pieces deferred before it
+    const size_t ntop = top.nodes.size();
+    std::vector<uint32_t> top_id(ntop + 1);
+    std::vector<uint32_t> piece_base(pieces.size());
+    {
+        uint32_t id = 0;
+        size_t k = 0;
+        for (size_t i = 0; i <= ntop; ++i) {
+            for (; k < pieces.size() && top.deferred[k].at == i; ++k) {
+                piece_base[k] = id;
+                id += (uint32_t)pieces[k].nodes.size();
+            }
+            top_id[i] = id;
+            if (i < ntop) ++id;
+        }
+    }
+    std::vector<RNode>& nodes = out->rnodes;
+    std::vector<Box> nb;
+    {
+        size_t total = ntop;
+        for (auto& pc_ : pieces) total += pc_.nodes.size();
+        nodes.assign(total, RNode{});
+        nb.assign(total, Box{});
+    }
+    uint32_t maxdepth = top.maxdepth;
+    auto set_box = [](RNode& r, const Box& x) {
+        r.lo[0] = x.lo.x; r.lo[1] = x.lo.y; r.lo[2] = x.lo.z;
+        r.hi[0] = x.hi.x; r.hi[1] = x.hi.y; r.hi[2] = x.hi.z;
     };
-    std::vector<Task> stack;
-    stack.push_back(Task{0, N, -1, 0});
-    uint32_t maxdepth = 0;
-    std::vector<uint8_t> bk;  // bucket per position of the current range
-    while (!stack.empty()) {
-        Task T = stack.back();
-        stack.pop_back();
-        const uint32_t id = (uint32_t)nodes.size();
-        nodes.push_back(RNode{});
-        nb.push_back(Box{});
-        if (T.parent >= 0) nodes[(size_t)T.parent].off = id;
-        maxdepth = std::max(maxdepth, T.depth);
-        const uint32_t b = T.b, e = T.e, n = e - b;
-        // recursive_build bbox: bvh.rs:192-194 (fold in the range's order)
-        Box bb = box_empty();
-        for (uint32_t i = b; i < e; ++i) bb = box_union(bb, pb[ord[i]]);
-        auto make_leaf = [&]() {
-            nodes[id].off = b;
-            nodes[id].meta = 3u | (n << 2);
-            nb[id] = bb;
-        };
-        if (n == 1) { make_leaf(); continue; }
-        Box cb = box_empty();
-        for (uint32_t i = b; i < e; ++i) cb = box_union_p(cb, pc[ord[i]]);
-        const int dim = max_extent(cb);
-        const float cmin = comp(cb.lo, dim), cmax = comp(cb.hi, dim);
-        if (cmin == cmax) { make_leaf(); continue; }
-        uint32_t mid;
-        bool sah_leaf = false;
-        auto equal_counts = [&]() {
-            partition_by(ord.data() + b, n, pc, dim);
-            return b + n / 2;
-        };
-        if (split == 1) {
-            // split_middle: bvh.rs:86-90
-            float pmid = 0.5f * (cmin + cmax);
-            mid = b + (uint32_t)stable_partition(
-                          ord.data() + b, n, [&](uint32_t p) { return comp(pc[p], dim) < pmid; }, tmp.data());
-        } else if (split == 2 || n <= 4) {
-            mid = equal_counts();
-        } else {
-            // split_surface_area_heuristic: bvh.rs:105-186
-            uint32_t cnt[12] = {0};
-            Box bbk[12];
-            for (int k = 0; k < 12; ++k) bbk[k] = box_empty();
-            bk.resize(n);
-            for (uint32_t i = 0; i < n; ++i) {
-                uint32_t p = ord[b + i];
-                int k = bucket_for(comp(pc[p], dim), cmin, cmax);
-                bk[i] = (uint8_t)k;
-                cnt[k] += 1;
-                bbk[k] = box_union(bbk[k], pb[p]);
-            }
-            const float tsa = surface_area(bb);
-            int best = 0;
-            float mc = FLT_MAX;
-            Box b0 = box_empty();
-            uint32_t c0 = 0;
-            for (int i = 0; i < 11; ++i) {
-                c0 += cnt[i];
-                b0 = box_union(b0, bbk[i]);  // fold over buckets 0..i in order
-                Box b1 = box_empty();
-                uint32_t c1 = 0;
-                for (int k = i + 1; k < 12; ++k) { c1 += cnt[k]; b1 = box_union(b1, bbk[k]); }
-                float b0sa = (float)c0 * surface_area(b0);
-                float b1sa = (float)c1 * surface_area(b1);
-                float cost = 0.125f * (b0sa + b1sa) / tsa;
-                if (cost < mc) { best = i; mc = cost; }
-            }
-            size_t mcu = (mc != mc || mc <= 0.0f) ? 0 : (size_t)mc;
-            if (max_prims < n || mcu < n) {
-                // stable partition on the precomputed buckets
-                uint32_t* a = ord.data() + b;
-                size_t l = 0, r = 0;
-                for (uint32_t i = 0; i < n; ++i) {
-                    uint32_t x = a[i];
-                    if ((int)bk[i] <= best) a[l++] = x;
-                    else tmp[r++] = x;
+    for (size_t i = 0; i < ntop; ++i) {
+        RNode r = top.nodes[i];
+        if ((r.meta & 3u) != 3u && r.off != UNSET) r.off = top_id[r.off];
+        nodes[top_id[i]] = r;
+        nb[top_id[i]] = top.nb[i];
+    }
+    // the pieces (boxes complete) at their final ids
+    {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < pieces.size();) {
+                const Piece& pc_ = pieces[k];
+                const uint32_t base = piece_base[k];
+                for (size_t i = 0; i < pc_.nodes.size(); ++i) {
+                    RNode r = pc_.nodes[i];
+                    if ((r.meta & 3u) != 3u) r.off += base;
+                    set_box(r, pc_.nb[i]);
+                    nodes[base + i] = r;
+                    nb[base + i] = pc_.nb[i];
                 }
-                if (r) memcpy(a + l, tmp.data(), r * sizeof(uint32_t));
-                mid = b + (uint32_t)l;
-            } else {
-                sah_leaf = true;
-                mid = b;
             }
-        }
-        if (sah_leaf) { make_leaf(); continue; }
-        if (mid == b || mid == e) {
-            *err = "BVH split produced an empty side";  // bvh.rs:237-238 asserts
-            return false;
-        }
-        nodes[id].meta = (uint32_t)dim;
-        stack.push_back(Task{mid, e, (int64_t)id, T.depth + 1});
-        stack.push_back(Task{b, mid, -1, T.depth + 1});
+        };
+        std::vector<std::thread> th;
+        const unsigned nt = (unsigned)std::min<size_t>(hw, pieces.size());
+        for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
     }
-    // interior bounds = left ∪ right (bvh.rs:247), children follow parents
-    for (size_t i = nodes.size(); i-- > 0;) {
-        if ((nodes[i].meta & 3u) != 3u) nb[i] = box_union(nb[i + 1], nb[nodes[i].off]);
+    for (size_t k = 0; k < pieces.size(); ++k) {
+        // a deferred second child: its parent's second-child id
+        if (top.deferred[k].parent >= 0) nodes[top_id[(size_t)top.deferred[k].parent]].off = piece_base[k];
+        maxdepth = std::max(maxdepth, pieces[k].maxdepth);
     }
-    for (size_t i = 0; i < nodes.size(); ++i) {
-        nodes[i].lo[0] = nb[i].lo.x; nodes[i].lo[1] = nb[i].lo.y; nodes[i].lo[2] = nb[i].lo.z;
-        nodes[i].hi[0] = nb[i].hi.x; nodes[i].hi[1] = nb[i].hi.y; nodes[i].hi[2] = nb[i].hi.z;
+    pieces.clear();
+    // interior bounds of the top = left ∪ right (bvh.rs:247), children first
+    for (size_t i = ntop; i-- > 0;) {
+        const size_t f = top_id[i];
+        if ((nodes[f].meta & 3u) != 3u) nb[f] = box_union(nb[f + 1], nb[nodes[f].off]);
+        set_box(nodes[f], nb[f]);
     }
     out->depth = maxdepth;
     out->order = ord;
@@ -412,7 +689,8 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
 
     // primitive records in leaf order
     out->prims.resize(N);
-    for (uint32_t k = 0; k < N; ++k) {
+    parallel_for(N, [&](size_t k0, size_t k1) {
+    for (size_t k = k0; k < k1; ++k) {
         uint32_t id = ord[k];
         PrimRec& r = out->prims[k];
         memset(&r, 0, sizeof(r));
@@ -429,9 +707,11 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
             r.w[11] = bits_f(1u);
         }
     }
+    });
 
     // the last record of every leaf carries bit 8 of the kind word
-    for (size_t i = 0; i < nodes.size(); ++i) {
+    parallel_for(nodes.size(), [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; ++i) {
         if ((nodes[i].meta & 3u) != 3u) continue;
         uint32_t cnt = nodes[i].meta >> 2;
         PrimRec& r = out->prims[nodes[i].off + cnt - 1];
@@ -439,6 +719,7 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
         memcpy(&kind, &r.w[11], 4);
         r.w[11] = bits_f(kind | 0x100u);
     }
+    });
 
     // four-wide layout: records in depth-first pre-order of the record tree
     auto is_leaf = [&](size_t i) { return (nodes[i].meta & 3u) == 3u; };
@@ -467,7 +748,8 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
         }
     }
     out->qnodes.assign(order_q.size(), QNode{});
-    for (size_t qi = 0; qi < order_q.size(); ++qi) {
+    parallel_for(order_q.size(), [&](size_t q0, size_t q1) {
+    for (size_t qi = q0; qi < q1; ++qi) {
         const size_t a = order_q[qi];
         QNode& q = out->qnodes[qi];
         size_t s[4];
@@ -488,6 +770,7 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
             q.hx[k] = c.hi[0]; q.hy[k] = c.hi[1]; q.hz[k] = c.hi[2];
         }
     }
+    });
     out->qroot = is_leaf(0) ? ~(int32_t)nodes[0].off : 0;
     out->qdepth = qdepth;
     if (!compress_layout(nodes, out)) {

@@ -54,6 +54,19 @@ def test_icosphere_and_quads_same_tree():
     assert bvh.info()["nodes"] == 2 * (s["idx"].shape[0] - 2) - 1
 
 
+@pytest.mark.parametrize("split", ["sah", "middle", "equal"])
+def test_large_trees_same_tree(split):
+    # large enough for the builder to set subtrees aside and build them on
+    # worker threads: the spliced tree must still be the oracle's
+    v, idx = scenes.random_triangles(300000, 77)
+    bvh, orc = mesh_pair(v, idx, split)
+    compare(bvh, orc)
+    if split == "sah":
+        s = scenes.direct_lighting_scene(level=7)
+        bvh, orc = mesh_pair(s["verts"], s["idx"], "sah")
+        compare(bvh, orc)
+
+
 def test_degenerate_centroids_make_one_leaf():
     # identical triangles: centroid bounds collapse -> one leaf with all prims (bvh.rs:212-220)
     v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (7, 1))
