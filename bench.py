@@ -52,6 +52,13 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=600, help="requests in the fixed-rate run")
     ap.add_argument("--serve-batch", type=int, default=1 << 22, help="rays per serving step")
     ap.add_argument("--serve-arena-gb", type=int, default=8)
+    ap.add_argument("--no-render-serve", action="store_true", help="skip the render-request serving measurement")
+    ap.add_argument("--rs-small", action="store_true",
+                    help="render serving without the 64M-triangle scene (16 scenes only)")
+    ap.add_argument("--rs-pool", type=int, default=32, help="distinct render requests in the pool")
+    ap.add_argument("--rs-requests", type=int, default=96, help="render requests in the fixed-rate run")
+    ap.add_argument("--rs-batch", type=int, default=1 << 23, help="paths per render-serving batch")
+    ap.add_argument("--rs-arena-gb", type=int, default=40)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--verbose", action="store_true")
@@ -69,8 +76,8 @@ def copy_calibration(nbytes):
 
 
 def log(args, *a):
-    if args.verbose:
-        print(*a, file=sys.stderr, flush=True)
+    """progress lines on stderr (the result is the one JSON line on stdout)"""
+    print(*a, file=sys.stderr, flush=True)
 
 
 def main():
@@ -130,6 +137,9 @@ def main():
     workloads = None
     if not args.no_workloads:
         workloads = other_workloads(args, px, scenes, lambda m: log(args, m))
+    render_serve = None
+    if not args.no_render_serve:
+        render_serve = render_serving(args, px, scenes, lambda m: log(args, m))
 
     # ---- roofline of the dominant kernel (per-launch averages, HIP events on
     # the render stream) ----
@@ -202,6 +212,7 @@ def main():
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "serving": serve,
+        "render_serving": render_serve,
         "workloads": workloads,
         "cpu_baseline": cpu,
         "device": px.device_name(),
@@ -376,6 +387,140 @@ def serving(args, px, scenes, log_fn):
         "rate_requests_per_s": round(req_per_s, 2),
         "requests": nreq,
         "latency": {"batched": lat_b, "plain": lat_p},
+    }
+
+
+def render_serving(args, px, scenes, log_fn):
+    """Render-request serving: 16 resident scenes of 15 level-6 displaced
+    icospheres each (1.23M triangles, 19.7M in all) plus one 64M-triangle
+    scene of 782 level-6 displaced icospheres (left out with --rs-small);
+    requests of square images with side log-uniform in 128..2048, spp from
+    {1, 4, 16, 64}, path depth 5 with roulette after bounce 3, a look-at
+    camera on a sphere around the scene.  Saturated throughput with every
+    request queued at once, then p50/p99 latency under Poisson arrivals at 70%
+    of the batched server's saturated request rate, for the batched server
+    (progressive tiles of the requests of a scene in shared wavefront
+    batches) and for the plain one (one request at a time on the plain path).
+    Latency includes the film and its copy to page-locked host memory."""
+    import concurrent.futures as cf
+    import threading
+    t0 = time.time()
+    integ = px.PathIntegrator(5, 3)
+    srv = px.RenderServer(arena_bytes=args.rs_arena_gb << 30, batch_samples=args.rs_batch, mode="batched")
+    defs = [(15, 10 + k, 10.0) for k in range(16)]
+    if not args.rs_small:
+        defs.append((782, 99, 40.0))
+    tris = [0] * len(defs)
+
+    def add(k):
+        n, seed, ext = defs[k]
+        sc = scenes.sphere_field(n, seed=seed, extent=ext)
+        tris[k] = int(sc["idx"].shape[0])
+        sid = srv.add_scene(sc, integ)
+        log_fn("render serving: scene %d (%d triangles) resident after %.1fs" % (sid, tris[k], time.time() - t0))
+        return sid
+
+    scenes.displaced_pool()
+    with cf.ThreadPoolExecutor(4) as ex:  # the host builds release the interpreter lock
+        ids = list(ex.map(add, range(16)))
+    if not args.rs_small:
+        ids.append(add(16))
+    sid = dict(zip(range(len(defs)), ids))  # server ids come in completion order
+    rng = np.random.default_rng(0xC4)
+    pool = []
+    for j in range(args.rs_pool):
+        s = int(rng.integers(0, len(defs)))
+        side = int(round(np.exp(rng.uniform(np.log(128), np.log(2048)))))
+        k = int(rng.choice([1, 2, 4, 8]))
+        u = rng.normal(size=3)
+        eye = u / np.linalg.norm(u) * defs[s][2] * 3.2
+        cam = scenes.look_at(side, side, eye, (0.0, 0.0, 0.0), fov=60.0)
+        out = px.PinnedArray((side, side, 3), np.float32)
+        pool.append((sid[s], cam, px.StratifiedSampler(k, k, 1000 + j), out))
+    samples = [p[1]["xres"] * p[1]["yres"] * p[2].xs * p[2].ys for p in pool]
+    log_fn("render serving: %d scenes, %d-request pool (%.1fM samples) ready in %.1fs"
+           % (len(defs), len(pool), sum(samples) / 1e6, time.time() - t0))
+
+    def rays_of(c):
+        return c["camera_rays"] + c["shadow_rays"] + c["extension_rays"]
+
+    # first batch per scene (trace scratch sizing) outside the clock
+    warm = scenes.look_at(64, 64, (0.0, 0.0, -40.0), (0.0, 0.0, 0.0))
+    for s in ids:
+        srv.wait(srv.submit(s, warm, px.StratifiedSampler(1, 1, 0)))
+
+    def saturated(mode):
+        srv.set_mode(mode)
+        c0 = srv.counters()
+        t = time.perf_counter()
+        tick = [srv.submit(s, cam, smp, out.array) for s, cam, smp, out in pool]
+        for tk in tick:
+            srv.wait(tk)
+        dt = time.perf_counter() - t
+        c1 = srv.counters()
+        return (rays_of(c1) - rays_of(c0)) / dt / 1e6, len(pool) / dt
+
+    def at_rate(mode, req_per_s, nreq):
+        srv.set_mode(mode)
+        arr = np.cumsum(np.random.default_rng(11).exponential(1.0 / req_per_s, nreq))
+        tickets = [None] * nreq
+        ready = threading.Semaphore(0)
+
+        def producer():
+            t0_ = time.perf_counter()
+            for i in range(nreq):
+                while True:
+                    lag = arr[i] - (time.perf_counter() - t0_)
+                    if lag <= 0:
+                        break
+                    if lag > 2e-3:
+                        time.sleep(lag - 1e-3)
+                s, cam, smp, out = pool[i % len(pool)]
+                tickets[i] = srv.submit(s, cam, smp, out.array)
+                ready.release()
+
+        c0 = srv.counters()
+        th = threading.Thread(target=producer)
+        t = time.perf_counter()
+        th.start()
+        lat = []
+        for i in range(nreq):
+            ready.acquire()
+            _, st = srv.wait(tickets[i])
+            lat.append(st["latency_ms"])
+        th.join()
+        dt = time.perf_counter() - t
+        c1 = srv.counters()
+        lat = np.array(lat)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3),
+                "achieved_mrays_per_s": round((rays_of(c1) - rays_of(c0)) / dt / 1e6, 2)}
+
+    sat_b, rps_b = saturated("batched")
+    sat_p, rps_p = saturated("plain")
+    log_fn("render serving saturated: batched %.1f Mrays/s (%.2f req/s), plain %.1f Mrays/s (%.2f req/s)"
+           % (sat_b, rps_b, sat_p, rps_p))
+    req_per_s = 0.7 * rps_b
+    lat_b = at_rate("batched", req_per_s, args.rs_requests)
+    lat_p = at_rate("plain", req_per_s, args.rs_requests)
+    log_fn("render serving at %.2f req/s: batched %s plain %s" % (req_per_s, lat_b, lat_p))
+    c = srv.counters()
+    srv.close()
+    for p in pool:
+        p[3].free()
+    return {
+        "workload": "%d resident scenes of level-6 displaced icospheres (%d triangles in all, largest %d), "
+                    "%d-request pool: square images of side 128-2048 (log-uniform), spp from {1,4,16,64}, "
+                    "path depth 5, roulette after bounce 3, look-at cameras; films copied to page-locked host "
+                    "memory (latency includes the copy)" % (len(defs), sum(tris), max(tris), len(pool)),
+        "saturated_mrays_per_s": {"batched": round(sat_b, 2), "plain": round(sat_p, 2)},
+        "saturated_requests_per_s": {"batched": round(rps_b, 3), "plain": round(rps_p, 3)},
+        "served_over_plain": round(sat_b / sat_p, 3),
+        "rate_requests_per_s": round(req_per_s, 3),
+        "requests": args.rs_requests,
+        "latency": {"batched": lat_b, "plain": lat_p},
+        "batch_paths": args.rs_batch,
+        "scene_bytes": int(c["scene_bytes"]),
     }
 
 

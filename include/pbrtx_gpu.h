@@ -265,6 +265,9 @@ int pbrtx_render_server_create(const pbrtx_render_server_config* cfg, pbrtx_rend
    (path), and its max_depth / rr_start apply to every request of the scene.
    Builds the BVH on the host and uploads it. */
 int pbrtx_render_server_add_scene(pbrtx_render_server* srv, const pbrtx_render_desc* desc, uint32_t* scene_id);
+/* switches between PBRTX_SERVE_BATCHED and PBRTX_SERVE_PLAIN; only while no
+   request is pending (every submitted ticket waited for) */
+int pbrtx_render_server_set_mode(pbrtx_render_server* srv, int32_t mode);
 /* rgb_out: host memory for xres*yres*3 floats (the film's final rgb, as
    pbrtx_renderer_render); it must stay valid until the wait returns */
 int pbrtx_render_server_submit(pbrtx_render_server* srv, const pbrtx_render_request* req, float* rgb_out,

@@ -150,6 +150,7 @@ def lib():
     L.pbrtx_render_server_wait.argtypes = [vp, u64, c.POINTER(RenderRequestStats)]
     L.pbrtx_render_server_stats.argtypes = [vp, c.POINTER(RenderServerCounters)]
     L.pbrtx_render_server_destroy.argtypes = [vp]
+    L.pbrtx_render_server_set_mode.argtypes = [vp, i32]
     if hasattr(L, "pbrtx_renderer_create"):
         L.pbrtx_renderer_create.argtypes = [c.POINTER(RenderDesc), c.POINTER(vp)]
         L.pbrtx_renderer_render.argtypes = [vp, i32, vp, c.POINTER(RenderStats)]

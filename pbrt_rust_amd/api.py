@@ -398,6 +398,10 @@ class RenderServer:
         self._h = h.value
         self._pending = {}
 
+    def set_mode(self, mode):
+        """"batched" or "plain"; only with no request pending."""
+        check(lib().pbrtx_render_server_set_mode(self._h, self.MODES[mode]))
+
     def add_scene(self, scene, integrator, split="sah"):
         """scene: a scene dict (see scenes.py); integrator: a PathIntegrator
         (its depth and roulette start apply to every request of the scene)."""

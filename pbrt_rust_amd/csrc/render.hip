@@ -963,6 +963,7 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
     S->build_ms = S->bvh.build_ms;
     int rc = scene_from_host(S->bvh, &S->scene);
     if (rc) return rc;
+    S->bvh = HostBvh{};  // the device copy is all that renders need
     S->nfaces = d->nfaces;
     auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
     std::vector<FaceShade> fsh(d->nfaces);

@@ -61,7 +61,8 @@ struct RRequest {
     uint32_t batches = 0;
     hipEvent_t out = nullptr;  // film copied out
     bool copying = false;
-    int status = 0;          // 0 queued, 1 admitted, 2 done, < 0 error code
+    int status = 0;          // published (under the lock): 0 pending, 2 done, < 0 error code
+    int state = 0;           // the worker's own: 0 queued, 1 admitted, 2 done, < 0 error code
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
 };
 
@@ -121,21 +122,38 @@ struct RenderServer {
     int serve_plain(RRequest* r);
     int start_film(RRequest* r, hipStream_t st);
     void finish(RRequest* r, int status);
+    void publish();
+    std::vector<RRequest*> to_publish;  // finished, not yet published
     void poll_copies();
 };
 
+// Ends r on the worker: frees its device resources and queues its result for
+// publication.  The worker touches r no more once publish() has run, since
+// a waiter may then retire (and free) it.
 void RenderServer::finish(RRequest* r, int status) {
     if (r->bytes) arena.release(r->off, r->bytes);
     r->bytes = 0;
     if (r->out) (void)hipEventDestroy(r->out);
     r->out = nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    r->status = status;
+    r->state = status;
     r->t_done = now_ms();
-    if (status == 2) {
-        samples_done += r->n;
-        ++requests_done;
+    to_publish.push_back(r);
+}
+
+// Drops finished requests from the active list, then makes their results
+// visible to waiters.
+void RenderServer::publish() {
+    active.erase(std::remove_if(active.begin(), active.end(), [](RRequest* r) { return r->state != 1; }), active.end());
+    if (to_publish.empty()) return;
+    std::lock_guard<std::mutex> lk(mu);
+    for (RRequest* r : to_publish) {
+        if (r->state == 2) {
+            samples_done += r->n;
+            ++requests_done;
+        }
+        r->status = r->state;
     }
+    to_publish.clear();
     cv_done.notify_all();
 }
 
@@ -150,7 +168,7 @@ bool RenderServer::admit_one(RRequest* r) {
     {
         std::lock_guard<std::mutex> lk(mu);
         r->t_admit = now_ms();
-        r->status = 1;
+        r->state = 1;
     }
     if (e != hipSuccess) finish(r, PBRTX_ERR_HIP);
     return true;
@@ -168,18 +186,11 @@ int RenderServer::start_film(RRequest* r, hipStream_t st) {
 }
 
 void RenderServer::poll_copies() {
-    for (size_t k = 0; k < active.size();) {
-        RRequest* r = active[k];
-        if (r->status == 1 && r->copying) {
+    for (RRequest* r : active) {
+        if (r->state == 1 && r->copying) {
             hipError_t q = hipEventQuery(r->out);
             if (q == hipSuccess) finish(r, 2);
             else if (q != hipErrorNotReady) finish(r, PBRTX_ERR_HIP);
-        }
-        if (r->status != 1) {
-            active[k] = active.back();
-            active.pop_back();
-        } else {
-            ++k;
         }
     }
 }
@@ -192,7 +203,7 @@ int RenderServer::launch_batches() {
     {
         std::vector<RRequest*> cand;
         for (RRequest* r : active)
-            if (r->status == 1 && r->next < r->n) cand.push_back(r);
+            if (r->state == 1 && r->next < r->n) cand.push_back(r);
         std::sort(cand.begin(), cand.end(), [](RRequest* a, RRequest* b) { return a->ticket < b->ticket; });
         for (RRequest* r : cand)
             if (std::find(order.begin(), order.end(), r->scene) == order.end()) order.push_back(r->scene);
@@ -206,7 +217,7 @@ int RenderServer::launch_batches() {
         Slot& S = slots[si];
         std::vector<RRequest*> reqs;
         for (RRequest* r : active)
-            if (r->status == 1 && r->scene == sc && r->next < r->n) reqs.push_back(r);
+            if (r->state == 1 && r->scene == sc && r->next < r->n) reqs.push_back(r);
         std::sort(reqs.begin(), reqs.end(), [](RRequest* a, RRequest* b) { return a->ticket < b->ticket; });
         if (reqs.size() > MAX_RENDER_SEGS) reqs.resize(MAX_RENDER_SEGS);
         const uint64_t cap = cfg.batch_samples;
@@ -306,7 +317,7 @@ int RenderServer::retire(bool wait_oldest) {
         for (auto& pt : S.parts) {
             RRequest* r = pt.first;
             r->done += pt.second;
-            if (r->done == r->n && r->status == 1) {
+            if (r->done == r->n && r->state == 1) {
                 int rc = start_film(r, S.st);
                 if (rc) return rc;
             }
@@ -382,15 +393,14 @@ void RenderServer::run() {
                 std::lock_guard<std::mutex> lk(mu);
                 queued.pop_front();
             }
-            if (r->status == 1) active.push_back(r);
+            if (r->state == 1) active.push_back(r);
         }
         int rc = PBRTX_OK;
         if (cfg.mode == PBRTX_SERVE_PLAIN) {
             if (!active.empty()) {
                 RRequest* r = active.front();
                 rc = serve_plain(r);
-                if (rc == PBRTX_OK && r->status == 1) finish(r, 2);
-                if (r->status != 1) active.erase(active.begin());
+                if (rc == PBRTX_OK && r->state == 1) finish(r, 2);
             }
         } else {
             int launched = launch_batches();
@@ -418,12 +428,14 @@ void RenderServer::run() {
                 S.parts.clear();
             }
             for (RRequest* r : active)
-                if (r->status == 1) finish(r, rc);
+                if (r->state == 1) finish(r, rc);
             active.clear();
             std::lock_guard<std::mutex> lk(mu);
             worker_error = msg;
         }
+        publish();
     }
+    publish();
     for (Slot& S : slots) (void)hipStreamSynchronize(S.st);
 }
 
@@ -450,8 +462,7 @@ int pbrtx_render_server_create(const pbrtx_render_server_config* cfg, pbrtx_rend
     hipError_t e = hipGetDevice(&s.device);
     if (e == hipSuccess) e = s.arena.init(cfg->arena_bytes);
     int rc = PBRTX_OK;
-    const int nslot = cfg->mode == PBRTX_SERVE_PLAIN ? 1 : NSLOT;
-    for (int k = 0; k < nslot && e == hipSuccess && rc == PBRTX_OK; ++k) {
+    for (int k = 0; k < NSLOT && e == hipSuccess && rc == PBRTX_OK; ++k) {
         Slot& S = s.slots[k];
         e = hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&S.start);
@@ -472,6 +483,17 @@ int pbrtx_render_server_create(const pbrtx_render_server_config* cfg, pbrtx_rend
     }
     s.worker = std::thread([&s] { s.run(); });
     *out = p;
+    return PBRTX_OK;
+}
+
+int pbrtx_render_server_set_mode(pbrtx_render_server* p, int32_t mode) {
+    using namespace px;
+    if (!p) return fail(PBRTX_ERR_ARG, "null server");
+    if (mode != PBRTX_SERVE_BATCHED && mode != PBRTX_SERVE_PLAIN) return fail(PBRTX_ERR_ARG, "unknown mode");
+    RenderServer& s = p->s;
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.queued.empty() || !s.all.empty()) return fail(PBRTX_ERR_STATE, "requests in flight");
+    s.cfg.mode = mode;
     return PBRTX_OK;
 }
 

@@ -58,7 +58,8 @@ struct Request {
     bool copying = false;
     uint32_t launched = 0;       // rays handed to launches so far
     uint32_t launches = 0;
-    int status = 0;              // 0 queued, 1 admitted, 2 done, < 0 error code
+    int status = 0;              // published (under the lock): 0 pending, 2 done, < 0 error code
+    int state = 0;               // the worker's own: 0 queued, 1 admitted, 2 done, < 0 error code
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
 };
 
@@ -115,21 +116,38 @@ struct Server {
     int serve_plain(Request* r);
     void start_copy_out(Request* r, hipEvent_t after);
     void finish(Request* r, int status);
+    void publish();
+    std::vector<Request*> to_publish;  // finished, not yet published
 };
 
+// Ends r on the worker: frees its device resources and queues its result for
+// publication.  The worker touches r no more once publish() has run, since
+// a waiter may then retire (and free) it.
 void Server::finish(Request* r, int status) {
     if (r->bytes) arena.release(r->off, r->bytes);
     r->bytes = 0;
     if (r->ready) (void)hipEventDestroy(r->ready);
     if (r->out) (void)hipEventDestroy(r->out);
     r->ready = r->out = nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    r->status = status;
+    r->state = status;
     r->t_done = now_ms();
-    if (status == 2) {
-        rays_done += r->n;
-        ++requests_done;
+    to_publish.push_back(r);
+}
+
+// Drops finished requests from the active list, then makes their results
+// visible to waiters.
+void Server::publish() {
+    active.erase(std::remove_if(active.begin(), active.end(), [](Request* r) { return r->state != 1; }), active.end());
+    if (to_publish.empty()) return;
+    std::lock_guard<std::mutex> lk(mu);
+    for (Request* r : to_publish) {
+        if (r->state == 2) {
+            rays_done += r->n;
+            ++requests_done;
+        }
+        r->status = r->state;
     }
+    to_publish.clear();
     cv_done.notify_all();
 }
 
@@ -149,7 +167,7 @@ bool Server::admit_one(Request* r) {
     {
         std::lock_guard<std::mutex> lk(mu);
         r->t_admit = now_ms();
-        r->status = 1;
+        r->state = 1;
     }
     if (e != hipSuccess) finish(r, PBRTX_ERR_HIP);
     return true;
@@ -175,7 +193,7 @@ void Server::start_copy_out(Request* r, hipEvent_t after) {
 int Server::launch_step() {
     std::vector<Request*> cand;
     for (Request* r : active) {
-        if (r->status != 1) continue;
+        if (r->state != 1) continue;
         if (!r->is_ready) {
             hipError_t q = hipEventQuery(r->ready);
             if (q == hipSuccess) r->is_ready = true;
@@ -272,18 +290,11 @@ int Server::retire_steps(bool wait_oldest) {
         spare.push_back(st);
         inflight.pop_front();
     }
-    for (size_t k = 0; k < active.size();) {
-        Request* r = active[k];
-        if (r->status == 1 && r->copying) {
+    for (Request* r : active) {
+        if (r->state == 1 && r->copying) {
             hipError_t q = hipEventQuery(r->out);
             if (q == hipSuccess) finish(r, 2);
             else if (q != hipErrorNotReady) finish(r, PBRTX_ERR_HIP);
-        }
-        if (r->status != 1) {
-            active[k] = active.back();
-            active.pop_back();
-        } else {
-            ++k;
         }
     }
     return PBRTX_OK;
@@ -319,7 +330,7 @@ int Server::serve_plain(Request* r) {
     ++launches;
     r->launched = r->n;
     start_copy_out(r, st.end);
-    if (r->status != 1) return PBRTX_OK;
+    if (r->state != 1) return PBRTX_OK;
     e = hipEventSynchronize(r->out);
     if (e != hipSuccess) return hip_fail(e, "result copy");
     return PBRTX_OK;
@@ -347,15 +358,14 @@ void Server::run() {
                 std::lock_guard<std::mutex> lk(mu);
                 queued.pop_front();
             }
-            if (r->status == 1) active.push_back(r);
+            if (r->state == 1) active.push_back(r);
         }
         int rc = PBRTX_OK;
         if (cfg.mode == PBRTX_SERVE_PLAIN) {
             if (!active.empty()) {
                 Request* r = active.front();
                 rc = serve_plain(r);
-                if (rc == PBRTX_OK && r->status == 1) finish(r, 2);
-                if (r->status != 1) active.erase(active.begin());
+                if (rc == PBRTX_OK && r->state == 1) finish(r, 2);
             }
         } else {
             // keep two steps in flight: launch while the previous one runs
@@ -379,12 +389,14 @@ void Server::run() {
             (void)hipStreamSynchronize(s_down);
             inflight.clear();
             for (Request* r : active)
-                if (r->status == 1) finish(r, rc);
+                if (r->state == 1) finish(r, rc);
             active.clear();
             std::lock_guard<std::mutex> lk(mu);
             worker_error = msg;
         }
+        publish();
     }
+    publish();
     (void)hipStreamSynchronize(s_comp);
     (void)hipStreamSynchronize(s_down);
 }

@@ -194,6 +194,67 @@ def spheres_scene(n_spheres=64, level=6, seed=2, mirror_frac=0.3, light_radiance
                 area_le=np.array([[light_radiance] * 3], F32), point_lights=np.zeros((0, 6), F32))
 
 
+_DISPLACED_POOL = {}
+
+
+def displaced_pool(count=16, level=6, seed=100):
+    """`count` level-`level` icospheres displaced by fBm with seeds seed..,
+    made once per process (the instances of the sphere fields below)."""
+    key = (count, level, seed)
+    if key not in _DISPLACED_POOL:
+        _DISPLACED_POOL[key] = [displaced_icosphere(level, 0.05, seed + k) for k in range(count)]
+    return _DISPLACED_POOL[key]
+
+
+def sphere_field(n_spheres, seed=3, extent=10.0, level=6, light_radiance=10.0):
+    """n_spheres displaced level-`level` icospheres (instances of a pool of 16
+    fBm-displaced ones) at U[-extent, extent]^3 with scales U(0.3, 1.5) *
+    extent / 10, Lambertian albedo 0.5 or 0.8, under one quad area light
+    above the field (normal pointing down)."""
+    rng = np.random.default_rng(seed)
+    pool = displaced_pool(level=level)
+    nv, nf = pool[0][0].shape[0], pool[0][1].shape[0]
+    k = rng.integers(0, len(pool), n_spheres)
+    c = rng.uniform(-extent, extent, (n_spheres, 3))
+    sc = rng.uniform(0.3, 1.5, n_spheres) * (extent / 10.0)
+    verts = np.empty((n_spheres * nv + 4, 3), F32)
+    idx = np.empty((n_spheres * nf + 2, 3), np.uint32)
+    for j in range(n_spheres):
+        v, f = pool[k[j]]
+        verts[j * nv:(j + 1) * nv] = (v * F32(sc[j]) + c[j].astype(F32)).astype(F32)
+        idx[j * nf:(j + 1) * nf] = f + np.uint32(j * nv)
+    h, w = 1.4 * extent, 0.3 * extent
+    lv, lf = quad([-w, h, -w], [-w, h, w], [w, h, w], [w, h, -w])
+    verts[n_spheres * nv:] = lv
+    idx[n_spheres * nf:] = lf + np.uint32(n_spheres * nv)
+    nfaces = idx.shape[0]
+    face_mat = np.repeat(rng.integers(0, 2, n_spheres).astype(np.uint32), nf)
+    face_mat = np.concatenate([face_mat, np.full(2, 2, np.uint32)])
+    face_light = np.full(nfaces, -1, np.int32)
+    face_light[n_spheres * nf:] = 0
+    mats = np.array([[0, 0.5, 0.5, 0.5], [0, 0.8, 0.8, 0.8], [0, 0.0, 0.0, 0.0]], F32)
+    return dict(verts=verts, idx=idx, face_mat=face_mat, mats=mats, face_light=face_light,
+                area_le=np.array([[light_radiance] * 3], F32), point_lights=np.zeros((0, 6), F32))
+
+
+def look_at(xres, yres, eye, target, fov=60.0, up=(0.0, 1.0, 0.0)):
+    """Perspective camera at eye looking at target (pbrt's LookAt: camera +z
+    toward the target, +y up), as a camera dict like pinhole's."""
+    eye, target, up = (np.asarray(a, np.float64) for a in (eye, target, up))
+    d = target - eye
+    d /= np.linalg.norm(d)
+    r = np.cross(up / np.linalg.norm(up), d)
+    if np.linalg.norm(r) < 1e-9:
+        r = np.cross(np.array([1.0, 0.0, 0.0]), d)
+    r /= np.linalg.norm(r)
+    u = np.cross(d, r)
+    c2w = np.eye(4, dtype=F32)
+    c2w[:3, 0], c2w[:3, 1], c2w[:3, 2], c2w[:3, 3] = r, u, d, eye
+    cam = pinhole(xres, yres, fov)
+    cam["cam_to_world"] = c2w
+    return cam
+
+
 def sub_image_camera(cam, xres, yres):
     """The same view at another resolution."""
     c = dict(cam)
