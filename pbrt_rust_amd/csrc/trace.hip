@@ -322,6 +322,9 @@ __device__ __forceinline__ bool prim_test64(const SceneView& sv, uint32_t pos, c
 
 // SEG: the rays come from a segment table (served launch) instead of one array.
 // CQ: the compressed records (CNode, PrimRec64) instead of QNode / PrimRec.
+// Camera-ray packets dealt to one XCD at a time (see k_closest_packet).
+constexpr uint32_t CAM_RUN = 16;
+
 // Rays a wave of the persistent per-ray kernel claims beyond a refill's
 // need (see k_trace_wide).  On the 1024x1024 direct-lighting frame's 26.5M
 // shadow rays the occlusion kernel took 6.7 ms claiming exactly what idle
@@ -746,11 +749,30 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_closest_packet(SceneView sv, Ray*
     __shared__ unsigned long long s_msk[PWAVES][PSTK];
     const uint32_t lane = __lane_id();
     const uint32_t w = threadIdx.x >> 6;
+    // GEN (camera rays, their count known at launch): a fixed schedule with
+    // no shared counter.  Packets go in runs of CAM_RUN to the 8 XCDs in turn
+    // (blocks are dealt to the XCDs in turn), and an XCD's waves take its
+    // packets in turn: neighbouring pixels share an L2, and every XCD gets
+    // a share of every part of the image.  Otherwise waves claim packets
+    // from the launch's counter.
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t nxcd = (gridDim.x % 8u == 0u) ? 8u : 1u;
+    const uint64_t npk = ((uint64_t)n + 63u) / 64u;
+    const uint32_t xcd = blockIdx.x % nxcd;
+    const uint32_t wstride = (gridDim.x / nxcd) * wpb;
+    uint64_t k = (uint64_t)(blockIdx.x / nxcd) * wpb + w;  // this wave's next packet among its XCD's
     while (true) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(counter, 64u);
-        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-        if (base >= n) break;
+        if constexpr (GEN) {
+            const uint64_t pk = (k / CAM_RUN * nxcd + xcd) * CAM_RUN + k % CAM_RUN;
+            if (pk >= npk) break;
+            base = (uint32_t)(pk * 64u);
+            k += wstride;
+        } else {
+            if (lane == 0) base = atomicAdd(counter, 64u);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+            if (base >= n) break;
+        }
         const uint32_t i = base + lane;
         const bool valid = i < n;
         RayState r{};
