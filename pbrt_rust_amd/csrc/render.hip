@@ -473,9 +473,139 @@ __global__ __launch_bounds__(RB) void k_path_init(RenderParams P, uint64_t g0, u
 }
 
 // One path vertex (PathIntegrator, deviation D1): emission on the first or a
-// specular bounce, one light sample (no MIS), BSDF sample, roulette.
-// Queue form: paths pid_q[0..*q_count) with their rays rays_q / hits_q;
-// plain form (pid_q == nullptr): every path i with rays[i] / hits[i].
+// specular bounce, one light sample (no MIS), BSDF sample, roulette.  Path
+// i of the bounce: queue form, the path pid_q[i] with ray rays_in[i] and hit
+// hits[i]; plain form (pid_q unused), path i.  fresh (rays_in null): the
+// first vertex of new paths whose camera rays the traversal made itself --
+// path i, initial state, the camera ray made again here.  Writes the path's
+// state (and its radiance when it ends with no shadow ray pending) and
+// returns the rays it wants traced.
+struct VertexOut {
+    Ray sray, nray;
+    f3 contrib;
+    uint32_t pid;
+    bool shadow, next;
+};
+__device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, uint32_t bounce, uint32_t i,
+                                            bool queued, bool fresh, const uint32_t* pid_q, const Ray* rays_in,
+                                            const Hit* hits, PathState* st, float4* sample_l, VertexOut& o) {
+    const uint32_t pid = (queued && !fresh) ? pid_q[i] : i;
+    PathState ps;
+    if (fresh) ps = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
+    else ps = st[pid];
+    const bool active = queued || !(ps.flags & 4u);  // plain form: bit 4 marks a finished path
+    Hit h = active ? hits[i] : Hit{NO_HIT, 0, 0, 0};
+    bool want_shadow = false, want_next = false, finish = false;
+    Ray& sray = o.sray;
+    Ray& nray = o.nray;
+    f3 contrib = mk3(0, 0, 0);
+    if (active && h.prim == NO_HIT) {
+        finish = true;  // a miss ends the path (no infinite lights)
+    } else if (active) {
+        Ray r;
+        if (fresh) camera_ray_of(P, g0 + i, &r);
+        else r = rays_in[i];
+        const f3 d = mk3(r.dx, r.dy, r.dz);
+        const f3 p = add3(mk3(r.ox, r.oy, r.oz), mul3(d, h.t));
+        const FaceShade fs = P.faces[h.prim];
+        const f3 nn = ld3(fs.nn);
+        const f3 wo = neg3(d);
+        const float eps = h.t * 5e-4f;
+        f3 beta = ld3(ps.beta), L = ld3(ps.L);
+        const bool specular = (ps.flags & 1u) != 0u;
+        if (bounce == 0 || specular) {
+            f3 le = mk3(0, 0, 0);
+            if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+            L = add3(L, mulc(beta, le));
+        }
+        const MatRec mat = P.mats[fs.mat];
+        const SampleRef sr = sample_ref(P, g0, ps, sample_l);
+        const uint32_t pixel = (uint32_t)(sr.g / sr.spp), s = (uint32_t)(sr.g % sr.spp);
+        float ua[4], ub[4];
+        rng4(sr.seed, pixel, s, 1 + 2 * bounce, ua);
+        rng4(sr.seed, pixel, s, 2 + 2 * bounce, ub);
+        if (P.nlights > 0 && mat.kind == 0u) {
+            uint32_t ln = (uint32_t)(ua[0] * (float)P.nlights);
+            if (ln > P.nlights - 1) ln = P.nlights - 1;
+            f3 c;
+            if (sample_light(P, ln, p, eps, nn, wo, mat, ua[1], ua[2], ua[3], &sray, &c)) {
+                want_shadow = true;
+                contrib = mulc(beta, mul3(c, (float)P.nlights));
+            }
+        }
+        bool cont = bounce != P.max_depth;
+        if (cont) {
+            // BSDF frame (bsdf/mod.rs:69-84) and world_to_local (:103-105)
+            const f3 tn = ld3(fs.tn);
+            const f3 sn = cross3(nn, tn);
+            const f3 wol = mk3(dot3(wo, sn), dot3(wo, tn), dot3(wo, nn));
+            f3 wil, f;
+            float pdf;
+            bool spec;
+            if (mat.kind == 0u) {
+                float dx, dy;
+                concentric_disk(ub[0], ub[1], &dx, &dy);
+                float z = sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
+                wil = mk3(dx, dy, z);
+                if (wol.z < 0.0f) wil.z = -wil.z;
+                pdf = (wol.z * wil.z > 0.0f) ? fabsf(wil.z) * (1.0f / PI_F) : 0.0f;
+                f = mul3(mk3(mat.r, mat.g, mat.b), 1.0f / PI_F);
+                spec = false;
+            } else {
+                // SpecularReflection::sample_f (specular.rs:34-40), Fresnel::noop
+                wil = mk3(-wol.x, -wol.y, wol.z);
+                pdf = 1.0f;
+                f = divs3(mk3(mat.r, mat.g, mat.b), fabsf(wil.z));
+                spec = true;
+            }
+            if (black3(f) || pdf == 0.0f) {
+                cont = false;
+            } else {
+                // local_to_world: bsdf/mod.rs:107-111
+                const f3 wi = mk3(sn.x * wil.x + tn.x * wil.y + nn.x * wil.z,
+                                  sn.y * wil.x + tn.y * wil.y + nn.y * wil.z,
+                                  sn.z * wil.x + tn.z * wil.y + nn.z * wil.z);
+                ps.flags = spec ? (ps.flags | 1u) : (ps.flags & ~1u);
+                beta = mulc(beta, divs3(mul3(f, fabsf(dot3(wi, nn))), pdf));
+                if (bounce > P.rr_start) {
+                    const float y = 0.212671f * beta.x + 0.715160f * beta.y + 0.072169f * beta.z;
+                    const float q = (y < 0.5f) ? y : 0.5f;
+                    if (ub[2] > q) cont = false;
+                    else beta = divs3(beta, q);
+                }
+                if (cont) nray = Ray{p.x, p.y, p.z, eps, wi.x, wi.y, wi.z, FLT_MAX};
+            }
+        }
+        ps.beta[0] = beta.x; ps.beta[1] = beta.y; ps.beta[2] = beta.z;
+        ps.L[0] = L.x; ps.L[1] = L.y; ps.L[2] = L.z;
+        want_next = cont;
+        finish = !cont;
+    }
+    // paths that end with no shadow ray pending are final now
+    if (active) {
+        bool keep = true;
+        if (finish && !want_shadow) {
+            *sample_ref(P, g0, ps, sample_l).out = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
+            ps.flags |= 4u;
+            keep = !queued;  // queue form: a finished path is in no queue, its state is not read again
+        } else if (finish) {
+            ps.flags |= 2u;
+        }
+        if (keep) st[pid] = ps;
+    }
+    o.contrib = contrib;
+    o.pid = pid;
+    o.shadow = want_shadow;
+    o.next = want_next;
+}
+
+// The path vertex kernel.  PSHADE_QUEUE: persistent blocks take tiles of
+// RB * PATH_PER paths (PATH_PER consecutive paths a thread) in ticket order
+// (the look-back's progress guarantee), and write the tile's shadow and
+// next rays densely in path order at offsets found by one look-back per
+// queue; a ticket is a device-scope atomic on one address, so fewer, larger
+// tiles.  PSHADE_PLAIN: one path per thread, every path's slot written.
+constexpr uint32_t PATH_PER = 2;
 __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, uint32_t bounce, uint32_t cap,
                                                    const uint32_t* q_count, const uint32_t* pid_q,
                                                    const Ray* rays_in, const Hit* hits, PathState* st,
@@ -485,168 +615,79 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
                                                    unsigned long long* tiles_s, unsigned long long* tiles_n,
                                                    uint32_t* ticket, int mode) {
     const uint32_t n = q_count ? *q_count : cap;
-    const bool queued = mode == PSHADE_QUEUE;
-    // fresh (rays_in null): the first vertex of new paths whose camera rays
-    // the traversal made itself -- path i, initial state, the camera ray made
-    // again here
     const bool fresh = rays_in == nullptr;
-    __shared__ uint32_t s_bid, s_es, s_en;
-    // PSHADE_QUEUE: persistent blocks take the tiles of RB paths that the
-    // queue's count covers, in ticket order (the look-back's progress
-    // guarantee); PSHADE_PLAIN: one tile per block.
-    const uint32_t ntiles = (n + RB - 1) / RB;
-    for (uint32_t round = 0;; ++round) {
-        uint32_t bid;
-        if (mode == PSHADE_QUEUE) {
-            if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
-            __syncthreads();
-            bid = s_bid;
-            __syncthreads();  // every thread has read s_bid before the next round
-            if (bid >= ntiles) break;
-        } else {
-            if (round > 0) break;
-            bid = blockIdx.x;
-        }
-        const uint32_t base = bid * RB;
-        const uint32_t i = base + threadIdx.x;
-        const bool live = i < n;
-        uint32_t pid = live ? ((queued && !fresh) ? pid_q[i] : i) : 0u;
-        PathState ps{};
-        bool active = false;
-        if (live) {
-            if (fresh) ps = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
-            else ps = st[pid];
-            active = queued || !(ps.flags & 4u);  // plain form: bit 4 marks a finished path
-        }
-        Hit h = active ? hits[i] : Hit{NO_HIT, 0, 0, 0};
-        bool want_shadow = false, want_next = false, finish = false;
-        Ray sray, nray;
-        f3 contrib = mk3(0, 0, 0);
-        if (active && h.prim == NO_HIT) {
-            finish = true;  // a miss ends the path (no infinite lights)
-        } else if (active) {
-            Ray r;
-            if (fresh) camera_ray_of(P, g0 + i, &r);
-            else r = rays_in[i];
-            const f3 d = mk3(r.dx, r.dy, r.dz);
-            const f3 p = add3(mk3(r.ox, r.oy, r.oz), mul3(d, h.t));
-            const FaceShade fs = P.faces[h.prim];
-            const f3 nn = ld3(fs.nn);
-            const f3 wo = neg3(d);
-            const float eps = h.t * 5e-4f;
-            f3 beta = ld3(ps.beta), L = ld3(ps.L);
-            const bool specular = (ps.flags & 1u) != 0u;
-            if (bounce == 0 || specular) {
-                f3 le = mk3(0, 0, 0);
-                if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
-                L = add3(L, mulc(beta, le));
-            }
-            const MatRec mat = P.mats[fs.mat];
-            const SampleRef sr = sample_ref(P, g0, ps, sample_l);
-            const uint32_t pixel = (uint32_t)(sr.g / sr.spp), s = (uint32_t)(sr.g % sr.spp);
-            float ua[4], ub[4];
-            rng4(sr.seed, pixel, s, 1 + 2 * bounce, ua);
-            rng4(sr.seed, pixel, s, 2 + 2 * bounce, ub);
-            if (P.nlights > 0 && mat.kind == 0u) {
-                uint32_t ln = (uint32_t)(ua[0] * (float)P.nlights);
-                if (ln > P.nlights - 1) ln = P.nlights - 1;
-                f3 c;
-                if (sample_light(P, ln, p, eps, nn, wo, mat, ua[1], ua[2], ua[3], &sray, &c)) {
-                    want_shadow = true;
-                    contrib = mulc(beta, mul3(c, (float)P.nlights));
-                }
-            }
-            bool cont = bounce != P.max_depth;
-            if (cont) {
-                // BSDF frame (bsdf/mod.rs:69-84) and world_to_local (:103-105)
-                const f3 tn = ld3(fs.tn);
-                const f3 sn = cross3(nn, tn);
-                const f3 wol = mk3(dot3(wo, sn), dot3(wo, tn), dot3(wo, nn));
-                f3 wil, f;
-                float pdf;
-                bool spec;
-                if (mat.kind == 0u) {
-                    float dx, dy;
-                    concentric_disk(ub[0], ub[1], &dx, &dy);
-                    float z = sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
-                    wil = mk3(dx, dy, z);
-                    if (wol.z < 0.0f) wil.z = -wil.z;
-                    pdf = (wol.z * wil.z > 0.0f) ? fabsf(wil.z) * (1.0f / PI_F) : 0.0f;
-                    f = mul3(mk3(mat.r, mat.g, mat.b), 1.0f / PI_F);
-                    spec = false;
-                } else {
-                    // SpecularReflection::sample_f (specular.rs:34-40), Fresnel::noop
-                    wil = mk3(-wol.x, -wol.y, wol.z);
-                    pdf = 1.0f;
-                    f = divs3(mk3(mat.r, mat.g, mat.b), fabsf(wil.z));
-                    spec = true;
-                }
-                if (black3(f) || pdf == 0.0f) {
-                    cont = false;
-                } else {
-                    // local_to_world: bsdf/mod.rs:107-111
-                    const f3 wi = mk3(sn.x * wil.x + tn.x * wil.y + nn.x * wil.z,
-                                      sn.y * wil.x + tn.y * wil.y + nn.y * wil.z,
-                                      sn.z * wil.x + tn.z * wil.y + nn.z * wil.z);
-                    ps.flags = spec ? (ps.flags | 1u) : (ps.flags & ~1u);
-                    beta = mulc(beta, divs3(mul3(f, fabsf(dot3(wi, nn))), pdf));
-                    if (bounce > P.rr_start) {
-                        const float y = 0.212671f * beta.x + 0.715160f * beta.y + 0.072169f * beta.z;
-                        const float q = (y < 0.5f) ? y : 0.5f;
-                        if (ub[2] > q) cont = false;
-                        else beta = divs3(beta, q);
-                    }
-                    if (cont) nray = Ray{p.x, p.y, p.z, eps, wi.x, wi.y, wi.z, FLT_MAX};
-                }
-            }
-            ps.beta[0] = beta.x; ps.beta[1] = beta.y; ps.beta[2] = beta.z;
-            ps.L[0] = L.x; ps.L[1] = L.y; ps.L[2] = L.z;
-            want_next = cont;
-            finish = !cont;
-        }
-        // paths that end with no shadow ray pending are final now
-        if (live && active) {
-            bool keep = true;
-            if (finish && !want_shadow) {
-                *sample_ref(P, g0, ps, sample_l).out = make_float4(ps.L[0], ps.L[1], ps.L[2], 0.0f);
-                ps.flags |= 4u;
-                keep = !queued;  // queue form: a finished path is in no queue, its state is not read again
-            } else if (finish) {
-                ps.flags |= 2u;
-            }
-            if (keep) st[pid] = ps;
+    if (mode == PSHADE_PLAIN) {
+        const uint32_t i = blockIdx.x * RB + threadIdx.x;
+        VertexOut o;
+        o.shadow = o.next = false;
+        o.contrib = mk3(0, 0, 0);
+        if (i < n) path_vertex(P, g0, bounce, i, false, fresh, pid_q, rays_in, hits, st, sample_l, o);
+        if (i < n) {
+            if (!o.shadow) set_dead(&o.sray);
+            if (!o.next) set_dead(&o.nray);
+            sh[i] = o.sray;
+            shc[i] = make_float4(o.contrib.x, o.contrib.y, o.contrib.z, o.shadow ? 1.0f : 0.0f);
+            rays_out[i] = o.nray;
         }
         uint32_t ts, tn;
-        const uint32_t rs = block_exclusive_scan(want_shadow ? 1u : 0u, &ts);
-        const uint32_t rn = block_exclusive_scan(want_next ? 1u : 0u, &tn);
-        if (mode == PSHADE_QUEUE) {
-            // the two look-backs side by side: wave 0 the shadow queue, wave 1
-            // the next-ray queue
-            if (threadIdx.x < 128) {
-                const bool shq = threadIdx.x < 64;
-                const uint32_t tot = shq ? ts : tn;
-                const uint32_t e = lookback(shq ? tiles_s : tiles_n, bid, tot);
-                if (__lane_id() == 0) {
-                    if (shq) s_es = e;
-                    else s_en = e;
-                    if (bid == ntiles - 1) *(shq ? sh_count : out_count) = e + tot;  // the last tile
-                }
+        block_exclusive_scan(o.shadow ? 1u : 0u, &ts);
+        block_exclusive_scan(o.next ? 1u : 0u, &tn);
+        if (threadIdx.x == 0) {
+            if (ts) atomicAdd(sh_count, ts);
+            if (tn) atomicAdd(out_count, tn);
+        }
+        return;
+    }
+    __shared__ uint32_t s_bid, s_es, s_en;
+    constexpr uint32_t TILE = RB * PATH_PER;
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
+    while (true) {
+        if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t bid = s_bid;
+        __syncthreads();  // every thread has read s_bid before the next tile
+        if (bid >= ntiles) break;
+        const uint32_t first = bid * TILE + threadIdx.x * PATH_PER;
+        VertexOut o[PATH_PER];
+        uint32_t cs = 0, cn = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PATH_PER; ++k) {
+            o[k].shadow = o[k].next = false;
+            if (first + k < n) path_vertex(P, g0, bounce, first + k, true, fresh, pid_q, rays_in, hits, st, sample_l, o[k]);
+            cs += o[k].shadow ? 1u : 0u;
+            cn += o[k].next ? 1u : 0u;
+        }
+        // one scan for both queues: shadow count in the low half, next in
+        // the high half (a tile holds at most 1024 of each)
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(cs | (cn << 16), &tot);
+        const uint32_t ts = tot & 0xFFFFu, tn = tot >> 16;
+        // the two look-backs side by side: wave 0 the shadow queue, wave 1
+        // the next-ray queue
+        if (threadIdx.x < 128) {
+            const bool shq = threadIdx.x < 64;
+            const uint32_t t = shq ? ts : tn;
+            const uint32_t e = lookback(shq ? tiles_s : tiles_n, bid, t);
+            if (__lane_id() == 0) {
+                if (shq) s_es = e;
+                else s_en = e;
+                if (bid == ntiles - 1) *(shq ? sh_count : out_count) = e + t;  // the last tile
             }
-            __syncthreads();
-            const uint32_t qs = s_es + rs, qn = s_en + rn;
-            if (want_shadow) { sh[qs] = sray; sh_pid[qs] = pid; shc[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f); }
-            if (want_next) { rays_out[qn] = nray; pid_out[qn] = pid; }
-        } else {
-            if (live) {
-                if (!want_shadow) set_dead(&sray);
-                if (!want_next) set_dead(&nray);
-                sh[i] = sray;
-                shc[i] = make_float4(contrib.x, contrib.y, contrib.z, want_shadow ? 1.0f : 0.0f);
-                rays_out[i] = nray;
+        }
+        __syncthreads();
+        uint32_t qs = s_es + (ex & 0xFFFFu), qn = s_en + (ex >> 16);
+#pragma unroll
+        for (uint32_t k = 0; k < PATH_PER; ++k) {
+            if (o[k].shadow) {
+                sh[qs] = o[k].sray;
+                sh_pid[qs] = o[k].pid;
+                shc[qs] = make_float4(o[k].contrib.x, o[k].contrib.y, o[k].contrib.z, 0.0f);
+                ++qs;
             }
-            if (threadIdx.x == 0) {
-                if (ts) atomicAdd(sh_count, ts);
-                if (tn) atomicAdd(out_count, tn);
+            if (o[k].next) {
+                rays_out[qn] = o[k].nray;
+                pid_out[qn] = o[k].pid;
+                ++qn;
             }
         }
     }
