@@ -94,14 +94,12 @@ def main():
     if args.tuning:
         autotune.load(args.tuning)
 
+    from pbrt_rust_amd import workloads
     t0 = time.time()
-    scene = scenes.direct_lighting_scene(level=args.level)
-    cam = scenes.pinhole(args.res, args.res)
-    nx = ny = args.spp_side
-    seed = 1
-    integ = px.DirectLightingIntegrator(4)
-    n_samples = args.res * args.res * nx * ny
-    r = px.SamplerRenderer(scene, cam, px.StratifiedSampler(nx, ny, seed), integ, chunk=n_samples)
+    scene, cam, sampler, integ, n_samples = workloads.make("direct", spp_side=args.spp_side, res=args.res,
+                                                           level=args.level)
+    nx, ny, seed = sampler.xs, sampler.ys, sampler.seed
+    r = px.SamplerRenderer(scene, cam, sampler, integ, chunk=n_samples)
     log(args, "scene + BVH ready in %.1fs" % (time.time() - t0))
 
     def run(path, steps, warmup):
@@ -238,10 +236,9 @@ def other_workloads(args, px, scenes, log_fn):
     at (2,2,-2)) and the path-tracing workload (64 level-6 icospheres, 5.2M
     triangles, 70% Lambertian / 30% mirror, quad light, 1920x1080, 64 spp,
     depth 5, roulette after bounce 3), batched and plain."""
+    from pbrt_rust_amd import workloads
     out = {}
-    sc = scenes.ray_query_scene(16384)
-    cam = scenes.pinhole(512, 512)
-    r = px.SamplerRenderer(sc, cam, px.StratifiedSampler(1, 1, 0xC0), px.DirectLightingIntegrator(1), split="sah")
+    r, sc = workloads.renderer("ray_queries")
     mse, rays, st = frame_rate(px, r, "batched", 20, 3)
     msp, _, _ = frame_rate(px, r, "plain", 20, 3)
     # the small frame is launch-bound: the batched frames replayed from a
@@ -258,10 +255,7 @@ def other_workloads(args, px, scenes, log_fn):
                           "plain_mrays_per_s": round(rays / msp / 1e3, 1), "batched_over_plain": round(msp / msb, 3)}
     log_fn("ray queries %s" % json.dumps(out["ray_queries"]))
     side = args.pt_spp_side
-    sc = scenes.spheres_scene(n_spheres=64, level=6, seed=2)
-    cam = scenes.pinhole(1920, 1080, fov=60.0, cam_pos=(0.0, 0.0, -25.0))
-    n = 1920 * 1080 * side * side
-    r = px.SamplerRenderer(sc, cam, px.StratifiedSampler(side, side, 2), px.PathIntegrator(5, 3), chunk=n)
+    r, sc = workloads.renderer("path_tracing", spp_side=side)
     msb, rays, st = frame_rate(px, r, "batched", 2, 1)
     msp, _, _ = frame_rate(px, r, "plain", 1, 1)
     r.close()
