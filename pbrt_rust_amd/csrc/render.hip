@@ -808,44 +808,95 @@ __device__ __forceinline__ bool covers(float ixf, int32_t x) {
     int32_t a = (int32_t)ceilf(dimg - 0.5f), b = (int32_t)floorf(dimg + 0.5f);
     return a <= x && x <= b;
 }
+// A sample at image position ix of pixel column qx (qx <= ix <= qx + 1)
+// covers qx, and also qx - 1 when ix == qx and qx + 1 when ix == qx + 1;
+// likewise in y.  Each pixel's edge bits say which neighbours its samples
+// reach: 1 left (ix == qx), 2 right (ix == qx + 1), 4 up (iy == qy), 8 down
+// (iy == qy + 1).  A 16 x 16 tile of pixels finds the bits of itself and its
+// one-pixel ring first, then each pixel gathers its own samples and those
+// of the neighbours whose bits reach it -- the same additions, in the same
+// order, as a gather over all nine neighbourhoods, without evaluating the
+// sample positions of neighbours that cannot contribute.
+constexpr int FT = 16;  // film tile side (FT * FT = RB threads)
+static_assert(FT * FT == RB, "one film tile per render block");
 __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sample_l, float* rgb) {
-    const uint32_t npix = P.xres * P.yres;
-    for (uint32_t pix = blockIdx.x * RB + threadIdx.x; pix < npix; pix += gridDim.x * RB) {
-        const int32_t x = (int32_t)(pix % P.xres), y = (int32_t)(pix / P.xres);
-        float xyz[3] = {0.0f, 0.0f, 0.0f};
-        float wsum = 0.0f;
-        for (int32_t qy = y - 1; qy <= y + 1; ++qy) {
-            if (qy < 0 || qy >= (int32_t)P.yres) continue;
-            for (int32_t qx = x - 1; qx <= x + 1; ++qx) {
-                if (qx < 0 || qx >= (int32_t)P.xres) continue;
-                const uint32_t qpix = (uint32_t)qy * P.xres + (uint32_t)qx;
-                for (uint32_t s = 0; s < P.spp; ++s) {
-                    float ix, iy;
-                    sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
-                    if (!covers(ix, x) || !covers(iy, y)) continue;
-                    const float4 v = sample_l[(size_t)qpix * P.spp + s];
-                    // rgb_to_xyz: spectrum.rs:37-41
-                    const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
-                    const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
-                    const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
-                    const float wt = 1.0f;
-                    xyz[0] = xyz[0] + wt * c0;
-                    xyz[1] = xyz[1] + wt * c1;
-                    xyz[2] = xyz[2] + wt * c2;
-                    wsum = wsum + wt;
-                }
+    __shared__ uint8_t s_edge[FT + 2][FT + 2];
+    const int32_t tx0 = (int32_t)blockIdx.x * FT, ty0 = (int32_t)blockIdx.y * FT;
+    for (uint32_t h = threadIdx.x; h < (FT + 2) * (FT + 2); h += RB) {
+        const int32_t qx = tx0 - 1 + (int32_t)(h % (FT + 2)), qy = ty0 - 1 + (int32_t)(h / (FT + 2));
+        uint8_t e = 0;
+        if (qx >= 0 && qx < (int32_t)P.xres && qy >= 0 && qy < (int32_t)P.yres) {
+            for (uint32_t s = 0; s < P.spp; ++s) {
+                float ix, iy;
+                sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                if (ix == (float)qx) e |= 1u;
+                if (ix == (float)(qx + 1)) e |= 2u;
+                if (iy == (float)qy) e |= 4u;
+                if (iy == (float)(qy + 1)) e |= 8u;
             }
         }
-        // xyz_to_rgb: spectrum.rs:31-35
-        float o0 = 3.240479f * xyz[0] - 1.37150f * xyz[1] - 0.498535f * xyz[2];
-        float o1 = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
-        float o2 = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
-        if (wsum != 0.0f) {
-            const float inv = 1.0f / wsum;
-            o0 = fmaxf(o0 * inv, 0.0f); o1 = fmaxf(o1 * inv, 0.0f); o2 = fmaxf(o2 * inv, 0.0f);
-        }
-        rgb[3 * (size_t)pix] = o0; rgb[3 * (size_t)pix + 1] = o1; rgb[3 * (size_t)pix + 2] = o2;
+        s_edge[h / (FT + 2)][h % (FT + 2)] = e;
     }
+    __syncthreads();
+    const int32_t x = tx0 + (int32_t)(threadIdx.x % FT), y = ty0 + (int32_t)(threadIdx.x / FT);
+    if (x >= (int32_t)P.xres || y >= (int32_t)P.yres) return;
+    float xyz[3] = {0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f;
+    for (int32_t dy = -1; dy <= 1; ++dy) {
+        const int32_t qy = y + dy;
+        if (qy < 0 || qy >= (int32_t)P.yres) continue;
+        for (int32_t dx = -1; dx <= 1; ++dx) {
+            const int32_t qx = x + dx;
+            if (qx < 0 || qx >= (int32_t)P.xres) continue;
+            const uint32_t e = s_edge[qy - ty0 + 1][qx - tx0 + 1];
+            // can a sample of (qx, qy) reach (x, y)?
+            const bool reach_x = dx == 0 || (dx < 0 ? (e & 2u) : (e & 1u));
+            const bool reach_y = dy == 0 || (dy < 0 ? (e & 8u) : (e & 4u));
+            if (!reach_x || !reach_y) continue;
+            const uint32_t qpix = (uint32_t)qy * P.xres + (uint32_t)qx;
+            const float4* sl = sample_l + (size_t)qpix * P.spp;
+            // rgb_to_xyz (spectrum.rs:37-41), weight 1
+            auto add = [&](const float4 v) {
+                const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
+                const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
+                const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
+                const float wt = 1.0f;
+                xyz[0] = xyz[0] + wt * c0;
+                xyz[1] = xyz[1] + wt * c1;
+                xyz[2] = xyz[2] + wt * c2;
+                wsum = wsum + wt;
+            };
+            if (dx == 0 && dy == 0) {
+                // every own sample covers the pixel: read them eight at a time
+                // (one 128-byte line a thread), add them in order
+                uint32_t s = 0;
+                for (; s + 8 <= P.spp; s += 8) {
+                    float4 v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = sl[s + k];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) add(v[k]);
+                }
+                for (; s < P.spp; ++s) add(sl[s]);
+                continue;
+            }
+            for (uint32_t s = 0; s < P.spp; ++s) {
+                float ix, iy;
+                sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                if (covers(ix, x) && covers(iy, y)) add(sl[s]);
+            }
+        }
+    }
+    // xyz_to_rgb: spectrum.rs:31-35
+    float o0 = 3.240479f * xyz[0] - 1.37150f * xyz[1] - 0.498535f * xyz[2];
+    float o1 = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
+    float o2 = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
+    if (wsum != 0.0f) {
+        const float inv = 1.0f / wsum;
+        o0 = fmaxf(o0 * inv, 0.0f); o1 = fmaxf(o1 * inv, 0.0f); o2 = fmaxf(o2 * inv, 0.0f);
+    }
+    const size_t pix = (size_t)y * P.xres + (size_t)x;
+    rgb[3 * pix] = o0; rgb[3 * pix + 1] = o1; rgb[3 * pix + 2] = o2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1098,7 +1149,8 @@ size_t render_spill_bytes(const SceneView& sv, int path) {
 }
 
 hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st) {
-    hipLaunchKernelGGL(k_film, dim3(grid_for((size_t)P.xres * P.yres)), dim3(RB), 0, st, P, sample_l, rgb);
+    const dim3 grid((P.xres + FT - 1) / FT, (P.yres + FT - 1) / FT);
+    hipLaunchKernelGGL(k_film, grid, dim3(RB), 0, st, P, sample_l, rgb);
     return hipGetLastError();
 }
 
