@@ -48,9 +48,9 @@ int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out) {
     std::lock_guard<std::mutex> lk(g_scr_mu);
     TraceScratch& s = g_scratch[st];
     if (!s.counter) {
-        hipError_t e = hipMalloc((void**)&s.counter, 64);
+        hipError_t e = hipMalloc((void**)&s.counter, TRACE_COUNTER_WORDS * sizeof(uint32_t));
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(counter)");
-        e = fill_u32(s.counter, 0u, 16, st);
+        e = fill_u32(s.counter, 0u, TRACE_COUNTER_WORDS, st);
         if (e != hipSuccess) return hip_fail(e, "fill");
     }
     if (spill > s.spill_bytes) {

@@ -325,12 +325,20 @@ __device__ __forceinline__ bool prim_test64(const SceneView& sv, uint32_t pos, c
 // Camera-ray packets dealt to one XCD at a time (see k_closest_packet).
 constexpr uint32_t CAM_RUN = 16;
 
-// Rays a wave of the persistent per-ray kernel claims beyond a refill's
-// need (see k_trace_wide).  On the 1024x1024 direct-lighting frame's 26.5M
-// shadow rays the occlusion kernel took 6.7 ms claiming exactly what idle
-// lanes needed, 5.37 ms with 64 more, 5.23 ms with 192, 5.27 ms with 384 and
-// 5.47 ms with 768.
-constexpr uint32_t WIDE_RESERVE = 192;
+// Work claims of the persistent per-ray kernel (see k_trace_wide): rays per
+// claim and counters (parts of the ray range), WIDE_PART_STRIDE words apart.
+// On the 1024x1024 direct-lighting frame's 26.5M shadow rays the occlusion
+// kernel took 6.7 ms with one counter and claims of exactly what idle lanes
+// needed, 5.2 ms with claims of 256, and 5.5 ms with claims of 256 from
+// eight per-XCD parts (faster on the path tracer's shadow rays, 3.4 -> 3.2
+// ms, slower on its extension rays): one part.
+constexpr uint32_t WIDE_CHUNK = 256;
+constexpr uint32_t WIDE_PARTS = 1;
+constexpr uint32_t WIDE_PART_STRIDE = 32;
+// where the part counters sit in a stream's scratch counter block (words;
+// word 0 is the packet kernels' counter, word 1 the stack-overflow flag)
+constexpr uint32_t WIDE_PARTS_AT = 32;
+static_assert(WIDE_PARTS_AT + WIDE_PARTS * WIDE_PART_STRIDE <= TRACE_COUNTER_WORDS, "scratch counter block");
 
 template <bool ANY, bool SEG, bool CQ>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
@@ -349,11 +357,17 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + tid;
     const uint32_t total = gridDim.x * blockDim.x;
-    // With plenty of rays per lane a wave claims WIDE_RESERVE rays beyond
-    // what its idle lanes need and hands those out at later refills: fewer
-    // claims on the shared counter.
-    const uint32_t reserve = (uint64_t)n >= 8ull * total ? WIDE_RESERVE : 0u;
+    // Work is claimed in chunks (WIDE_CHUNK rays with plenty of rays per
+    // lane, else 64) from WIDE_PARTS counters, one per contiguous part of
+    // the rays: a wave starts on its XCD's part (blocks are dealt to the 8
+    // XCDs in turn), so neighbouring rays share an L2 and the claims spread
+    // over eight addresses, and moves on to the next part when its part is
+    // used up.  A part's counter only grows: a part found used up stays so.
+    const uint32_t nparts = (gridDim.x % WIDE_PARTS == 0u) ? WIDE_PARTS : 1u;
+    const uint32_t chunk = (uint64_t)n >= 8ull * total ? WIDE_CHUNK : 64u;
+    uint32_t cur_part = blockIdx.x % nparts, tries = 0;
     uint32_t res_b = 0, res_e = 0;  // the wave's claimed ids not handed out yet
+    auto part_start = [&](uint32_t q) { return (uint32_t)((uint64_t)n * q / nparts); };
     const uint32_t lane = __lane_id();
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
@@ -387,25 +401,40 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* ray
         // ---- refill idle lanes with the next rays (one atomic per wave)
         const unsigned long long need = __ballot(!has && !drained);
         if (need) {
-            // the wave's reserve first (ascending ids), then a fresh claim;
-            // every lane in need gets an id, so each refill ends in work or
-            // a drained lane
-            const uint32_t k = (uint32_t)__popcll(need), avail = res_e - res_b;
-            uint32_t b = 0;
-            if (avail < k) {
-                if (lane == 0) b = atomicAdd(counter, (k - avail) + reserve);
-                b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
-            }
-            const uint32_t pre = (uint32_t)__popcll(need & lt_mask);
-            const uint32_t id = pre < avail ? res_b + pre : b + (pre - avail);
-            if (avail < k) {
-                res_b = b + (k - avail);
-                res_e = res_b + reserve;
-            } else {
-                res_b += k;
+            // ids to the lanes in need until each has one or every part is
+            // used up: the wave's chunk first, then fresh chunks
+            unsigned long long pend = need;
+            uint32_t id = 0;
+            bool got = false;
+            while (pend) {
+                const uint32_t avail = res_e - res_b;
+                if (avail == 0) {
+                    if (tries >= nparts) break;
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(counter + WIDE_PART_STRIDE * cur_part, chunk);
+                    b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+                    const uint32_t p0 = part_start(cur_part), plen = part_start(cur_part + 1) - p0;
+                    if (b >= plen) {
+                        cur_part = cur_part + 1 == nparts ? 0u : cur_part + 1;
+                        ++tries;
+                        continue;
+                    }
+                    res_b = p0 + b;
+                    res_e = p0 + (plen - b < chunk ? plen : b + chunk);
+                    continue;
+                }
+                const uint32_t pre = (uint32_t)__popcll(pend & lt_mask);
+                const bool mine = ((pend >> lane) & 1ull) && pre < avail;
+                if (mine) {
+                    id = res_b + pre;
+                    got = true;
+                }
+                const unsigned long long taken = __ballot(mine);
+                res_b += (uint32_t)__popcll(taken);
+                pend &= ~taken;
             }
             if (!has && !drained) {
-                if (id < n) {
+                if (got) {
                     if (SEG) {
                         rseg = find_seg(segs, nseg, id);
                         rid = id - segs[rseg].start;
@@ -1050,15 +1079,16 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     const int grid = cq ? wide_grid(k_trace_wide<ANY, false, true>, tun) : wide_grid(k_trace_wide<ANY, false, false>, tun);
     const uint32_t lv = wide_levels(s);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
-    hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+    uint32_t* parts = scr->counter + WIDE_PARTS_AT;
+    hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
     if (cq)
         hipLaunchKernelGGL((k_trace_wide<ANY, false, true>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
-                           occ, scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
+                           occ, parts, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     else
         hipLaunchKernelGGL((k_trace_wide<ANY, false, false>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
-                           occ, scr->counter, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
+                           occ, parts, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     return hipGetLastError();
 }
 
@@ -1095,13 +1125,14 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     if (cq) grid = any ? wide_grid(k_trace_wide<true, true, true>, tun) : wide_grid(k_trace_wide<false, true, true>, tun);
     else grid = any ? wide_grid(k_trace_wide<true, true, false>, tun) : wide_grid(k_trace_wide<false, true, false>, tun);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
-    hipError_t e = fill_u32(scr->counter, 0u, 1, st);
+    uint32_t* parts = scr->counter + WIDE_PARTS_AT;
+    hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
     uint32_t* err = scr->counter + 1;
 #define PX_SEG_LAUNCH(A, C)                                                                                   \
     hipLaunchKernelGGL((k_trace_wide<A, true, C>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,    \
-                       (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, scr->counter,             \
+                       (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, parts,                    \
                        (uint2*)scr->spill, lv, refill, err, d_segs, nseg)
     if (any && cq) PX_SEG_LAUNCH(true, true);
     else if (any) PX_SEG_LAUNCH(true, false);

@@ -39,11 +39,13 @@ struct TraceTuning {
     int scheme = SCHEME_RAY;
 };
 
-// Scratch owned by the caller (per stream): spill stacks and a work counter.
+// Scratch owned by the caller (per stream): spill stacks and a block of
+// TRACE_COUNTER_WORDS device words for work counters and the overflow flag.
+constexpr uint32_t TRACE_COUNTER_WORDS = 512;
 struct TraceScratch {
     void* spill = nullptr;       // spill stack entries
     size_t spill_bytes = 0;
-    uint32_t* counter = nullptr; // one u32, device
+    uint32_t* counter = nullptr; // TRACE_COUNTER_WORDS words, device
 };
 
 // Launch helpers; return hipSuccess or the launch error.
