@@ -28,8 +28,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # workload, scaled by a device-copy calibration (written next to its summary)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_direct1024.json")
-KERNEL_ANY = "k_trace_wide<true, false, true>"
-KERNEL_CLOSEST = "k_closest_packet<true>"
+KERNEL_ANY = "k_trace_wide<true, false, true, 16>"
+KERNEL_CLOSEST = "k_trace_wide<false, false, true, 8>"
 
 
 def parse():

@@ -111,6 +111,15 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
    stack per ray over the exact records.  Process-wide setting. */
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme);
+/* LDS traversal-stack entries per lane (8, 16 or 24; default 16; deeper
+   entries spill to global memory) of the batched path's per-ray kernel for
+   that kind.  Process-wide setting, kept by pbrtx_set_tuning. */
+int pbrtx_set_stack_depth(int32_t path, int32_t kind, int32_t entries);
+/* Back to the built-in launch settings (the autotuner's choices on the
+   direct-lighting and path-tracing frames).  Unless set explicitly, the
+   batched path's camera rays use one stack per wave at 32 or more samples
+   per pixel and a stack per ray below. */
+int pbrtx_reset_tuning(void);
 
 /* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */
 /* Scene + camera + sampler + integrator description.  Triangles (world-space

@@ -128,6 +128,8 @@ def lib():
     L.pbrtx_intersect.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_intersect_p.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
+    L.pbrtx_set_stack_depth.argtypes = [i32, i32, i32]
+    L.pbrtx_reset_tuning.argtypes = []
     L.pbrtx_device_copy.argtypes = [vp, vp, u64, vp]
     L.pbrtx_device_malloc.argtypes = [c.POINTER(vp), u64]
     L.pbrtx_device_free.argtypes = [vp]

@@ -33,9 +33,30 @@ int hip_fail(hipError_t e, const char* what) {
 uint32_t g_tuning_gen = 1;
 // [path][kind]: closest hit, occlusion, camera rays of the renderer (coherent:
 // the batched path walks them as packets)
-TraceTuning g_tuning[2][3] = {
+// Defaults from the autotuner (python -m pbrt_rust_amd.autotune --path) on
+// the 1024x1024 direct-lighting and 1920x1080 path-tracing frames: batched
+// closest hit 256 threads, occupancy grid, refill below 60% busy lanes, 8
+// LDS stack entries; occlusion the same with 16 entries.
+static const TraceTuning k_default_tuning[2][3] = {
     {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 0, SCHEME_RAY}},
-    {TraceTuning{256, 0, 0, SCHEME_RAY}, TraceTuning{256, 0, 10, SCHEME_RAY}, TraceTuning{128, 8, 0, SCHEME_PACKET}}};
+    {TraceTuning{256, 0, 60, SCHEME_RAY, 8}, TraceTuning{256, 0, 60, SCHEME_RAY, 16},
+     TraceTuning{128, 8, 0, SCHEME_PACKET}}};
+TraceTuning g_tuning[2][3] = {
+    {k_default_tuning[0][0], k_default_tuning[0][1], k_default_tuning[0][2]},
+    {k_default_tuning[1][0], k_default_tuning[1][1], k_default_tuning[1][2]}};
+bool g_camera_auto = true;
+
+// Camera rays of a frame with spp samples per pixel.  Unless set explicitly,
+// the batched path picks by coherence: at 32 or more samples per pixel a
+// wave's 64 camera rays come from one or two pixels and one stack per wave
+// wins (1920x1080 at 64 spp: 24.3 ms a frame against 25.9 with a stack per
+// ray); below that a stack per ray wins (1024x1024 at 16 spp: 8.66 ms
+// against 9.17).
+TraceTuning camera_tuning(int path, uint32_t spp) {
+    if (path == PATH_BATCHED && g_camera_auto)
+        return spp >= 32 ? TraceTuning{128, 8, 0, SCHEME_PACKET} : TraceTuning{256, 0, 0, SCHEME_RAY, 8};
+    return g_tuning[path][2];
+}
 
 namespace {
 std::mutex g_scr_mu;
@@ -281,8 +302,29 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
     if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET && scheme != SCHEME_RAY_EXACT)
         return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
     if (scheme == SCHEME_PACKET && path != 1) return fail(PBRTX_ERR_ARG, "the packet scheme exists for the batched path only");
-    g_tuning[path][kind] = TraceTuning{block, blocks_per_cu, refill_pct, scheme};
+    TraceTuning t{block, blocks_per_cu, refill_pct, scheme};
+    t.stack = g_tuning[path][kind].stack;
+    g_tuning[path][kind] = t;
+    if (kind == 2 && path == PATH_BATCHED) g_camera_auto = false;  // an explicit choice
     ++g_tuning_gen;  // captured frames are captured again
+    return PBRTX_OK;
+}
+
+int pbrtx_reset_tuning(void) {
+    for (int p = 0; p < 2; ++p)
+        for (int k = 0; k < 3; ++k) g_tuning[p][k] = k_default_tuning[p][k];
+    g_camera_auto = true;
+    ++g_tuning_gen;
+    return PBRTX_OK;
+}
+
+int pbrtx_set_stack_depth(int32_t path, int32_t kind, int32_t entries) {
+    if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");
+    if (kind < 0 || kind > 2) return fail(PBRTX_ERR_ARG, "kind must be 0 (closest hit), 1 (occlusion) or 2 (camera)");
+    if (entries != 8 && entries != 16 && entries != 24) return fail(PBRTX_ERR_ARG, "stack depth must be 8, 16 or 24");
+    g_tuning[path][kind].stack = entries;
+    if (kind == 2 && path == PATH_BATCHED) g_camera_auto = false;
+    ++g_tuning_gen;
     return PBRTX_OK;
 }
 

@@ -24,6 +24,8 @@ int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, 
 int run_any(const SceneView& v, const Ray* rays, uint32_t n, uint32_t* occ, int path, hipStream_t st);
 extern TraceTuning g_tuning[2][3];  // [path][0 closest hit, 1 occlusion, 2 camera rays]
 extern uint32_t g_tuning_gen;       // bumped by every tuning change
+extern bool g_camera_auto;          // camera rays: scheme picked by samples per pixel
+TraceTuning camera_tuning(int path, uint32_t spp);
 
 }  // namespace px
 

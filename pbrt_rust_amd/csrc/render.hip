@@ -1144,8 +1144,10 @@ CamParams cam_params(const RenderParams& P, uint64_t g0) {
 size_t render_spill_bytes(const SceneView& sv, int path) {
     const size_t a = trace_spill_bytes(sv, path, g_tuning[path][0]);
     const size_t b = trace_spill_bytes(sv, path, g_tuning[path][1]);
-    const size_t c = trace_spill_bytes(sv, path, g_tuning[path][2]);
-    return std::max(a, std::max(b, c));
+    // camera rays: either scheme the frame's samples per pixel may pick
+    const size_t c = trace_spill_bytes(sv, path, camera_tuning(path, 1));
+    const size_t d = trace_spill_bytes(sv, path, camera_tuning(path, 1u << 20));
+    return std::max(std::max(a, b), std::max(c, d));
 }
 
 hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st) {
@@ -1181,7 +1183,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
     const bool batched = path == PATH_BATCHED;
     const TraceTuning& tunC = g_tuning[path][0];
     const TraceTuning& tunA = g_tuning[path][1];
-    const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
+    const TraceTuning tunCam = camera_tuning(path, P.spp);  // camera rays (coherent)
     const bool osort = batched && osort_req;
     ICHK(reserve_path_work(W, n, osort));
     // batched frames: the packet kernel makes the camera rays and the first
@@ -1337,7 +1339,7 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
     ICHK(R->counters.ensure(N_COUNTERS));
     const bool batched = path == PATH_BATCHED;
     const TraceTuning& tunA = g_tuning[path][1];
-    const TraceTuning& tunCam = g_tuning[path][2];  // camera rays (coherent)
+    const TraceTuning tunCam = camera_tuning(path, P.spp);  // camera rays (coherent)
     uint32_t* cnt = R->counters.p;
     RCHK(fill_u32(cnt, 0u, N_COUNTERS, st));
     const SceneView& sv = R->sc.scene.view;

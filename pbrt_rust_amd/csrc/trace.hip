@@ -28,7 +28,11 @@ namespace px {
 namespace {
 
 constexpr int STK_PLAIN = 24;   // LDS entries per lane (u32 node index)
-constexpr int STK_WIDE = 16;    // LDS entries per lane (ref + entry distance)
+// LDS traversal stack entries per lane of the persistent per-ray kernel
+// (ref + entry distance; deeper entries spill to global memory): a launch
+// parameter, one kernel instance per supported depth
+constexpr int STK_DEFAULT = 16;
+inline bool stack_supported(int e) { return e == 8 || e == 16 || e == 24; }
 constexpr int MAX_BLOCK = 256;
 
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
@@ -340,7 +344,7 @@ constexpr uint32_t WIDE_PART_STRIDE = 32;
 constexpr uint32_t WIDE_PARTS_AT = 32;
 static_assert(WIDE_PARTS_AT + WIDE_PARTS * WIDE_PART_STRIDE <= TRACE_COUNTER_WORDS, "scratch counter block");
 
-template <bool ANY, bool SEG, bool CQ>
+template <bool ANY, bool SEG, bool CQ, int STK_WIDE>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
                                                           const uint32_t* n_dev, Hit* hits,
                                                           uint32_t* occ, uint32_t* counter,
@@ -1005,10 +1009,20 @@ static uint32_t plain_levels(const SceneView& s) {
     uint32_t need = s.depth + 2;
     return need > (uint32_t)STK_PLAIN ? need - (uint32_t)STK_PLAIN : 0u;
 }
-static uint32_t wide_levels(const SceneView& s) {
+static uint32_t wide_levels(const SceneView& s, int stack) {
     // at most three pending slots per record level on the current path
     uint32_t need = 3 * s.qdepth + 1;
-    return need > (uint32_t)STK_WIDE ? need - (uint32_t)STK_WIDE : 0u;
+    return need > (uint32_t)stack ? need - (uint32_t)stack : 0u;
+}
+
+// the per-ray kernel instance for a stack depth
+template <bool ANY, bool SEG, bool CQ>
+static auto wide_kernel(int stack) -> decltype(&k_trace_wide<ANY, SEG, CQ, STK_DEFAULT>) {
+    switch (stack) {
+        case 8: return &k_trace_wide<ANY, SEG, CQ, 8>;
+        case 24: return &k_trace_wide<ANY, SEG, CQ, 24>;
+        default: return &k_trace_wide<ANY, SEG, CQ, 16>;
+    }
 }
 
 // the packet kernel's stack holds at most three pending slots per record
@@ -1030,14 +1044,13 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
     TraceTuning t = tun;
     // the largest persistent grid of any batched traversal kernel
     size_t threads = 0;
-    for (size_t g : {wide_grid(k_trace_wide<false, false, false>, t), wide_grid(k_trace_wide<true, false, false>, t),
-                     wide_grid(k_trace_wide<false, true, false>, t), wide_grid(k_trace_wide<true, true, false>, t),
-                     wide_grid(k_trace_wide<false, false, true>, t), wide_grid(k_trace_wide<true, false, true>, t),
-                     wide_grid(k_trace_wide<false, true, true>, t), wide_grid(k_trace_wide<true, true, true>, t)})
+    const int k = t.stack;
+    for (size_t g : {wide_grid(wide_kernel<false, false, false>(k), t), wide_grid(wide_kernel<true, false, false>(k), t),
+                     wide_grid(wide_kernel<false, true, false>(k), t), wide_grid(wide_kernel<true, true, false>(k), t),
+                     wide_grid(wide_kernel<false, false, true>(k), t), wide_grid(wide_kernel<true, false, true>(k), t),
+                     wide_grid(wide_kernel<false, true, true>(k), t), wide_grid(wide_kernel<true, true, true>(k), t)})
         threads = std::max(threads, g * (size_t)block);
-    size_t threads2 = threads;
-    if (threads2 > threads) threads = threads2;
-    return (size_t)wide_levels(s) * threads * sizeof(uint2);
+    return (size_t)wide_levels(s, k) * threads * sizeof(uint2);
 }
 
 template <bool ANY>
@@ -1075,20 +1088,18 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
                            st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{});
         return hipGetLastError();
     }
+    if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
     const bool cq = use_compressed(s, tun);
-    const int grid = cq ? wide_grid(k_trace_wide<ANY, false, true>, tun) : wide_grid(k_trace_wide<ANY, false, false>, tun);
-    const uint32_t lv = wide_levels(s);
+    const auto kern = cq ? wide_kernel<ANY, false, true>(tun.stack) : wide_kernel<ANY, false, false>(tun.stack);
+    const int grid = wide_grid(kern, tun);
+    const uint32_t lv = wide_levels(s, tun.stack);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
     uint32_t* parts = scr->counter + WIDE_PARTS_AT;
     hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    if (cq)
-        hipLaunchKernelGGL((k_trace_wide<ANY, false, true>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
-                           occ, parts, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
-    else
-        hipLaunchKernelGGL((k_trace_wide<ANY, false, false>), dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits,
-                           occ, parts, (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ, parts,
+                       (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     return hipGetLastError();
 }
 
@@ -1119,26 +1130,20 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     if (total == 0 || nseg == 0) return hipSuccess;
     const int block = tun.block;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    const uint32_t lv = wide_levels(s);
+    if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
+    const uint32_t lv = wide_levels(s, tun.stack);
     const bool cq = use_compressed(s, tun);
-    int grid;
-    if (cq) grid = any ? wide_grid(k_trace_wide<true, true, true>, tun) : wide_grid(k_trace_wide<false, true, true>, tun);
-    else grid = any ? wide_grid(k_trace_wide<true, true, false>, tun) : wide_grid(k_trace_wide<false, true, false>, tun);
+    const auto kern = cq ? (any ? wide_kernel<true, true, true>(tun.stack) : wide_kernel<false, true, true>(tun.stack))
+                         : (any ? wide_kernel<true, true, false>(tun.stack) : wide_kernel<false, true, false>(tun.stack));
+    const int grid = wide_grid(kern, tun);
     if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
     uint32_t* parts = scr->counter + WIDE_PARTS_AT;
     hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
     uint32_t* err = scr->counter + 1;
-#define PX_SEG_LAUNCH(A, C)                                                                                   \
-    hipLaunchKernelGGL((k_trace_wide<A, true, C>), dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total,    \
-                       (const uint32_t*)nullptr, (Hit*)nullptr, (uint32_t*)nullptr, parts,                    \
-                       (uint2*)scr->spill, lv, refill, err, d_segs, nseg)
-    if (any && cq) PX_SEG_LAUNCH(true, true);
-    else if (any) PX_SEG_LAUNCH(true, false);
-    else if (cq) PX_SEG_LAUNCH(false, true);
-    else PX_SEG_LAUNCH(false, false);
-#undef PX_SEG_LAUNCH
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total, (const uint32_t*)nullptr,
+                       (Hit*)nullptr, (uint32_t*)nullptr, parts, (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
     return hipGetLastError();
 }
 

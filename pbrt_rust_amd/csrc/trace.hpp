@@ -37,6 +37,7 @@ struct TraceTuning {
     int blocks_per_cu = 0;    // persistent grid: blocks per CU (0 = occupancy maximum)
     int refill_pct = 50;      // refill when fewer than this % of a wave's lanes trace
     int scheme = SCHEME_RAY;
+    int stack = 16;           // LDS stack entries per lane of the per-ray kernel: 8, 16 or 24
 };
 
 // Scratch owned by the caller (per stream): spill stacks and a block of

@@ -199,11 +199,41 @@ def test_paths_agree_at_full_size():
     assert np.array_equal(bvh.intersect_p(px.RayBatch(o, d), "plain"), occ)
 
 
+@pytest.mark.parametrize("stack", [8, 24])
+def test_stack_depths_match_oracle(stack):
+    # the per-ray kernel at another LDS stack depth: the 29-level chain (deep
+    # spills at 8 entries, none at 24) and incoherent rays on a random scene
+    try:
+        for kind in ("closest", "any"):
+            px.set_tuning("batched", kind, 256, 0, 10, "ray", stack=stack)
+        n = 31
+        xs = 16.0 ** np.arange(n)
+        v = np.concatenate([np.array([[x, 0, 0], [x, 1, 0], [x, 0, 1]]) for x in xs]).astype(F32)
+        idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+        bvh, orc = mesh_pair(v, idx)
+        rng = np.random.default_rng(2)
+        m = 3000
+        k = rng.integers(0, n, m)
+        tmin = np.where(k == 0, 0.0, xs[np.maximum(k - 1, 0)] * 1.5).astype(F32)
+        o = np.stack([np.full(m, -1.0), rng.uniform(0.05, 0.45, m), rng.uniform(0.05, 0.45, m)], 1).astype(F32)
+        d = np.tile(np.array([[1, 0, 0]], F32), (m, 1))
+        check_closest(bvh, orc, o, d, mint=tmin)
+        check_any(bvh, orc, o, d, mint=tmin)
+        v, idx = scenes.random_triangles(20000, 12)
+        bvh, orc = mesh_pair(v, idx)
+        o, d = random_rays(30000, 13)
+        check_closest(bvh, orc, o, d)
+        check_any(bvh, orc, o, d, maxt=0.7)
+        px.synchronize()  # raises if a traversal stack overflowed
+    finally:
+        px.reset_tuning()
+
+
 @pytest.fixture
 def packet_occlusion():
     px.set_tuning("batched", "any", 256, 0, 0, "packet")
     yield
-    px.set_tuning("batched", "any", 256, 0, 10, "ray")
+    px.reset_tuning()
 
 
 @pytest.mark.skip(reason="hung once on the GPU (no output for 180 s) while a build with early block exits and "
@@ -241,8 +271,7 @@ def exact_records():
     px.set_tuning("batched", "closest", 256, 0, 0, "ray_exact")
     px.set_tuning("batched", "any", 256, 0, 10, "ray_exact")
     yield
-    px.set_tuning("batched", "closest", 256, 0, 0, "ray")
-    px.set_tuning("batched", "any", 256, 0, 10, "ray")
+    px.reset_tuning()
 
 
 def test_exact_record_scheme_matches_oracle(exact_records):
@@ -261,7 +290,7 @@ def test_exact_record_scheme_matches_oracle(exact_records):
 def packet_closest():
     px.set_tuning("batched", "closest", 256, 0, 0, "packet")
     yield
-    px.set_tuning("batched", "closest", 256, 0, 0, "ray")
+    px.reset_tuning()
 
 
 def test_packet_closest_scheme_matches_oracle(packet_closest):

@@ -158,18 +158,24 @@ def test_film_boundary_samples():
 
 
 def test_camera_ray_schemes_give_the_same_frame():
-    # the batched path walks camera rays as packets by default; the per-ray
-    # kernel (a tuning choice) must give the same samples
+    # every camera-ray scheme (chosen by samples per pixel unless set) must
+    # give the same samples, at few and at many samples per pixel
     scene = scenes.direct_lighting_scene(level=4)
     cam = scenes.pinhole(64, 48)
     ref = oracle_render(scene, cam, DIRECT)
+    many = dict(DIRECT, nx=6, ny=6, light_samples=1)
+    cam2 = scenes.pinhole(24, 16)
+    ref2 = oracle_render(scene, cam2, many)
     try:
-        for scheme in ("ray", "ray_exact", "packet"):
-            px.set_tuning("batched", "camera", 256, 0, 0, scheme)
+        for scheme in (None, "ray", "ray_exact", "packet"):
+            if scheme:
+                px.set_tuning("batched", "camera", 256, 0, 0, scheme)
             img, samples, st = gpu_render(scene, cam, DIRECT, "batched")
             assert_same(ref, img, samples)
+            img, samples, st = gpu_render(scene, cam2, many, "batched")
+            assert_same(ref2, img, samples)
     finally:
-        px.set_tuning("batched", "camera", 128, 8, 0, "packet")
+        px.reset_tuning()
 
 
 @pytest.mark.parametrize("integ", [DIRECT, PATH], ids=["direct", "path"])
@@ -205,8 +211,8 @@ def test_graph_replays_match_eager_frames(integ):
                 other.render("batched")
                 px.synchronize()
             r.set_graph(False)
-            px.set_tuning("batched", "any", 256, 0, 10, "ray")
+            px.reset_tuning()
     finally:
-        px.set_tuning("batched", "any", 256, 0, 10, "ray")
+        px.reset_tuning()
         r.close()
         other.close()
