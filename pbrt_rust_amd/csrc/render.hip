@@ -122,8 +122,9 @@ __global__ __launch_bounds__(RB) void k_camera(RenderParams P, uint64_t g0, uint
 // Slots per sample: point lights first, then light_samples per area light.
 // Exclusive prefix sum of v over the RB threads of a block (wave shuffles,
 // then one LDS round for the four wave totals).
+template <int NB = RB>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-    __shared__ uint32_t wsum[RB / 64];
+    __shared__ uint32_t wsum[NB / 64];
     const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
     uint32_t x = v;
     for (int off = 1; off < 64; off <<= 1) {
@@ -133,7 +134,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     if (lane == 63) wsum[wid] = x;
     __syncthreads();
     uint32_t before = 0, tot = 0;
-    for (uint32_t w = 0; w < RB / 64; ++w) {
+    for (uint32_t w = 0; w < NB / 64; ++w) {
         const uint32_t s = wsum[w];
         if (w < wid) before += s;
         tot += s;
@@ -228,7 +229,14 @@ enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_QUEUE = 1 };
 static_assert(RB >= 128, "the path vertex kernel runs its two look-backs on two waves");
 constexpr uint32_t SH_CACHE = 4;
 
-__global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
+// Threads per block of the shade kernels: a block is one look-back tile, and
+// every tile costs a ticket and published words on shared lines (256 / 512 /
+// 1024 threads: direct-lighting frame 8.63 / 8.41 / 8.32 ms, path-tracing
+// frame 24.3 / 23.0 / 22.6 ms).
+constexpr int SHADE_B = 1024;
+constexpr int VERTEX_B = 1024;
+
+__global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n, const Ray* rays,
                                                      const Hit* hits, Ray* sh, float4* contrib,
                                                      uint32_t* qidx, float4* lbase, uint32_t* sh_count,
                                                      unsigned long long* tiles, uint32_t* ticket, int mode) {
@@ -242,7 +250,7 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         __syncthreads();
         bid = s_bid;
     }
-    const uint32_t i = bid * RB + threadIdx.x;
+    const uint32_t i = bid * SHADE_B + threadIdx.x;
     const bool live = i < n;
     Hit h = live ? hits[i] : Hit{NO_HIT, 0, 0, 0};
     const bool hit = live && h.prim != NO_HIT;
@@ -294,7 +302,7 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
             live_rays += valid ? 1u : 0u;
         }
         uint32_t tot;
-        block_exclusive_scan(live_rays, &tot);
+        block_exclusive_scan<SHADE_B>(live_rays, &tot);
         if (threadIdx.x == 0 && tot) atomicAdd(sh_count, tot);
         return;
     }
@@ -325,7 +333,7 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
     }
     // rank of each lane's slot k among the block's valid slots k, and the
     // block's count of valid slots k (wave ballots, then the four waves)
-    __shared__ uint32_t s_cnt[SH_CACHE][RB / 64];
+    __shared__ uint32_t s_cnt[SH_CACHE][SHADE_B / 64];
     const uint32_t wid = threadIdx.x / 64;
     const unsigned long long ltm = lt_mask();
     uint32_t before[SH_CACHE];
@@ -338,13 +346,13 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         }
     }
     uint32_t tot;
-    const uint32_t local = block_exclusive_scan(cnt, &tot);  // also orders the s_cnt writes
+    const uint32_t local = block_exclusive_scan<SHADE_B>(cnt, &tot);  // also orders the s_cnt writes
     uint32_t run = 0;  // valid slots of earlier k in this block
     if (cached) {
 #pragma unroll
         for (uint32_t k = 0; k < SH_CACHE; ++k) {
             uint32_t wb = 0, bt = 0;
-            for (uint32_t w2 = 0; w2 < RB / 64; ++w2) {
+            for (uint32_t w2 = 0; w2 < SHADE_B / 64; ++w2) {
                 const uint32_t c = s_cnt[k][w2];
                 if (w2 < wid) wb += c;
                 bt += c;
@@ -357,7 +365,7 @@ __global__ __launch_bounds__(RB) void k_shade_direct(RenderParams P, uint64_t g0
         const uint32_t e = lookback(tiles, bid, tot);
         if (threadIdx.x == 0) {
             s_excl = e;
-            if ((bid + 1) * RB >= n) *sh_count = e + tot;  // the last block
+            if ((bid + 1) * SHADE_B >= n) *sh_count = e + tot;  // the last block
         }
     }
     __syncthreads();
@@ -604,9 +612,9 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
 // (the look-back's progress guarantee), and write the tile's shadow and
 // next rays densely in path order at offsets found by one look-back per
 // queue; a ticket is a device-scope atomic on one address, so fewer, larger
-// tiles.  PSHADE_PLAIN: one path per thread, every path's slot written.
+// tiles (VERTEX_B * PATH_PER paths).  PSHADE_PLAIN: one path per thread, every path's slot written.
 constexpr uint32_t PATH_PER = 2;
-__global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, uint32_t bounce, uint32_t cap,
+__global__ __launch_bounds__(VERTEX_B) void k_path_shade(RenderParams P, uint64_t g0, uint32_t bounce, uint32_t cap,
                                                    const uint32_t* q_count, const uint32_t* pid_q,
                                                    const Ray* rays_in, const Hit* hits, PathState* st,
                                                    Ray* rays_out, uint32_t* pid_out, uint32_t* out_count,
@@ -617,7 +625,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
     const uint32_t n = q_count ? *q_count : cap;
     const bool fresh = rays_in == nullptr;
     if (mode == PSHADE_PLAIN) {
-        const uint32_t i = blockIdx.x * RB + threadIdx.x;
+        const uint32_t i = blockIdx.x * VERTEX_B + threadIdx.x;
         VertexOut o;
         o.shadow = o.next = false;
         o.contrib = mk3(0, 0, 0);
@@ -630,8 +638,8 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
             rays_out[i] = o.nray;
         }
         uint32_t ts, tn;
-        block_exclusive_scan(o.shadow ? 1u : 0u, &ts);
-        block_exclusive_scan(o.next ? 1u : 0u, &tn);
+        block_exclusive_scan<VERTEX_B>(o.shadow ? 1u : 0u, &ts);
+        block_exclusive_scan<VERTEX_B>(o.next ? 1u : 0u, &tn);
         if (threadIdx.x == 0) {
             if (ts) atomicAdd(sh_count, ts);
             if (tn) atomicAdd(out_count, tn);
@@ -639,7 +647,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         return;
     }
     __shared__ uint32_t s_bid, s_es, s_en;
-    constexpr uint32_t TILE = RB * PATH_PER;
+    constexpr uint32_t TILE = VERTEX_B * PATH_PER;
     const uint32_t ntiles = (n + TILE - 1) / TILE;
     while (true) {
         if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
@@ -660,7 +668,7 @@ __global__ __launch_bounds__(RB) void k_path_shade(RenderParams P, uint64_t g0, 
         // one scan for both queues: shadow count in the low half, next in
         // the high half (a tile holds at most 1024 of each)
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(cs | (cn << 16), &tot);
+        const uint32_t ex = block_exclusive_scan<VERTEX_B>(cs | (cn << 16), &tot);
         const uint32_t ts = tot & 0xFFFFu, tn = tot >> 16;
         // the two look-backs side by side: wave 0 the shadow queue, wave 1
         // the next-ray queue
@@ -1222,7 +1230,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         RCHK(fill_u32(cout, 0u, 1, st));
         RCHK(fill_u32(cnt + 5, 0u, 1, st));
         {
-            const uint32_t nb = (n + RB - 1) / RB;
+            const uint32_t nb = (n + VERTEX_B - 1) / VERTEX_B;
             // look-back tiles of the two queues (two words per block each)
             // and the block ticket
             ICHK(W.bcounts.ensure(4 * (size_t)nb + 2));
@@ -1230,7 +1238,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             unsigned long long* tiles = (unsigned long long*)W.bcounts.p;
             // queue form: persistent blocks over the live paths' tiles
             const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
-            hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(RB), 0, st, P, g0, bounce, n,
+            hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(VERTEX_B), 0, st, P, g0, bounce, n,
                                batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
                                batched ? (const uint32_t*)pin : (const uint32_t*)nullptr,
                                (fused && bounce == 0) ? (const Ray*)nullptr : rin, W.hits.p, W.st.p, rout,
@@ -1377,13 +1385,13 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         a = sg.begin();
         RCHK(fill_u32(cnt + 5, 0u, 1, st));
         {
-            const uint32_t nb = (n + RB - 1) / RB;
+            const uint32_t nb = (n + SHADE_B - 1) / SHADE_B;
             if (batched) {
                 // look-back tiles (two words per block) and the block ticket
                 ICHK(W.bcounts.ensure(2 * (size_t)nb + 2));
                 RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)nb + 2, st));
             }
-            hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(RB), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
+            hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(SHADE_B), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
                                W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (unsigned long long*)W.bcounts.p,
                                W.bcounts.p + 2 * (size_t)nb, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN));
             RCHK(hipGetLastError());
