@@ -118,6 +118,48 @@ def test_edge_cases(path):
     assert bvh.intersect(px.RayBatch(np.zeros((0, 3)), np.zeros((0, 3))), path).shape == (0,)
 
 
+def cube_grid(n=8, step=0.5, size=0.25):
+    """Axis-aligned cubes on a dyadic grid: node boxes whose faces lie on
+    the planes that the rays below start on and graze."""
+    cv = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)], F32) * F32(size)
+    cf = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                   [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.int32)
+    vs, fs = [], []
+    rng = np.random.default_rng(3)
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                if rng.random() < 0.4:
+                    fs.append(cf + 8 * len(vs))
+                    vs.append(cv + np.array([i, j, k], F32) * F32(step) - F32(n * step / 2))
+    return np.concatenate(vs), np.concatenate(fs)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_grazing_rays_on_box_planes(path):
+    # Origins on the cubes' face planes, directions with components from
+    # 2^-46 to 2^-30 (1/d on both sides of the 2^40 limit of the ray-space
+    # record test) and origins beyond 2^40: near-parallel rays that skim
+    # node faces, mixed in one wave with ordinary rays.
+    v, idx = cube_grid()
+    bvh, orc = mesh_pair(v, idx)
+    rng = np.random.default_rng(41)
+    n = 6000
+    o = (np.round(rng.uniform(-2.2, 2.2, (n, 3)) * 4) / 4).astype(F32)
+    d = rng.normal(size=(n, 3)).astype(F32)
+    k = rng.integers(0, 3, n)
+    sel = rng.random(n) < 0.6
+    tiny = (rng.choice([-1.0, 1.0], n) * 2.0 ** rng.uniform(-46, -30, n)).astype(F32)
+    d[sel, k[sel]] = tiny[sel]
+    sel2 = rng.random(n) < 0.2
+    d[sel2, (k[sel2] + 1) % 3] = tiny[::-1][sel2]
+    far = rng.random(n) < 0.05
+    o[far] = (o[far] * F32(2.0 ** 42)).astype(F32)
+    check_closest(bvh, orc, o, d, path=path)
+    check_any(bvh, orc, o, d, path=path)
+    check_any(bvh, orc, o, d, mint=1e-3, maxt=0.75, path=path)
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_mixed_finite_and_infinite_inverse_directions(path):
     # Rays with finite and with infinite 1/d components (NaN slab bounds)
