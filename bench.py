@@ -28,6 +28,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # workload, scaled by a device-copy calibration (written next to its summary)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_direct1024.json")
+PROFILE_VALU = os.path.join(ROOT, "profiles", "valu_direct1024.json")
 KERNEL_ANY = "k_trace_wide<true, false, true, 16>"
 KERNEL_CLOSEST = "k_trace_wide<false, false, true, 8>"
 
@@ -165,6 +166,15 @@ def main():
             traffic = json.load(open(PROFILE_TRAFFIC))["kernels"][kname]["hbm_bytes"]
         except (KeyError, ValueError):
             traffic = None
+    # share of the SIMDs' VALU issue slots the kernel used (SQ_INSTS_VALU
+    # against the dispatch's clock, one wave64 instruction per SIMD every 4
+    # cycles), from the committed counter pass
+    valu_issue = None
+    if os.path.exists(PROFILE_VALU):
+        try:
+            valu_issue = round(json.load(open(PROFILE_VALU))["kernels"][kname]["valu_issue_frac"], 3)
+        except (KeyError, ValueError):
+            valu_issue = None
 
     out = {
         "metric": "Mrays/s batched vs plain GPU kernel; p50/p99 request latency at fixed rate",
@@ -206,7 +216,8 @@ def main():
             "rays_per_launch": int(n_rays),
             "measured_hbm_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
             "valu": {"achieved": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+                     "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                     "issue_frac": valu_issue},
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "serving": serve,

@@ -117,8 +117,7 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
 int pbrtx_set_stack_depth(int32_t path, int32_t kind, int32_t entries);
 /* Back to the built-in launch settings (the autotuner's choices on the
    direct-lighting and path-tracing frames).  Unless set explicitly, the
-   batched path's camera rays use one stack per wave at 32 or more samples
-   per pixel and a stack per ray below. */
+   batched path's camera rays use one stack per wave of 64 rays. */
 int pbrtx_reset_tuning(void);
 
 /* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */

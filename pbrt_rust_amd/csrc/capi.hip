@@ -47,16 +47,16 @@ TraceTuning g_tuning[2][3] = {
 bool g_camera_auto = true;
 
 // Camera rays of a frame with spp samples per pixel.  Unless set explicitly,
-// the batched path picks by coherence: at 32 or more samples per pixel a
-// wave's 64 camera rays come from one or two pixels and one stack per wave
-// made in the traversal wins (1920x1080 at 64 spp: 24.3 ms a frame against
-// 25.9 with a stack per ray, which also needs the rays written first; the
-// packet shape, 256 threads and 16 blocks per CU, from the autotuner on that
-// frame); below that a stack per ray wins (1024x1024 at 16 spp: 8.66 ms
-// against 9.17).
+// the batched path traces them with one stack per wave made in the
+// traversal (a wave's 64 camera rays are neighbouring samples of one row of
+// pixels), 256 threads and 16 blocks per CU: 1920x1080 at 64 spp, 19.4 ms a
+// frame; 1024x1024 at 16 spp, 7.09 ms against 7.36 with a stack per ray;
+// 512x512 at 1 spp, 0.464 ms against 0.469.  (With the packet kernel at
+// five waves per SIMD, before its register budget was cut to eight, a
+// stack per ray had won below 32 samples per pixel.)
 TraceTuning camera_tuning(int path, uint32_t spp) {
-    if (path == PATH_BATCHED && g_camera_auto)
-        return spp >= 32 ? TraceTuning{256, 16, 0, SCHEME_PACKET} : TraceTuning{256, 0, 0, SCHEME_RAY, 8};
+    (void)spp;
+    if (path == PATH_BATCHED && g_camera_auto) return TraceTuning{256, 16, 0, SCHEME_PACKET};
     return g_tuning[path][2];
 }
 
