@@ -76,6 +76,53 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}))
 
 
+def measured_peaks():
+    """Achievable ceilings on this GPU: a streaming device copy of 2 GiB
+    (bytes read + written per second) and the fma-chain probe (non-packed
+    FP32 fmas), best of a few timed runs after a warm-up."""
+    import ctypes
+    import pbrt_rust_amd as px
+    from pbrt_rust_amd._lib import check
+    L = px.lib()
+    ms = ctypes.c_float()
+
+    def timed(fn, reps):
+        fn()
+        px.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            check(L.pbrtx_timer_start(None))
+            fn()
+            check(L.pbrtx_timer_stop(None, ctypes.byref(ms)))
+            best = min(best, ms.value)
+        return best
+
+    nbytes = 1 << 31
+    a, b = px.DeviceBuffer(nbytes), px.DeviceBuffer(nbytes)
+    copy_ms = timed(lambda: check(L.pbrtx_device_copy(b.ptr, a.ptr, nbytes, None)), 5)
+    a.free()
+    b.free()
+    threads = int(L.pbrtx_fma_probe_threads())
+    out = px.DeviceBuffer(threads * 4)
+    flops = ctypes.c_uint64()
+    iters = 1 << 15
+    fma_ms = timed(lambda: check(L.pbrtx_fma_probe(out.ptr, threads, iters, ctypes.byref(flops), None)), 3)
+    out.free()
+    return {"hbm_copy_gbs": round(2 * nbytes / (copy_ms * 1e-3) / 1e9, 1),
+            "fp32_fma_tflops": round(flops.value / (fma_ms * 1e-3) / 1e12, 2),
+            "how": "2 GiB device copy (read + write); %d threads x 8 chains x %d fmas" % (threads, iters)}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(args, *a):
     """progress lines on stderr (the result is the one JSON line on stdout)"""
     print(*a, file=sys.stderr, flush=True)
@@ -140,6 +187,8 @@ def main():
     if not args.no_render_serve:
         render_serve = render_serving(args, px, scenes, lambda m: log(args, m))
 
+    peaks = measured_peaks()
+    log(args, "measured ceilings: %s" % peaks)
     # ---- roofline of the dominant kernel (per-launch averages, HIP events on
     # the render stream) ----
     any_ms = float(np.mean([s["ms_trace_any"] / max(1, s["launches_any"]) for s in sts_b]))
@@ -215,9 +264,12 @@ def main():
             "alg_bytes_per_ray": round(bytes_per_ray, 2),
             "rays_per_launch": int(n_rays),
             "measured_hbm_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
+            "measured_peaks": peaks,
+            "frac_of_measured_hbm": round(achieved / peaks["hbm_copy_gbs"], 4),
             "valu": {"achieved": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                     "issue_frac": valu_issue},
+                     "issue_frac": valu_issue,
+                     "frac_of_measured_fp32": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / peaks["fp32_fma_tflops"], 4)},
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "serving": serve,
@@ -537,7 +589,9 @@ def cpu_baseline(scene, cam, nx, ny, seed, seconds):
     per-unit figures the roofline uses."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
-    cores = os.cpu_count() or 1
+    # the host share given to this process (the pool sets OMP_NUM_THREADS),
+    # else every core
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
     integ = dict(kind="direct", nx=nx, ny=ny, seed=seed, light_samples=4)
     # calibrate: 2 rows first, then as many rows as fit the budget
     y0 = cam["yres"] // 2
@@ -551,9 +605,14 @@ def cpu_baseline(scene, cam, nx, ny, seed, seconds):
     st = [int(v) for v in res["stats"]]
     trace_s = st[4] * 1e-9
     rays = st[0] + st[1]
+    # one thread on two rows, for the per-core rate
+    one = ol.render(scene, cam, integ, y0=y0, y1=y0 + 2, nthreads=1)
+    s1 = [int(v) for v in one["stats"]]
     cpu = {"value": round(rays / trace_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
            "sample": "oracle render of rows %d..%d of the same frame (%d camera + %d shadow rays, "
-                     "shading included), BVH build excluded" % (y0, y0 + rows, st[0], st[1])}
+                     "shading included), BVH build excluded" % (y0, y0 + rows, st[0], st[1]),
+           "one_thread_mrays_per_s": round((s1[0] + s1[1]) / (s1[4] * 1e-9) / 1e6, 4),
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     work = {"closest_nodes": st[5] / max(1, st[0]), "closest_tris": st[6] / max(1, st[0]),
             "any_nodes": st[7] / max(1, st[1]), "any_tris": st[8] / max(1, st[1])}
     return cpu, work

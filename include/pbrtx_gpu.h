@@ -297,6 +297,12 @@ int pbrtx_synchronize(void);
    point for calibrating memory-traffic counters: it moves exactly `bytes`
    in and `bytes` out) */
 int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+/* FP32 vector probe: pbrtx_fma_probe_threads() threads each run eight
+   independent chains of `iters` fused multiply-adds and write one float to
+   d_out (out_floats >= that count); *flops receives the flops launched.
+   Time it with the timer below for the achievable non-packed FP32 rate. */
+uint64_t pbrtx_fma_probe_threads(void);
+int pbrtx_fma_probe(float* d_out, uint64_t out_floats, uint32_t iters, uint64_t* flops, void* stream);
 /* HIP-event timer on a stream: start, then stop returns elapsed ms */
 int pbrtx_timer_start(void* stream);
 int pbrtx_timer_stop(void* stream, float* ms);

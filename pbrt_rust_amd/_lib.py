@@ -131,6 +131,9 @@ def lib():
     L.pbrtx_set_stack_depth.argtypes = [i32, i32, i32]
     L.pbrtx_reset_tuning.argtypes = []
     L.pbrtx_device_copy.argtypes = [vp, vp, u64, vp]
+    L.pbrtx_fma_probe_threads.argtypes = []
+    L.pbrtx_fma_probe_threads.restype = u64
+    L.pbrtx_fma_probe.argtypes = [vp, u64, ctypes.c_uint32, ctypes.POINTER(u64), vp]
     L.pbrtx_device_malloc.argtypes = [c.POINTER(vp), u64]
     L.pbrtx_device_free.argtypes = [vp]
     L.pbrtx_memcpy_htod.argtypes = [vp, vp, u64]

@@ -386,6 +386,16 @@ int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stre
     return PBRTX_OK;
 }
 
+int pbrtx_fma_probe(float* d_out, uint64_t out_floats, uint32_t iters, uint64_t* flops, void* stream) {
+    const uint64_t n = fma_chain_threads();
+    if (!d_out || out_floats < n) return fail(PBRTX_ERR_ARG, "fma probe needs a device buffer of pbrtx_fma_probe_threads() floats");
+    hipError_t e = fma_chain(d_out, iters, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "fma probe");
+    if (flops) *flops = n * 8ull * 2ull * iters;
+    return PBRTX_OK;
+}
+uint64_t pbrtx_fma_probe_threads(void) { return fma_chain_threads(); }
+
 int pbrtx_timer_start(void* stream) {
     std::lock_guard<std::mutex> lk(g_ev_mu);
     hipError_t e;

@@ -1032,10 +1032,41 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void k_copy16(uint4* dst, const uint4* src, size_t n) {
+// Streaming copy, four 16-byte loads in flight a thread before their
+// stores, marked non-temporal (read once, written once).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_copy16(uint4* dst_, const uint4* src_, size_t n) {
+    u32x4* dst = reinterpret_cast<u32x4*>(dst_);
+    const u32x4* src = reinterpret_cast<const u32x4*>(src_);
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
     for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// Eight independent fma chains a thread: a measure of the achievable FP32
+// vector rate (one fma per lane per instruction; no packed math).
+__global__ __launch_bounds__(256) void k_fma_chain(float* out, uint32_t iters, float a, float b) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = (float)(gid & 1023u) * 1e-3f + (float)k;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = __builtin_fmaf(x[k], a, b);
+    }
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += x[k];
+    out[gid] = t;
 }
 
 int g_cu_count = 0;
@@ -1080,9 +1111,17 @@ hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st) 
     const size_t n = bytes / 16;
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;
-    const size_t cap = (size_t)device_cu_count() * 8;
+    const size_t cap = (size_t)device_cu_count() * 16;
     if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n);
+    return hipGetLastError();
+}
+
+uint64_t fma_chain_threads() { return (uint64_t)device_cu_count() * 8 * 256; }
+
+hipError_t fma_chain(float* out, uint32_t iters, hipStream_t st) {
+    const unsigned blocks = (unsigned)(fma_chain_threads() / 256);
+    hipLaunchKernelGGL(k_fma_chain, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f, 1e-3f);
     return hipGetLastError();
 }
 

@@ -86,6 +86,10 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
 // streaming 16-byte-vector copy (bytes a multiple of 16)
 hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st);
+// fma-chain probe: fma_chain_threads() threads, 8 chains of `iters` fmas
+// each; out holds one float per thread
+uint64_t fma_chain_threads();
+hipError_t fma_chain(float* out, uint32_t iters, hipStream_t st);
 int device_cu_count();
 
 }  // namespace px
