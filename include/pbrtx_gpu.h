@@ -171,6 +171,18 @@ int pbrtx_renderer_set_graph(pbrtx_renderer* r, int32_t enable);
    its queue position.  No counterpart in the reference (its rays are traced
    one at a time); off by default. */
 int pbrtx_renderer_set_octant_sort(pbrtx_renderer* r, int32_t enable);
+/* The film's filter (Film::image with a Filter, film.rs:77-122; filter.rs:44-86):
+   kind 0 mean (box), 1 triangle, 2 gaussian (p0 = alpha), 3 mitchell
+   (p0 = B, p1 = C), 4 lanczos (p0 = tau); xw, yw the half-widths.  Samples
+   are added as Film::add_sample (film.rs:192-249) does, with the 16 x 16
+   weight table, in pixel and sample order.  The default is Filter::mean(0.5,
+   0.5). */
+int pbrtx_renderer_set_filter(pbrtx_renderer* r, int32_t kind, float xw, float yw, float p0, float p1);
+/* Filter::evaluate (filter.rs:89-124) and the film's weight table
+   (film.rs:89-97, 256 floats, row-major in y) of a filter given as above;
+   host only, no device needed. */
+int pbrtx_filter_eval(int32_t kind, float xw, float yw, float p0, float p1, float x, float y, float* out);
+int pbrtx_filter_table(int32_t kind, float xw, float yw, float p0, float p1, float* out256);
 /* renders one frame with the given traversal path; rgb_out (host, may be
    NULL) receives xres*yres*3 floats: the film's final rgb */
 int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);

@@ -1197,6 +1197,105 @@ void orc_film_extents(uint32_t xres, uint32_t yres, float xw, float yw, const fl
     out[4] = xs; out[5] = xs + xc; out[6] = ys; out[7] = ys + yc;
 }
 
+}  // extern "C"
+
+// ---- Filter (filter.rs:4-124), sinc_1d (utils/mod.rs:207-217) and the
+// film's weight table (film.rs:89-97).  kind 0 mean, 1 triangle, 2 gaussian
+// (p0 = alpha), 3 mitchell (p0 = b, p1 = c), 4 lanczos (p0 = tau).
+struct OFilter {
+    int kind;
+    float xw, yw, ixw, iyw;
+    float alpha, exp_x, exp_y, b, c, tau;
+};
+static bool make_filter(int kind, float xw, float yw, float p0, float p1, OFilter* f) {
+    if (kind < 0 || kind > 4 || !(xw > 0.0f) || !(yw > 0.0f)) return false;
+    f->kind = kind;
+    f->xw = xw; f->yw = yw;
+    f->ixw = 1.0f / xw; f->iyw = 1.0f / yw;  // FilterBase::new
+    f->alpha = p0;
+    f->exp_x = std::exp(-p0 * xw * xw);   // Filter::gaussian: (-a * xw * xw).exp()
+    f->exp_y = std::exp(-p0 * yw * yw);
+    f->b = p0; f->c = p1; f->tau = p0;
+    return true;
+}
+static float sinc_1d(float x, float tau) {
+    float v = std::fabs(x);
+    if (v < 1e-5f) return 1.0f;
+    if (v >= 1.0f) return 0.0f;
+    v = v * 3.14159265358979323846f;  // f32::consts::PI
+    const float vtau = v * tau;
+    const float s = std::sin(vtau) / vtau;
+    return s * std::sin(v) / v;
+}
+static float mitchell_1d(float v, float b, float c) {
+    const float t = std::fabs(v * 2.0f);
+    float p;
+    if (t >= 2.0f) p = 0.0f;
+    else if (t > 1.0f)
+        p = (-b - 6.0f * c) * t * t * t + (6.0f * b + 30.0f * c) * t * t + (-12.0f * b - 48.0f * c) * t +
+            (8.0f * b + 24.0f * c);
+    else
+        p = (12.0f - 9.0f * b - 6.0f * c) * t * t * t + (-18.0f + 12.0f * b + 6.0f * c) * t * t + (6.0f - 2.0f * b);
+    return (1.0f / 6.0f) * p;
+}
+static float filter_eval(const OFilter& f, float x, float y) {
+    switch (f.kind) {
+        case 0: return 1.0f;
+        case 1: {
+            const float dx = (f.xw - std::fabs(x)) * f.ixw;
+            const float dy = (f.yw - std::fabs(y)) * f.iyw;
+            return rmax(dx, 0.0f) * rmax(dy, 0.0f);
+        }
+        case 2: {
+            const float gx = rmax(std::exp(-f.alpha * x * x) - f.exp_x, 0.0f);
+            const float gy = rmax(std::exp(-f.alpha * y * y) - f.exp_y, 0.0f);
+            return gx * gy;
+        }
+        case 3: return mitchell_1d(x * f.ixw, f.b, f.c) * mitchell_1d(y * f.iyw, f.b, f.c);
+        default: return sinc_1d(x * f.ixw, f.tau) * sinc_1d(y * f.iyw, f.tau);
+    }
+}
+constexpr int FTD = 16;  // FILTER_TABLE_DIM (film.rs:12)
+static void filter_table(const OFilter& f, float* ft) {
+    for (int y = 0; y < FTD; ++y) {
+        const float fy = ((float)y + 0.5f) * f.yw / (float)FTD;
+        for (int x = 0; x < FTD; ++x) {
+            const float fx = ((float)x + 0.5f) * f.xw / (float)FTD;
+            ft[y * FTD + x] = filter_eval(f, fx, fy);
+        }
+    }
+}
+// Film::add_sample's raster extent on one axis (film.rs:197-208): pixels
+// [lo, hi] of an image of n pixels; and the table offset of pixel x
+// (film.rs:214-222).
+static void sample_extent(float image_v, float w, int32_t n, int32_t* lo, int32_t* hi) {
+    const float d = image_v - 0.5f;
+    *lo = std::max(0, (int32_t)std::ceil(d - w));
+    *hi = std::min(n - 1, (int32_t)std::floor(d + w));
+}
+static int table_offset(int32_t x, float image_v, float inv_w) {
+    const float d = image_v - 0.5f;
+    const float f = ((float)x - d) * inv_w * (float)FTD;
+    const float a = std::floor(std::fabs(f));
+    if (!(a < (float)(FTD - 1))) return a == a ? FTD - 1 : 0;  // `as usize` saturates; NaN -> 0
+    return (int)a;
+}
+
+extern "C" {
+
+int orc_filter_eval(int kind, float xw, float yw, float p0, float p1, float x, float y, float* out) {
+    OFilter f;
+    if (!make_filter(kind, xw, yw, p0, p1, &f)) return -1;
+    *out = filter_eval(f, x, y);
+    return 0;
+}
+int orc_filter_table(int kind, float xw, float yw, float p0, float p1, float* out256) {
+    OFilter f;
+    if (!make_filter(kind, xw, yw, p0, p1, &f)) return -1;
+    filter_table(f, out256);
+    return 0;
+}
+
 orc_scene* orc_scene_new(void) { return new orc_scene(); }
 void orc_scene_free(orc_scene* s) { delete s; }
 
@@ -1475,9 +1574,12 @@ void orc_pixel_samples(uint32_t seed, uint32_t xres, uint32_t px, uint32_t py, u
 }
 
 // SamplerRenderer::render restated (sampler_renderer.rs:61-144,184-206) for
-// rows [y0,y1): per-sample Li, then the box-filtered film (Filter::mean(0.5,
-// 0.5), Film::add_sample film.rs:192-249) gathered per pixel from the samples
-// of the 3x3 neighbour pixels in row-major order, sample order within each.
+// rows [y0,y1): per-sample Li, then the film (Film::add_sample film.rs:192-249
+// with the desc's filter and its weight table; Filter::mean(0.5, 0.5) when
+// the filter widths are zero) gathered per pixel from the samples of the
+// neighbour pixels that can reach it, in row-major order of those pixels and
+// sample order within each -- the order the samples are added in when the
+// frame's samples are added in pixel and sample order.
 int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthreads,
                float* rgb_out, float* sample_l, uint64_t* stats) {
     if (!d || y1 > d->yres || y0 >= y1 || d->nx == 0 || d->ny == 0) return -1;
@@ -1500,24 +1602,26 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
     if (!perspective_r2c(d->xres, d->yres, d->screen, d->fov_deg, C.r2c)) return -4;
     C.c2w = m4_from(d->cam_to_world);
     const uint32_t xres = d->xres, yres = d->yres, spp = d->nx * d->ny;
-    // rows whose samples can land in [y0,y1): one halo row on each side
-    uint32_t ya = y0 > 0 ? y0 - 1 : 0, yb = std::min(yres, y1 + 1);
+    OFilter F;
+    if (d->filter_xw == 0.0f && d->filter_yw == 0.0f) make_filter(0, 0.5f, 0.5f, 0.0f, 0.0f, &F);
+    else if (!make_filter(d->filter_kind, d->filter_xw, d->filter_yw, d->filter_p0, d->filter_p1, &F)) return -5;
+    float ftab[FTD * FTD];
+    filter_table(F, ftab);
+    // a sample of pixel q reaches pixels within floor(w + 1/2) of q; one more
+    // for the rounding of its extent
+    const int32_t rx = (int32_t)std::floor(F.xw + 0.5f) + 1, ry = (int32_t)std::floor(F.yw + 0.5f) + 1;
+    // rows whose samples can land in [y0,y1)
+    uint32_t ya = (uint32_t)std::max(0, (int32_t)y0 - ry), yb = std::min(yres, y1 + (uint32_t)ry);
     uint32_t rows = yb - ya;
     std::vector<float> L((size_t)rows * xres * spp * 3, 0.0f);
     std::vector<uint8_t> need((size_t)rows * xres * spp, 0);
-    // a sample contributes to pixel x iff ceil(ix-1) <= x <= floor(ix) (box 0.5)
-    auto covers = [](float ixf, int32_t x) {
-        float dimg = ixf - 0.5f;
-        int32_t a = (int32_t)std::ceil(dimg - 0.5f), b = (int32_t)std::floor(dimg + 0.5f);
-        return a <= x && x <= b;
-    };
     for (uint32_t yy = ya; yy < yb; ++yy)
         for (uint32_t x = 0; x < xres; ++x)
             for (uint32_t s = 0; s < spp; ++s) {
                 float ix, iy;
                 sample_pos(d->seed, xres, d->nx, d->ny, true, x, yy, s, &ix, &iy);
-                float dimg = iy - 0.5f;
-                int32_t a = (int32_t)std::ceil(dimg - 0.5f), b = (int32_t)std::floor(dimg + 0.5f);
+                int32_t a, b;
+                sample_extent(iy, F.yw, (int32_t)yres, &a, &b);
                 bool in = !(b < (int32_t)y0 || a >= (int32_t)y1);
                 need[((size_t)(yy - ya) * xres + x) * spp + s] = in ? 1 : 0;
             }
@@ -1546,19 +1650,23 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
         for (uint32_t x = 0; x < xres; ++x) {
             float xyz[3] = {0, 0, 0};
             float wsum = 0.0f;
-            for (int32_t qy = (int32_t)y - 1; qy <= (int32_t)y + 1; ++qy) {
+            for (int32_t qy = (int32_t)y - ry; qy <= (int32_t)y + ry; ++qy) {
                 if (qy < 0 || qy >= (int32_t)yres) continue;
-                for (int32_t qx = (int32_t)x - 1; qx <= (int32_t)x + 1; ++qx) {
+                for (int32_t qx = (int32_t)x - rx; qx <= (int32_t)x + rx; ++qx) {
                     if (qx < 0 || qx >= (int32_t)xres) continue;
                     for (uint32_t s = 0; s < spp; ++s) {
                         float ix, iy;
                         sample_pos(d->seed, xres, d->nx, d->ny, true, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
-                        if (!covers(ix, (int32_t)x) || !covers(iy, (int32_t)y)) continue;
+                        int32_t xa, xb, ya_, yb_;
+                        sample_extent(ix, F.xw, (int32_t)xres, &xa, &xb);
+                        sample_extent(iy, F.yw, (int32_t)yres, &ya_, &yb_);
+                        if ((int32_t)x < xa || (int32_t)x > xb || (int32_t)y < ya_ || (int32_t)y > yb_) continue;
                         size_t k = ((size_t)((uint32_t)qy - ya) * xres + (uint32_t)qx) * spp + s;
                         RGB v{L[3 * k], L[3 * k + 1], L[3 * k + 2]};
                         float c[3];
                         rgb_to_xyz(v, c);
-                        const float wt = 1.0f;  // Filter::mean evaluates to 1 (filter.rs:89)
+                        const float wt = ftab[table_offset((int32_t)y, iy, F.iyw) * FTD +
+                                              table_offset((int32_t)x, ix, F.ixw)];
                         xyz[0] = xyz[0] + wt * c[0];
                         xyz[1] = xyz[1] + wt * c[1];
                         xyz[2] = xyz[2] + wt * c[2];

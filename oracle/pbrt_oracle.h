@@ -137,6 +137,10 @@ typedef struct orc_render_desc {
     uint32_t max_depth;     /* path */
     uint32_t rr_start;      /* path: roulette when bounce > rr_start */
     int32_t split; uint32_t max_prims;
+    /* film filter (filter.rs): kind 0 mean, 1 triangle, 2 gaussian (p0 =
+       alpha), 3 mitchell (p0 = b, p1 = c), 4 lanczos (p0 = tau); widths 0
+       mean Filter::mean(0.5, 0.5) */
+    int32_t filter_kind; float filter_xw, filter_yw, filter_p0, filter_p1;
 } orc_render_desc;
 
 /* Renders rows [y0,y1) of the image: SamplerRenderer::render restated with the
@@ -148,6 +152,11 @@ typedef struct orc_render_desc {
    tested by closest-hit queries (camera and extension rays), [7] nodes
    visited and [8] prims tested by occlusion queries up to their first hit
    (the work of an early-exit any-hit query).  nthreads >= 1. */
+/* Filter::evaluate (filter.rs:89-124) and the film's 16x16 weight table
+   (film.rs:89-97) of a filter given as in orc_render_desc. */
+int orc_filter_eval(int kind, float xw, float yw, float p0, float p1, float x, float y, float* out);
+int orc_filter_table(int kind, float xw, float yw, float p0, float p1, float* out256);
+
 int orc_render(const orc_render_desc* desc, uint32_t y0, uint32_t y1, uint32_t nthreads,
                float* rgb_out, float* sample_l, uint64_t* stats);
 

@@ -131,6 +131,10 @@ def lib():
     L.pbrtx_set_stack_depth.argtypes = [i32, i32, i32]
     L.pbrtx_reset_tuning.argtypes = []
     L.pbrtx_device_copy.argtypes = [vp, vp, u64, vp]
+    f32 = c.c_float
+    L.pbrtx_renderer_set_filter.argtypes = [vp, i32, f32, f32, f32, f32]
+    L.pbrtx_filter_eval.argtypes = [i32, f32, f32, f32, f32, f32, f32, c.POINTER(f32)]
+    L.pbrtx_filter_table.argtypes = [i32, f32, f32, f32, f32, c.POINTER(f32)]
     L.pbrtx_fma_probe_threads.argtypes = []
     L.pbrtx_fma_probe_threads.restype = u64
     L.pbrtx_fma_probe.argtypes = [vp, u64, ctypes.c_uint32, ctypes.POINTER(u64), vp]

@@ -568,11 +568,56 @@ def _set_camera(d, camera):
     d.xres, d.yres = camera["xres"], camera["yres"]
 
 
+class Filter:
+    """A film's filter (filter.rs:44-124): the same constructors and
+    evaluate(); the film samples it through Film::image's 16 x 16 weight
+    table (film.rs:89-97, table())."""
+    KINDS = {"mean": 0, "triangle": 1, "gaussian": 2, "mitchell": 3, "lanczos": 4}
+
+    def __init__(self, kind, xw, yw, p0=0.0, p1=0.0):
+        self.kind, self.xw, self.yw, self.p0, self.p1 = int(kind), float(xw), float(yw), float(p0), float(p1)
+        self.evaluate(0.0, 0.0)  # rejects an unknown kind or a width that is not positive
+
+    @classmethod
+    def mean(cls, xw, yw):
+        return cls(0, xw, yw)
+
+    @classmethod
+    def triangle(cls, xw, yw):
+        return cls(1, xw, yw)
+
+    @classmethod
+    def gaussian(cls, xw, yw, alpha):
+        return cls(2, xw, yw, alpha)
+
+    @classmethod
+    def mitchell(cls, xw, yw, b, c):
+        return cls(3, xw, yw, b, c)
+
+    @classmethod
+    def lanczos(cls, xw, yw, tau):
+        return cls(4, xw, yw, tau)
+
+    def args(self):
+        return (self.kind, self.xw, self.yw, self.p0, self.p1)
+
+    def evaluate(self, x, y):
+        out = ctypes.c_float()
+        check(lib().pbrtx_filter_eval(*self.args(), float(x), float(y), ctypes.byref(out)))
+        return out.value
+
+    def table(self):
+        t = np.zeros(256, F32)
+        check(lib().pbrtx_filter_table(*self.args(), t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return t.reshape(16, 16)
+
+
 class SamplerRenderer:
     """Renderer::render for a generated scene dict (see scenes.py), a camera
-    dict (scenes.pinhole) and an integrator; the frame runs on the GPU."""
+    dict (scenes.pinhole) and an integrator; the frame runs on the GPU.
+    filter: the film's Filter (default Filter.mean(0.5, 0.5))."""
 
-    def __init__(self, scene, camera, sampler, integrator, split="sah", chunk=None):
+    def __init__(self, scene, camera, sampler, integrator, split="sah", chunk=None, filter=None):
         d, self._keep = _render_desc(scene, camera, sampler, integrator, split)
         self.desc = d
         self.xres, self.yres, self.spp = d.xres, d.yres, d.nx * d.ny
@@ -581,6 +626,12 @@ class SamplerRenderer:
         self._h = h.value
         if chunk:
             check(lib().pbrtx_renderer_set_chunk(self._h, int(chunk)))
+        if filter is not None:
+            self.set_filter(filter)
+
+    def set_filter(self, f):
+        """The film's filter (a Filter)."""
+        check(lib().pbrtx_renderer_set_filter(self._h, *f.args()))
 
     def set_chunk(self, samples):
         """Samples per work chunk (whole pixels): the capacity of the ray

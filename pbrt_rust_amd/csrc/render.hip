@@ -907,9 +907,126 @@ __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sampl
     rgb[3 * pix] = o0; rgb[3 * pix + 1] = o1; rgb[3 * pix + 2] = o2;
 }
 
+// Film with a tabulated filter (Film::add_sample film.rs:192-249 with the
+// weight table of film.rs:89-97): each pixel gathers, in row-major order of
+// the pixels within the filter's reach and sample order within each, the
+// samples whose raster extent (film.rs:197-208) covers it, weighted by the
+// table entry of its offsets (film.rs:214-229) -- the additions the
+// reference makes for that pixel when the frame's samples are added in
+// pixel and sample order.
+__device__ __forceinline__ void film_extent(float v, float w, int32_t n, int32_t* lo, int32_t* hi) {
+    const float d = v - 0.5f;
+    *lo = max(0, (int32_t)ceilf(d - w));
+    *hi = min(n - 1, (int32_t)floorf(d + w));
+}
+__device__ __forceinline__ int32_t film_offset(int32_t p, float v, float inv_w) {
+    const float d = v - 0.5f;
+    const float a = floorf(fabsf(((float)p - d) * inv_w * (float)FILTER_TABLE_DIM));
+    if (!(a < (float)(FILTER_TABLE_DIM - 1))) return a == a ? FILTER_TABLE_DIM - 1 : 0;  // `as usize` saturates
+    return (int32_t)a;
+}
+__global__ __launch_bounds__(RB) void k_film_filter(RenderParams P, const float4* sample_l, float* rgb) {
+    __shared__ float s_tab[FILTER_TABLE_DIM * FILTER_TABLE_DIM];
+    for (uint32_t k = threadIdx.x; k < FILTER_TABLE_DIM * FILTER_TABLE_DIM; k += RB) s_tab[k] = P.ftab[k];
+    __syncthreads();
+    const int32_t x = (int32_t)(blockIdx.x * FT + threadIdx.x % FT), y = (int32_t)(blockIdx.y * FT + threadIdx.x / FT);
+    if (x >= (int32_t)P.xres || y >= (int32_t)P.yres) return;
+    const int32_t xres = (int32_t)P.xres, yres = (int32_t)P.yres;
+    float xyz[3] = {0.0f, 0.0f, 0.0f};
+    float wsum = 0.0f;
+    for (int32_t qy = max(0, y - P.fry); qy <= min(yres - 1, y + P.fry); ++qy) {
+        for (int32_t qx = max(0, x - P.frx); qx <= min(xres - 1, x + P.frx); ++qx) {
+            const float4* sl = sample_l + ((size_t)qy * P.xres + (size_t)qx) * P.spp;
+            for (uint32_t s = 0; s < P.spp; ++s) {
+                float ix, iy;
+                sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                int32_t x0, x1, y0, y1;
+                film_extent(ix, P.fxw, xres, &x0, &x1);
+                film_extent(iy, P.fyw, yres, &y0, &y1);
+                if (x < x0 || x > x1 || y < y0 || y > y1) continue;
+                const float wt = s_tab[film_offset(y, iy, P.fiyw) * FILTER_TABLE_DIM + film_offset(x, ix, P.fixw)];
+                const float4 v = sl[s];
+                // rgb_to_xyz (spectrum.rs:37-41)
+                const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
+                const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
+                const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
+                xyz[0] = xyz[0] + wt * c0;
+                xyz[1] = xyz[1] + wt * c1;
+                xyz[2] = xyz[2] + wt * c2;
+                wsum = wsum + wt;
+            }
+        }
+    }
+    // xyz_to_rgb: spectrum.rs:31-35
+    float o0 = 3.240479f * xyz[0] - 1.37150f * xyz[1] - 0.498535f * xyz[2];
+    float o1 = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
+    float o2 = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
+    if (wsum != 0.0f) {
+        const float inv = 1.0f / wsum;
+        o0 = fmaxf(o0 * inv, 0.0f); o1 = fmaxf(o1 * inv, 0.0f); o2 = fmaxf(o2 * inv, 0.0f);
+    }
+    const size_t pix = (size_t)y * P.xres + (size_t)x;
+    rgb[3 * pix] = o0; rgb[3 * pix + 1] = o1; rgb[3 * pix + 2] = o2;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+
+// ---- Filter::evaluate (filter.rs:89-124), sinc_1d (utils/mod.rs:207-217) and
+// the film's weight table (film.rs:89-97)
+bool filter_valid(const FilmFilter& f) {
+    return f.kind >= 0 && f.kind <= 4 && f.xw > 0.0f && f.yw > 0.0f && std::isfinite(f.xw) && std::isfinite(f.yw);
+}
+namespace {
+float sinc_1d(float x, float tau) {
+    float v = std::fabs(x);
+    if (v < 1e-5f) return 1.0f;
+    if (v >= 1.0f) return 0.0f;
+    v = v * 3.14159265358979323846f;
+    const float vtau = v * tau;
+    const float sv = std::sin(vtau) / vtau;
+    return sv * std::sin(v) / v;
+}
+float mitchell_1d(float v, float b, float c) {
+    const float t = std::fabs(v * 2.0f);
+    float p;
+    if (t >= 2.0f) p = 0.0f;
+    else if (t > 1.0f)
+        p = (-b - 6.0f * c) * t * t * t + (6.0f * b + 30.0f * c) * t * t + (-12.0f * b - 48.0f * c) * t +
+            (8.0f * b + 24.0f * c);
+    else
+        p = (12.0f - 9.0f * b - 6.0f * c) * t * t * t + (-18.0f + 12.0f * b + 6.0f * c) * t * t + (6.0f - 2.0f * b);
+    return (1.0f / 6.0f) * p;
+}
+}  // namespace
+float filter_evaluate(const FilmFilter& f, float x, float y) {
+    const float ixw = 1.0f / f.xw, iyw = 1.0f / f.yw;  // FilterBase::new
+    switch (f.kind) {
+        case 0: return 1.0f;
+        case 1: {
+            const float dx = (f.xw - std::fabs(x)) * ixw, dy = (f.yw - std::fabs(y)) * iyw;
+            return std::fmax(dx, 0.0f) * std::fmax(dy, 0.0f);
+        }
+        case 2: {
+            const float a = f.p0;
+            const float ex = std::exp(-a * f.xw * f.xw), ey = std::exp(-a * f.yw * f.yw);  // Filter::gaussian
+            return std::fmax(std::exp(-a * x * x) - ex, 0.0f) * std::fmax(std::exp(-a * y * y) - ey, 0.0f);
+        }
+        case 3: return mitchell_1d(x * ixw, f.p0, f.p1) * mitchell_1d(y * iyw, f.p0, f.p1);
+        default: return sinc_1d(x * ixw, f.p0) * sinc_1d(y * iyw, f.p0);
+    }
+}
+void filter_weight_table(const FilmFilter& f, float* out) {
+    for (int y = 0; y < FILTER_TABLE_DIM; ++y) {
+        const float fy = ((float)y + 0.5f) * f.yw / (float)FILTER_TABLE_DIM;
+        for (int x = 0; x < FILTER_TABLE_DIM; ++x) {
+            const float fx = ((float)x + 0.5f) * f.xw / (float)FILTER_TABLE_DIM;
+            out[y * FILTER_TABLE_DIM + x] = filter_evaluate(f, fx, fy);
+        }
+    }
+}
+
 namespace {
 
 // ---- Camera::perspective + Projection::new (camera/mod.rs:105-135,
@@ -1160,7 +1277,8 @@ size_t render_spill_bytes(const SceneView& sv, int path) {
 
 hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st) {
     const dim3 grid((P.xres + FT - 1) / FT, (P.yres + FT - 1) / FT);
-    hipLaunchKernelGGL(k_film, grid, dim3(RB), 0, st, P, sample_l, rgb);
+    if (P.ftab) hipLaunchKernelGGL(k_film_filter, grid, dim3(RB), 0, st, P, sample_l, rgb);
+    else hipLaunchKernelGGL(k_film, grid, dim3(RB), 0, st, P, sample_l, rgb);
     return hipGetLastError();
 }
 
@@ -1290,6 +1408,7 @@ struct Renderer {
     DevArr<float4> sample_l;
     DevArr<uint32_t> counters;
     DevArr<float> rgb;
+    DevArr<float> ftab;  // the film filter's weight table (none: box 0.5)
     hipStream_t stream = nullptr;
     EventPool events;
     bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
@@ -1533,6 +1652,46 @@ int pbrtx_renderer_set_octant_sort(pbrtx_renderer* h, int32_t enable) {
     return PBRTX_OK;
 }
 
+int pbrtx_renderer_set_filter(pbrtx_renderer* h, int32_t kind, float xw, float yw, float p0, float p1) {
+    if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
+    FilmFilter f;
+    f.kind = kind; f.xw = xw; f.yw = yw; f.p0 = p0; f.p1 = p1;
+    if (!filter_valid(f)) return fail(PBRTX_ERR_ARG, "unknown filter kind or width not positive");
+    Renderer& R = h->r;
+    R.drop_graphs();  // the frame's film launch changes
+    if (kind == 0 && xw == 0.5f && yw == 0.5f) {
+        R.P.ftab = nullptr;  // the box filter's own film kernel
+        return PBRTX_OK;
+    }
+    std::vector<float> t(FILTER_TABLE_DIM * FILTER_TABLE_DIM);
+    filter_weight_table(f, t.data());
+    ICHK(R.ftab.upload(t));
+    R.P.ftab = R.ftab.p;
+    R.P.fxw = xw; R.P.fyw = yw;
+    R.P.fixw = 1.0f / xw; R.P.fiyw = 1.0f / yw;
+    // a sample of pixel q reaches pixels within floor(w + 1/2) of q; one more
+    // for the rounding of its extent
+    R.P.frx = (int32_t)std::floor(xw + 0.5f) + 1;
+    R.P.fry = (int32_t)std::floor(yw + 0.5f) + 1;
+    return PBRTX_OK;
+}
+
+int pbrtx_filter_eval(int32_t kind, float xw, float yw, float p0, float p1, float x, float y, float* out) {
+    FilmFilter f;
+    f.kind = kind; f.xw = xw; f.yw = yw; f.p0 = p0; f.p1 = p1;
+    if (!filter_valid(f) || !out) return fail(PBRTX_ERR_ARG, "unknown filter kind, width not positive or null output");
+    *out = filter_evaluate(f, x, y);
+    return PBRTX_OK;
+}
+
+int pbrtx_filter_table(int32_t kind, float xw, float yw, float p0, float p1, float* out256) {
+    FilmFilter f;
+    f.kind = kind; f.xw = xw; f.yw = yw; f.p0 = p0; f.p1 = p1;
+    if (!filter_valid(f) || !out256) return fail(PBRTX_ERR_ARG, "unknown filter kind, width not positive or null output");
+    filter_weight_table(f, out256);
+    return PBRTX_OK;
+}
+
 int pbrtx_renderer_set_graph(pbrtx_renderer* h, int32_t enable) {
     if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
     h->r.use_graph = enable != 0;
@@ -1573,7 +1732,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.drop_graphs();
     R.sc.release();
     R.W.release();
-    R.sample_l.release(); R.counters.release(); R.rgb.release();
+    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;

@@ -62,7 +62,26 @@ struct RenderParams {
     const RenderSeg* segs;
     const ReqView* reqs;
     uint32_t nsegs;
+    // film filter (frames): its 16 x 16 weight table, widths and their
+    // inverses, and how many pixels a sample reaches either way; a null table
+    // is the box filter of half-width 0.5
+    const float* ftab;
+    float fxw, fyw, fixw, fiyw;
+    int32_t frx, fry;
 };
+
+// Filter (filter.rs:4-124) of a film: kind 0 mean, 1 triangle, 2 gaussian
+// (p0 = alpha), 3 mitchell (p0 = b, p1 = c), 4 lanczos (p0 = tau).
+struct FilmFilter {
+    int32_t kind = 0;
+    float xw = 0.5f, yw = 0.5f, p0 = 0.0f, p1 = 0.0f;
+};
+constexpr int FILTER_TABLE_DIM = 16;  // film.rs:12
+// Filter::evaluate and Film::image's weight table (film.rs:89-97); false for
+// an unknown kind or a width that is not positive and finite.
+bool filter_valid(const FilmFilter& f);
+float filter_evaluate(const FilmFilter& f, float x, float y);
+void filter_weight_table(const FilmFilter& f, float* out256);
 
 struct PathState {
     float beta[3];

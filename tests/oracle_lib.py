@@ -37,7 +37,24 @@ class RenderDesc(ctypes.Structure):
         ("integrator", ctypes.c_int32), ("light_samples", ctypes.c_uint32),
         ("max_depth", ctypes.c_uint32), ("rr_start", ctypes.c_uint32),
         ("split", ctypes.c_int32), ("max_prims", ctypes.c_uint32),
+        ("filter_kind", ctypes.c_int32), ("filter_xw", ctypes.c_float), ("filter_yw", ctypes.c_float),
+        ("filter_p0", ctypes.c_float), ("filter_p1", ctypes.c_float),
     ]
+
+# film filters as (kind, x width, y width, p0, p1), filter.rs:44-86
+FILTER_KINDS = {"mean": 0, "triangle": 1, "gaussian": 2, "mitchell": 3, "lanczos": 4}
+
+
+def filter_eval(filt, x, y):
+    out = ctypes.c_float()
+    assert lib().orc_filter_eval(*filt, x, y, ctypes.byref(out)) == 0
+    return out.value
+
+
+def filter_table(filt):
+    t = np.zeros(256, np.float32)
+    assert lib().orc_filter_table(*filt, t) == 0
+    return t
 
 
 def build():
@@ -56,6 +73,10 @@ def lib():
     L.orc_quadratic.argtypes = [ctypes.c_float] * 3 + [fp, fp]
     L.orc_partition_by_i32.argtypes = [I32P, ctypes.c_uint32]
     L.orc_partition_by_f32.argtypes = [F32P, ctypes.c_uint32]
+    L.orc_filter_eval.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, fp]
+    L.orc_filter_table.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   F32P]
     L.orc_get_crop_window.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, F32P]
     L.orc_compute_sub_window.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_uint32, ctypes.c_uint32, I32P]
     L.orc_matrix_inverse.argtypes = [F32P, F32P]
@@ -217,10 +238,14 @@ def make_render_desc(scene, cam, integ, keep):
     return d
 
 
-def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False):
+def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False, filt=None):
+    """filt: (kind, xw, yw, p0, p1) of the film filter, None for
+    Filter::mean(0.5, 0.5)"""
     L = lib()
     keep = {}
     d = make_render_desc(scene, cam, integ, keep)
+    if filt is not None:
+        d.filter_kind, d.filter_xw, d.filter_yw, d.filter_p0, d.filter_p1 = filt
     if y1 is None:
         y1 = cam["yres"]
     out = np.zeros((y1 - y0) * cam["xres"] * 3, np.float32)
