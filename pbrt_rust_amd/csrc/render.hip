@@ -913,7 +913,10 @@ __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sampl
 // samples whose raster extent (film.rs:197-208) covers it, weighted by the
 // table entry of its offsets (film.rs:214-229) -- the additions the
 // reference makes for that pixel when the frame's samples are added in
-// pixel and sample order.
+// pixel and sample order.  Every sample is read by up to (2R+1)^2 pixels, so
+// its position within its pixel and its xyz are staged once, sample-major:
+// the 64 pixels of a wave (a 16 x 4 block of the tile) then read sample s of
+// 64 neighbouring pixels as four contiguous runs.
 __device__ __forceinline__ void film_extent(float v, float w, int32_t n, int32_t* lo, int32_t* hi) {
     const float d = v - 0.5f;
     *lo = max(0, (int32_t)ceilf(d - w));
@@ -925,7 +928,24 @@ __device__ __forceinline__ int32_t film_offset(int32_t p, float v, float inv_w) 
     if (!(a < (float)(FILTER_TABLE_DIM - 1))) return a == a ? FILTER_TABLE_DIM - 1 : 0;  // `as usize` saturates
     return (int32_t)a;
 }
-__global__ __launch_bounds__(RB) void k_film_filter(RenderParams P, const float4* sample_l, float* rgb) {
+__global__ __launch_bounds__(RB) void k_film_stage(RenderParams P, const float4* sample_l, uint64_t n) {
+    const uint64_t npix = (uint64_t)P.xres * P.yres;
+    for (uint64_t g = (uint64_t)blockIdx.x * RB + threadIdx.x; g < n; g += (uint64_t)gridDim.x * RB) {
+        const uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
+        float u[4];
+        rng4(P.seed, pixel, s, 0, u);  // sample_pos's jitter
+        const uint32_t sx = s % P.nx, sy = s / P.nx;
+        const float4 v = sample_l[g];
+        // rgb_to_xyz (spectrum.rs:37-41)
+        const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
+        const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
+        const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
+        const uint64_t t = (uint64_t)s * npix + pixel;
+        P.fpos[t] = make_float2(((float)sx + u[0]) * P.dx, ((float)sy + u[1]) * P.dy);
+        P.fxyz[t] = make_float4(c0, c1, c2, 0.0f);
+    }
+}
+__global__ __launch_bounds__(RB) void k_film_filter(RenderParams P, float* rgb) {
     __shared__ float s_tab[FILTER_TABLE_DIM * FILTER_TABLE_DIM];
     for (uint32_t k = threadIdx.x; k < FILTER_TABLE_DIM * FILTER_TABLE_DIM; k += RB) s_tab[k] = P.ftab[k];
     __syncthreads();
@@ -936,23 +956,20 @@ __global__ __launch_bounds__(RB) void k_film_filter(RenderParams P, const float4
     float wsum = 0.0f;
     for (int32_t qy = max(0, y - P.fry); qy <= min(yres - 1, y + P.fry); ++qy) {
         for (int32_t qx = max(0, x - P.frx); qx <= min(xres - 1, x + P.frx); ++qx) {
-            const float4* sl = sample_l + ((size_t)qy * P.xres + (size_t)qx) * P.spp;
+            const size_t npix = (size_t)P.xres * P.yres, q = (size_t)qy * P.xres + (size_t)qx;
             for (uint32_t s = 0; s < P.spp; ++s) {
-                float ix, iy;
-                sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                // sample_pos: the position within the pixel plus the pixel's coordinates
+                const float2 f = P.fpos[(size_t)s * npix + q];
+                const float ix = f.x + (float)qx, iy = f.y + (float)qy;
                 int32_t x0, x1, y0, y1;
                 film_extent(ix, P.fxw, xres, &x0, &x1);
                 film_extent(iy, P.fyw, yres, &y0, &y1);
                 if (x < x0 || x > x1 || y < y0 || y > y1) continue;
                 const float wt = s_tab[film_offset(y, iy, P.fiyw) * FILTER_TABLE_DIM + film_offset(x, ix, P.fixw)];
-                const float4 v = sl[s];
-                // rgb_to_xyz (spectrum.rs:37-41)
-                const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
-                const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
-                const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
-                xyz[0] = xyz[0] + wt * c0;
-                xyz[1] = xyz[1] + wt * c1;
-                xyz[2] = xyz[2] + wt * c2;
+                const float4 c = P.fxyz[(size_t)s * npix + q];
+                xyz[0] = xyz[0] + wt * c.x;
+                xyz[1] = xyz[1] + wt * c.y;
+                xyz[2] = xyz[2] + wt * c.z;
                 wsum = wsum + wt;
             }
         }
@@ -1277,8 +1294,14 @@ size_t render_spill_bytes(const SceneView& sv, int path) {
 
 hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st) {
     const dim3 grid((P.xres + FT - 1) / FT, (P.yres + FT - 1) / FT);
-    if (P.ftab) hipLaunchKernelGGL(k_film_filter, grid, dim3(RB), 0, st, P, sample_l, rgb);
-    else hipLaunchKernelGGL(k_film, grid, dim3(RB), 0, st, P, sample_l, rgb);
+    if (P.ftab) {
+        const uint64_t n = (uint64_t)P.xres * P.yres * P.spp;
+        const uint64_t blocks = std::min<uint64_t>((n + RB - 1) / RB, 65536);
+        hipLaunchKernelGGL(k_film_stage, dim3((unsigned)blocks), dim3(RB), 0, st, P, sample_l, n);
+        hipLaunchKernelGGL(k_film_filter, grid, dim3(RB), 0, st, P, rgb);
+    } else {
+        hipLaunchKernelGGL(k_film, grid, dim3(RB), 0, st, P, sample_l, rgb);
+    }
     return hipGetLastError();
 }
 
@@ -1409,6 +1432,8 @@ struct Renderer {
     DevArr<uint32_t> counters;
     DevArr<float> rgb;
     DevArr<float> ftab;  // the film filter's weight table (none: box 0.5)
+    DevArr<float2> fpos;  // and its staging of the frame's samples
+    DevArr<float4> fxyz;
     hipStream_t stream = nullptr;
     EventPool events;
     bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
@@ -1652,12 +1677,24 @@ int pbrtx_renderer_set_octant_sort(pbrtx_renderer* h, int32_t enable) {
     return PBRTX_OK;
 }
 
+// How many pixels either way a sample reaches with half-width w: a sample
+// of pixel q lies in [q, q+1], so its extent [ceil(d - w), floor(d + w)]
+// with d = ix - 1/2 stays within floor(w + 1/2) of q -- unless rounding d +- w
+// crosses an integer, which needs w + 1/2 within an ulp of the coordinates
+// (2^-7 below 2^16) under the next integer: one more then.
+static int32_t filter_reach(float w) {
+    const float r = std::floor(w + 0.5f);
+    return (int32_t)r + ((w + 0.5f) - r > 1.0f - 0x1p-6f ? 1 : 0);
+}
+
 int pbrtx_renderer_set_filter(pbrtx_renderer* h, int32_t kind, float xw, float yw, float p0, float p1) {
     if (!h) return fail(PBRTX_ERR_ARG, "renderer is null");
     FilmFilter f;
     f.kind = kind; f.xw = xw; f.yw = yw; f.p0 = p0; f.p1 = p1;
     if (!filter_valid(f)) return fail(PBRTX_ERR_ARG, "unknown filter kind or width not positive");
     Renderer& R = h->r;
+    if (R.P.xres > 65536u || R.P.yres > 65536u || xw > 4096.0f || yw > 4096.0f)
+        return fail(PBRTX_ERR_ARG, "filtered films: at most 65536 pixels a side and half-widths up to 4096");
     R.drop_graphs();  // the frame's film launch changes
     if (kind == 0 && xw == 0.5f && yw == 0.5f) {
         R.P.ftab = nullptr;  // the box filter's own film kernel
@@ -1666,13 +1703,15 @@ int pbrtx_renderer_set_filter(pbrtx_renderer* h, int32_t kind, float xw, float y
     std::vector<float> t(FILTER_TABLE_DIM * FILTER_TABLE_DIM);
     filter_weight_table(f, t.data());
     ICHK(R.ftab.upload(t));
+    ICHK(R.fpos.ensure(R.n_samples));
+    ICHK(R.fxyz.ensure(R.n_samples));
     R.P.ftab = R.ftab.p;
+    R.P.fpos = R.fpos.p;
+    R.P.fxyz = R.fxyz.p;
     R.P.fxw = xw; R.P.fyw = yw;
     R.P.fixw = 1.0f / xw; R.P.fiyw = 1.0f / yw;
-    // a sample of pixel q reaches pixels within floor(w + 1/2) of q; one more
-    // for the rounding of its extent
-    R.P.frx = (int32_t)std::floor(xw + 0.5f) + 1;
-    R.P.fry = (int32_t)std::floor(yw + 0.5f) + 1;
+    R.P.frx = filter_reach(xw);
+    R.P.fry = filter_reach(yw);
     return PBRTX_OK;
 }
 
@@ -1732,7 +1771,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.drop_graphs();
     R.sc.release();
     R.W.release();
-    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release();
+    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;

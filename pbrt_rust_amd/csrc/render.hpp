@@ -68,6 +68,10 @@ struct RenderParams {
     const float* ftab;
     float fxw, fyw, fixw, fiyw;
     int32_t frx, fry;
+    // its staging, sample-major (index s * pixels + pixel): each sample's
+    // position within its pixel and its xyz
+    float2* fpos;
+    float4* fxyz;
 };
 
 // Filter (filter.rs:4-124) of a film: kind 0 mean, 1 triangle, 2 gaussian

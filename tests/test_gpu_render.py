@@ -219,10 +219,11 @@ def test_graph_replays_match_eager_frames(integ):
 
 
 FILM_FILTERS = [px.Filter.mean(1.5, 1.5), px.Filter.triangle(2.0, 1.25), px.Filter.gaussian(2.0, 2.0, 2.0),
-                px.Filter.mitchell(2.0, 2.0, 1 / 3, 1 / 3), px.Filter.lanczos(3.0, 3.0, 3.0)]
+                px.Filter.mitchell(2.0, 2.0, 1 / 3, 1 / 3), px.Filter.lanczos(3.0, 3.0, 3.0),
+                px.Filter.triangle(1.4999999, 0.9999)]  # half-widths just under the reach steps
 
 
-@pytest.mark.parametrize("f", FILM_FILTERS, ids=["mean1.5", "triangle", "gaussian", "mitchell", "lanczos"])
+@pytest.mark.parametrize("f", FILM_FILTERS, ids=["mean1.5", "triangle", "gaussian", "mitchell", "lanczos", "steps"])
 @pytest.mark.parametrize("integ", [DIRECT, PATH], ids=["direct", "path"])
 def test_film_filters_match_oracle(f, integ):
     # Film::add_sample with each filter's weight table: the image equals the
