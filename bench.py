@@ -218,10 +218,12 @@ def main():
     # share of the SIMDs' VALU issue slots the kernel used (SQ_INSTS_VALU
     # against the dispatch's clock, one wave64 instruction per SIMD every 4
     # cycles), from the committed counter pass
-    valu_issue = None
+    valu_issue = l2_hit = None
     if os.path.exists(PROFILE_VALU):
         try:
-            valu_issue = round(json.load(open(PROFILE_VALU))["kernels"][kname]["valu_issue_frac"], 3)
+            kv = json.load(open(PROFILE_VALU))["kernels"][kname]
+            valu_issue = round(kv["valu_issue_frac"], 3)
+            l2_hit = None if kv.get("l2_hit_rate") is None else round(kv["l2_hit_rate"], 3)
         except (KeyError, ValueError):
             valu_issue = None
 
@@ -265,6 +267,7 @@ def main():
             "rays_per_launch": int(n_rays),
             "measured_hbm_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
             "measured_peaks": peaks,
+            "l2_hit_rate": l2_hit,
             "frac_of_measured_hbm": round(achieved / peaks["hbm_copy_gbs"], 4),
             "valu": {"achieved": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
