@@ -145,6 +145,11 @@ typedef struct pbrtx_render_desc {
     uint32_t max_depth;
     uint32_t rr_start;
     int32_t split; uint32_t max_prims;
+    /* 0 StratifiedSampler (nx x ny strata, jittered; stratified.rs:60-127),
+       1 LDSampler (nx*ny samples per pixel rounded up to a power of two;
+       lds.rs:17-70), its scrambles and shuffles drawn from the counter-based
+       generator.  Requests of the render server are stratified. */
+    int32_t sampler;
 } pbrtx_render_desc;
 
 typedef struct pbrtx_render_stats {

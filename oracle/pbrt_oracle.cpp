@@ -966,6 +966,74 @@ static void sample_pos(uint32_t seed, uint32_t xres, uint32_t nx, uint32_t ny, b
     *iy = (((float)sy + jy) * dy) + (float)py;
 }
 
+// ---- LDSampler's image samples (lds.rs:47-70; sampler/utils.rs:5-32 and
+// ld_shuffle_scrambled_2d :52-69), with the reference's bit tricks as they
+// are (van_der_corput's last swap shifts by 2, sobol2 flips on a zero bit)
+// and its RNG draws taken from the counter-based generator (D6): draw k of
+// pixel p is (v0 << 32 | v1) mod (2^64 - 1) of pcg4d(p, k, LD_BLOCK, seed),
+// as RNG::random_uint reduces a u64 (rng.rs:19-21).
+static constexpr uint32_t LD_BLOCK = 0x4C44u;
+static uint64_t ld_draw(uint32_t seed, uint32_t pixel, uint32_t k) {
+    uint32_t v[4] = {pixel, k, LD_BLOCK, seed};
+    pcg4d(v);
+    const uint64_t u = ((uint64_t)v[0] << 32) | v[1];
+    return u % 0xFFFFFFFFFFFFFFFFull;
+}
+static float ld_vdc(uint32_t n, uint32_t scramble) {
+    n = (n << 16) | (n >> 16);
+    n = ((n & 0x00ff00ffu) << 8) | ((n & 0xff00ff00u) >> 8);
+    n = ((n & 0x0f0f0f0fu) << 4) | ((n & 0xf0f0f0f0u) >> 4);
+    n = ((n & 0x33333333u) << 2) | ((n & 0xCCCCCCCCu) >> 2);
+    n = ((n & 0x55555555u) << 2) | ((n & 0xAAAAAAAAu) >> 2);
+    n ^= scramble;
+    return (float)((double)((n >> 8) & 0xffffffu) / (double)(1u << 24));
+}
+static float ld_sobol2(uint32_t n, uint32_t scramble) {
+    uint32_t s = scramble, v = 1u << 31;
+    while (n != 0) {
+        if ((n & 0x1u) == 0) s ^= v;
+        v ^= v >> 1;
+        n >>= 1;
+    }
+    return (float)((double)((s >> 8) & 0xFFFFFFu) / (double)(1u << 24));
+}
+// the spp image samples of a pixel, positions within the pixel (x, y pairs)
+static void ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t n, float* pts) {
+    const uint32_t sc0 = (uint32_t)ld_draw(seed, pixel, 0), sc1 = (uint32_t)ld_draw(seed, pixel, 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        pts[2 * i] = ld_vdc(i, sc0);       // sample02 (utils.rs:35-37)
+        pts[2 * i + 1] = ld_sobol2(i, sc1);
+    }
+    // draws 2 .. n+1 go to shuffles of one point each (no change); then the
+    // points are shuffled (RNG::shuffle, rng.rs:23-32) with draws n+2 ..
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t other = i + (uint32_t)(ld_draw(seed, pixel, n + 2 + i) % (uint64_t)(n - i));
+        std::swap(pts[2 * i], pts[2 * other]);
+        std::swap(pts[2 * i + 1], pts[2 * other + 1]);
+    }
+}
+static uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+// The sampler of a render: stratified (nx x ny, jittered) or low-discrepancy
+// (spp rounded up to a power of two, LDSampler::new lds.rs:17-24).
+struct SamplerDesc {
+    int kind;  // 0 stratified, 1 low-discrepancy
+    uint32_t seed, xres, nx, ny, spp;
+};
+static void sample_at(const SamplerDesc& S, uint32_t px, uint32_t py, uint32_t s, float* ix, float* iy) {
+    if (S.kind == 0) {
+        sample_pos(S.seed, S.xres, S.nx, S.ny, true, px, py, s, ix, iy);
+        return;
+    }
+    std::vector<float> pts(2 * (size_t)S.spp);
+    ld_pixel_points(S.seed, py * S.xres + px, S.spp, pts.data());
+    *ix = (float)px + pts[2 * s];  // CameraSample::new(x_pos as f32 + image_samples[2i], ..) lds.rs / utils.rs:140-146
+    *iy = (float)py + pts[2 * s + 1];
+}
+
 // Direct lighting Li (deviation D1): emitted light plus, for every light,
 // the average of its samples' f*Li*|cos|/pdf over unoccluded samples
 // (whitted.rs:49-65 light loop; pbrt-v2 UniformSampleAllLights structure).
@@ -1282,6 +1350,13 @@ static int table_offset(int32_t x, float image_v, float inv_w) {
 }
 
 extern "C" {
+
+uint64_t orc_ld_draw(uint32_t seed, uint32_t pixel, uint32_t k) { return ld_draw(seed, pixel, k); }
+float orc_ld_vdc(uint32_t n, uint32_t scramble) { return ld_vdc(n, scramble); }
+float orc_ld_sobol2(uint32_t n, uint32_t scramble) { return ld_sobol2(n, scramble); }
+void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts) {
+    ld_pixel_points(seed, pixel, spp, pts);
+}
 
 int orc_filter_eval(int kind, float xw, float yw, float p0, float p1, float x, float y, float* out) {
     OFilter f;
@@ -1601,7 +1676,11 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
     auto tr = std::chrono::steady_clock::now();
     if (!perspective_r2c(d->xres, d->yres, d->screen, d->fov_deg, C.r2c)) return -4;
     C.c2w = m4_from(d->cam_to_world);
-    const uint32_t xres = d->xres, yres = d->yres, spp = d->nx * d->ny;
+    const uint32_t xres = d->xres, yres = d->yres;
+    if (d->sampler_kind != 0 && d->sampler_kind != 1) return -6;
+    const SamplerDesc S{d->sampler_kind, d->seed, xres, d->nx, d->ny,
+                        d->sampler_kind == 1 ? next_pow2(d->nx * d->ny) : d->nx * d->ny};
+    const uint32_t spp = S.spp;
     OFilter F;
     if (d->filter_xw == 0.0f && d->filter_yw == 0.0f) make_filter(0, 0.5f, 0.5f, 0.0f, 0.0f, &F);
     else if (!make_filter(d->filter_kind, d->filter_xw, d->filter_yw, d->filter_p0, d->filter_p1, &F)) return -5;
@@ -1619,7 +1698,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
         for (uint32_t x = 0; x < xres; ++x)
             for (uint32_t s = 0; s < spp; ++s) {
                 float ix, iy;
-                sample_pos(d->seed, xres, d->nx, d->ny, true, x, yy, s, &ix, &iy);
+                sample_at(S, x, yy, s, &ix, &iy);
                 int32_t a, b;
                 sample_extent(iy, F.yw, (int32_t)yres, &a, &b);
                 bool in = !(b < (int32_t)y0 || a >= (int32_t)y1);
@@ -1636,7 +1715,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
                 size_t k = (size_t)w * spp + s;
                 if (!need[k]) continue;
                 float ix, iy;
-                sample_pos(d->seed, xres, d->nx, d->ny, true, x, yy, s, &ix, &iy);
+                sample_at(S, x, yy, s, &ix, &iy);
                 Ray r = camera_ray(C, ix, iy);
                 RGB v = d->integrator == 1 ? li_path(C, pixel, s, r, st) : li_direct(C, pixel, s, r, st);
                 L[3 * k] = v.r; L[3 * k + 1] = v.g; L[3 * k + 2] = v.b;
@@ -1656,7 +1735,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
                     if (qx < 0 || qx >= (int32_t)xres) continue;
                     for (uint32_t s = 0; s < spp; ++s) {
                         float ix, iy;
-                        sample_pos(d->seed, xres, d->nx, d->ny, true, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                        sample_at(S, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
                         int32_t xa, xb, ya_, yb_;
                         sample_extent(ix, F.xw, (int32_t)xres, &xa, &xb);
                         sample_extent(iy, F.yw, (int32_t)yres, &ya_, &yb_);

@@ -141,7 +141,18 @@ typedef struct orc_render_desc {
        alpha), 3 mitchell (p0 = b, p1 = c), 4 lanczos (p0 = tau); widths 0
        mean Filter::mean(0.5, 0.5) */
     int32_t filter_kind; float filter_xw, filter_yw, filter_p0, filter_p1;
+    /* 0 StratifiedSampler (nx x ny, jittered), 1 LDSampler with nx*ny samples
+       per pixel rounded up to a power of two */
+    int32_t sampler_kind;
 } orc_render_desc;
+
+/* LDSampler's pieces (sampler/utils.rs:5-32, lds.rs): van_der_corput and
+   sobol2 as the reference has them, and a pixel's spp image-sample
+   positions within the pixel (x, y pairs) with the counter-based draws. */
+uint64_t orc_ld_draw(uint32_t seed, uint32_t pixel, uint32_t k);
+float orc_ld_vdc(uint32_t n, uint32_t scramble);
+float orc_ld_sobol2(uint32_t n, uint32_t scramble);
+void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts);
 
 /* Renders rows [y0,y1) of the image: SamplerRenderer::render restated with the
    intended semantics (sampler_renderer.rs:61-144,184-206; deviations D1-D7).

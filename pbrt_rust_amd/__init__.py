@@ -6,12 +6,13 @@ aggregate / renderer interfaces.
 """
 from ._lib import (HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, PinnedArray, device_name, lib,
                    synchronize)
-from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, Filter, PathIntegrator, Primitive, RayBatch,
+from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, Filter, LDSampler, PathIntegrator, Primitive,
+                  RayBatch,
                   RayServer, RenderServer, SamplerRenderer, Sphere, StratifiedSampler, Transform, TriangleMesh,
                   reset_tuning, set_tuning)
 
 __all__ = [
-    "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "Filter", "PathIntegrator", "SamplerRenderer",
+    "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "Filter", "LDSampler", "PathIntegrator", "SamplerRenderer",
     "StratifiedSampler", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "PinnedArray", "Primitive",
     "RAY_DTYPE", "RayBatch", "RayServer", "RenderServer", "Sphere", "Transform", "TriangleMesh", "device_name", "lib", "reset_tuning", "set_tuning",
     "synchronize",

@@ -42,6 +42,7 @@ class RenderDesc(ctypes.Structure):
         ("integrator", ctypes.c_int32), ("light_samples", ctypes.c_uint32),
         ("max_depth", ctypes.c_uint32), ("rr_start", ctypes.c_uint32),
         ("split", ctypes.c_int32), ("max_prims", ctypes.c_uint32),
+        ("sampler", ctypes.c_int32),
     ]
 
 

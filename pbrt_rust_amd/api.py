@@ -504,6 +504,28 @@ class StratifiedSampler:
         return self.xs * self.ys
 
 
+class LDSampler:
+    """LDSampler::new(x0, x1, y0, y1, samples_per_pixel, ...) over the whole
+    image (sampler/lds.rs:17-31): samples_per_pixel rounded up to a power of
+    two (with the reference's warning), image samples from the scrambled
+    (0,2) sequence shuffled per pixel (sampler/utils.rs:52-69); the
+    scrambles and shuffles draw from the counter-based generator keyed by
+    (seed, pixel)."""
+    kind = 1
+
+    def __init__(self, samples_per_pixel, seed=0):
+        n = int(samples_per_pixel)
+        if n < 1:
+            raise ValueError("samples_per_pixel must be positive")
+        ps = 1 << (n - 1).bit_length()
+        if ps != n:
+            print("Warning -- LDSampler using next power of two (%d) samples per pixel" % ps)  # lds.rs:19-22
+        self.xs, self.ys, self.seed = ps, 1, int(seed)
+
+    def maximum_sample_count(self):
+        return self.xs
+
+
 class DirectLightingIntegrator:
     """Direct lighting: emitted light plus, per light, the mean of
     light_samples light samples (shadow rays) -- pbrt-v2 DirectLighting with
@@ -550,6 +572,7 @@ def _render_desc(scene, camera, sampler, integrator, split="sah"):
         _set_camera(d, camera)
     if sampler is not None:
         d.nx, d.ny, d.seed = sampler.xs, sampler.ys, sampler.seed
+        d.sampler = getattr(sampler, "kind", 0)
     d.integrator = integrator.kind
     d.light_samples = getattr(integrator, "light_samples", 1)
     d.max_depth = getattr(integrator, "max_depth", 5)

@@ -12,6 +12,7 @@ struct CamParams {
     uint32_t xres, nx, spp, seed;
     float dx, dy;            // 1/nx, 1/ny
     uint64_t g0;             // global index of the launch's first sample
+    const float2* spos;      // low-discrepancy positions within the pixel, or null
 };
 
 // ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
@@ -31,8 +32,17 @@ __device__ __forceinline__ f3 xf_vector(const float* m, f3 v) {
 // StratifiedSampler image sample of pixel (px,py), sample s (stratified.rs:60-83,
 // montecarlo.rs:119-128) with the counter-based jitter.  P: any parameter
 // block with seed, xres, nx, dx, dy.
+// A frame sampled by LDSampler instead carries its samples' positions within
+// their pixels (spos, pixel-major): ix = x + the sample's position
+// (CameraSample::new(x_pos as f32 + image_samples[2i], ..), sampler/utils.rs).
 template <class P>
 __device__ __forceinline__ void sample_pos(const P& p, uint32_t px_, uint32_t py, uint32_t s, float* ix, float* iy) {
+    if (p.spos) {
+        const float2 f = p.spos[((size_t)py * p.xres + px_) * p.spp + s];
+        *ix = (float)px_ + f.x;
+        *iy = (float)py + f.y;
+        return;
+    }
     float u[4];
     rng4(p.seed, py * p.xres + px_, s, 0, u);
     uint32_t sx = s % p.nx, sy = s / p.nx;

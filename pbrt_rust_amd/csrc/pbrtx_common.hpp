@@ -151,6 +151,42 @@ PX_HD void rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t block, 
     out[0] = u01(a); out[1] = u01(b); out[2] = u01(c); out[3] = u01(d);
 }
 
+// ---- LDSampler's image samples (lds.rs:47-70; sampler/utils.rs:5-32 and
+// ld_shuffle_scrambled_2d :52-69) --------------------------------------------
+// The bit tricks as the reference has them: van_der_corput's last swap
+// shifts by 2 (bits move or drop), and sobol2 flips on a zero bit.  The
+// reference draws its scrambles and shuffles from a seeded RNG stream
+// (RNG::random_uint, rng.rs:19-21: a u64 reduced mod 2^64 - 1); here draw k
+// of a pixel comes from the counter-based generator instead.
+constexpr uint32_t LD_BLOCK = 0x4C44u;
+PX_HD uint64_t ld_draw(uint32_t seed, uint32_t pixel, uint32_t k) {
+    uint32_t a = pixel, b = k, c = LD_BLOCK, d = seed;
+    pcg4d(a, b, c, d);
+    return (((uint64_t)a << 32) | b) % 0xFFFFFFFFFFFFFFFFull;
+}
+PX_HD float ld_vdc(uint32_t n, uint32_t scramble) {
+    n = (n << 16) | (n >> 16);
+    n = ((n & 0x00ff00ffu) << 8) | ((n & 0xff00ff00u) >> 8);
+    n = ((n & 0x0f0f0f0fu) << 4) | ((n & 0xf0f0f0f0u) >> 4);
+    n = ((n & 0x33333333u) << 2) | ((n & 0xCCCCCCCCu) >> 2);
+    n = ((n & 0x55555555u) << 2) | ((n & 0xAAAAAAAAu) >> 2);
+    n ^= scramble;
+    return (float)((n >> 8) & 0xffffffu) * 0x1p-24f;  // k / 2^24, exact
+}
+PX_HD float ld_sobol2(uint32_t n, uint32_t scramble) {
+    uint32_t s = scramble, v = 1u << 31;
+    for (; n != 0; n >>= 1) {
+        if ((n & 1u) == 0) s ^= v;
+        v ^= v >> 1;
+    }
+    return (float)((s >> 8) & 0xffffffu) * 0x1p-24f;
+}
+PX_HD uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
 static const float PI_F = 3.14159265358979323846f;
 
 // sine / cosine on [-pi/4, pi/4] from their Taylor series through x^9 / x^10

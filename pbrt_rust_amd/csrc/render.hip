@@ -928,20 +928,45 @@ __device__ __forceinline__ int32_t film_offset(int32_t p, float v, float inv_w) 
     if (!(a < (float)(FILTER_TABLE_DIM - 1))) return a == a ? FILTER_TABLE_DIM - 1 : 0;  // `as usize` saturates
     return (int32_t)a;
 }
+// LDSampler image samples of every pixel (ld_shuffle_scrambled_2d,
+// sampler/utils.rs:52-69), one pixel a thread, in place in out (pixel-major):
+// the scrambled (0,2) points, then the pixel's shuffle (RNG::shuffle,
+// rng.rs:23-32).  Draws 0 and 1 give the scrambles and draws 2 .. spp+1 the
+// one-point shuffles that change nothing; the shuffle takes draws spp+2 ..
+__global__ __launch_bounds__(RB) void k_ld_points(uint32_t seed, uint32_t npix, uint32_t spp, float2* out) {
+    for (uint32_t pixel = blockIdx.x * RB + threadIdx.x; pixel < npix; pixel += gridDim.x * RB) {
+        float2* pts = out + (size_t)pixel * spp;
+        const uint32_t sc0 = (uint32_t)ld_draw(seed, pixel, 0), sc1 = (uint32_t)ld_draw(seed, pixel, 1);
+        for (uint32_t i = 0; i < spp; ++i) pts[i] = make_float2(ld_vdc(i, sc0), ld_sobol2(i, sc1));  // sample02
+        for (uint32_t i = 0; i < spp; ++i) {
+            const uint32_t other = i + (uint32_t)(ld_draw(seed, pixel, spp + 2 + i) % (uint64_t)(spp - i));
+            const float2 t = pts[i];
+            pts[i] = pts[other];
+            pts[other] = t;
+        }
+    }
+}
+
 __global__ __launch_bounds__(RB) void k_film_stage(RenderParams P, const float4* sample_l, uint64_t n) {
     const uint64_t npix = (uint64_t)P.xres * P.yres;
     for (uint64_t g = (uint64_t)blockIdx.x * RB + threadIdx.x; g < n; g += (uint64_t)gridDim.x * RB) {
         const uint32_t pixel = (uint32_t)(g / P.spp), s = (uint32_t)(g % P.spp);
-        float u[4];
-        rng4(P.seed, pixel, s, 0, u);  // sample_pos's jitter
-        const uint32_t sx = s % P.nx, sy = s / P.nx;
+        float2 f;
+        if (P.spos) {
+            f = P.spos[g];
+        } else {
+            float u[4];
+            rng4(P.seed, pixel, s, 0, u);  // sample_pos's jitter
+            const uint32_t sx = s % P.nx, sy = s / P.nx;
+            f = make_float2(((float)sx + u[0]) * P.dx, ((float)sy + u[1]) * P.dy);
+        }
         const float4 v = sample_l[g];
         // rgb_to_xyz (spectrum.rs:37-41)
         const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
         const float c1 = 0.212671f * v.x + 0.715160f * v.y + 0.072169f * v.z;
         const float c2 = 0.019334f * v.x + 0.119193f * v.y + 0.950227f * v.z;
         const uint64_t t = (uint64_t)s * npix + pixel;
-        P.fpos[t] = make_float2(((float)sx + u[0]) * P.dx, ((float)sy + u[1]) * P.dy);
+        P.fpos[t] = f;
         P.fxyz[t] = make_float4(c0, c1, c2, 0.0f);
     }
 }
@@ -1280,6 +1305,7 @@ CamParams cam_params(const RenderParams& P, uint64_t g0) {
     memcpy(cp.c2w, P.c2w, sizeof(cp.c2w));
     cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
     cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
+    cp.spos = P.spos;
     return cp;
 }
 
@@ -1432,6 +1458,7 @@ struct Renderer {
     DevArr<uint32_t> counters;
     DevArr<float> rgb;
     DevArr<float> ftab;  // the film filter's weight table (none: box 0.5)
+    DevArr<float2> spos;  // LDSampler: the samples' positions within their pixels
     DevArr<float2> fpos;  // and its staging of the frame's samples
     DevArr<float4> fxyz;
     hipStream_t stream = nullptr;
@@ -1468,6 +1495,24 @@ static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
     }
     R->P = R->sc.base;
     ICHK(camera_setup(&R->P, d->cam_to_world, d->fov_deg, d->screen, d->xres, d->yres, d->nx, d->ny, d->seed));
+    if (d->sampler == 1) {
+        // LDSampler::new (lds.rs:17-24): samples per pixel rounded up to a
+        // power of two; positions made once, as they depend on (seed, pixel)
+        const uint32_t spp = next_pow2(d->nx * d->ny);
+        if ((uint64_t)d->xres * d->yres * spp >= (1ull << 40)) return fail(PBRTX_ERR_ARG, "image too large");
+        R->P.nx = spp; R->P.ny = 1; R->P.spp = spp;
+        R->P.dx = 1.0f / (float)spp; R->P.dy = 1.0f;
+        const uint32_t npix = d->xres * d->yres;
+        ICHK(R->spos.ensure((size_t)npix * spp));
+        const uint32_t blocks = std::min<uint32_t>((npix + RB - 1) / RB, 65536u);
+        hipLaunchKernelGGL(k_ld_points, dim3(blocks), dim3(RB), 0, R->stream, d->seed, npix, spp, R->spos.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(R->stream);
+        if (e != hipSuccess) return hip_fail(e, "LDSampler positions");
+        R->P.spos = R->spos.p;
+    } else if (d->sampler != 0) {
+        return fail(PBRTX_ERR_ARG, "unknown sampler (0 stratified, 1 low-discrepancy)");
+    }
     R->n_samples = (uint64_t)d->xres * d->yres * R->P.spp;
     return PBRTX_OK;
 }
@@ -1771,7 +1816,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.drop_graphs();
     R.sc.release();
     R.W.release();
-    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release();
+    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release(); R.spos.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;

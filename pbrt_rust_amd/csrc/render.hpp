@@ -72,6 +72,9 @@ struct RenderParams {
     // position within its pixel and its xyz
     float2* fpos;
     float4* fxyz;
+    // LDSampler frames: each sample's position within its pixel (pixel-major);
+    // null for the stratified sampler
+    const float2* spos;
 };
 
 // Filter (filter.rs:4-124) of a film: kind 0 mean, 1 triangle, 2 gaussian

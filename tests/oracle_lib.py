@@ -39,6 +39,7 @@ class RenderDesc(ctypes.Structure):
         ("split", ctypes.c_int32), ("max_prims", ctypes.c_uint32),
         ("filter_kind", ctypes.c_int32), ("filter_xw", ctypes.c_float), ("filter_yw", ctypes.c_float),
         ("filter_p0", ctypes.c_float), ("filter_p1", ctypes.c_float),
+        ("sampler_kind", ctypes.c_int32),
     ]
 
 # film filters as (kind, x width, y width, p0, p1), filter.rs:44-86
@@ -77,6 +78,13 @@ def lib():
                                   ctypes.c_float, ctypes.c_float, fp]
     L.orc_filter_table.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    F32P]
+    L.orc_ld_draw.argtypes = [ctypes.c_uint32] * 3
+    L.orc_ld_draw.restype = ctypes.c_uint64
+    L.orc_ld_vdc.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.orc_ld_vdc.restype = ctypes.c_float
+    L.orc_ld_sobol2.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.orc_ld_sobol2.restype = ctypes.c_float
+    L.orc_ld_pixel_points.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, F32P]
     L.orc_get_crop_window.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, F32P]
     L.orc_compute_sub_window.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_uint32, ctypes.c_uint32, I32P]
     L.orc_matrix_inverse.argtypes = [F32P, F32P]
@@ -246,11 +254,14 @@ def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False, fil
     d = make_render_desc(scene, cam, integ, keep)
     if filt is not None:
         d.filter_kind, d.filter_xw, d.filter_yw, d.filter_p0, d.filter_p1 = filt
+    d.sampler_kind = integ.get("sampler", 0)
     if y1 is None:
         y1 = cam["yres"]
     out = np.zeros((y1 - y0) * cam["xres"] * 3, np.float32)
     st = np.zeros(9, np.uint64)
     spp = integ["nx"] * integ["ny"]
+    if integ.get("sampler", 0) == 1:
+        spp = 1 << (spp - 1).bit_length()  # LDSampler rounds up to a power of two
     samples = np.zeros((y1 - y0) * cam["xres"] * spp * 3, np.float32) if want_samples else None
     r = L.orc_render(ctypes.byref(d), y0, y1, nthreads, out,
                      samples.ctypes.data if want_samples else None, st)

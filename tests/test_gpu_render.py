@@ -254,3 +254,37 @@ def test_film_filters_match_oracle(f, integ):
         assert np.array_equal(img.view(np.uint32), box["rgb"].view(np.uint32))
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("integ", [dict(DIRECT, sampler=1, nx=3, ny=2), dict(PATH, sampler=1, nx=4, ny=4)],
+                         ids=["direct-6to8spp", "path-16spp"])
+def test_low_discrepancy_sampler_matches_oracle(integ):
+    # LDSampler: spp rounded up to a power of two, per-pixel scrambled (0,2)
+    # points shuffled; every sample's radiance and the film equal the
+    # oracle's on both paths, and with a filter
+    if integ["kind"] == "path":
+        scene = scenes.spheres_scene(n_spheres=8, level=3, seed=5)
+        cam = scenes.pinhole(40, 28, fov=60.0, cam_pos=(0.0, 0.0, -25.0))
+        it = px.PathIntegrator(integ["max_depth"], integ["rr_start"])
+    else:
+        scene = scenes.direct_lighting_scene(level=4)
+        cam = scenes.pinhole(37, 29)
+        it = px.DirectLightingIntegrator(integ["light_samples"])
+    ref = oracle_render(scene, cam, integ)
+    spp = ref["samples"].shape[2]
+    sampler = px.LDSampler(integ["nx"] * integ["ny"], integ["seed"])
+    assert sampler.maximum_sample_count() == spp
+    r = px.SamplerRenderer(scene, cam, sampler, it)
+    try:
+        for path in ("plain", "batched"):
+            img, st = r.render(path)
+            samples = r.sample_radiance().reshape(cam["yres"], cam["xres"], spp, 3)
+            assert_same(ref, img, samples)
+            assert st["camera_rays"] == cam["xres"] * cam["yres"] * spp
+        f = px.Filter.gaussian(2.0, 2.0, 2.0)
+        reff = ol.render(scene, cam, integ, nthreads=16, filt=f.args())
+        r.set_filter(f)
+        img, _ = r.render("batched")
+        assert np.array_equal(img.view(np.uint32), reff["rgb"].view(np.uint32))
+    finally:
+        r.close()
