@@ -288,3 +288,24 @@ def test_low_discrepancy_sampler_matches_oracle(integ):
         assert np.array_equal(img.view(np.uint32), reff["rgb"].view(np.uint32))
     finally:
         r.close()
+
+
+def test_scene_file_renders_as_the_oracle():
+    # a scene file through the scene-description API: the renderer it makes
+    # gives the oracle's image of the same scene, camera, sampler and filter
+    import test_pbrt_api as tp
+    from pbrt_rust_amd import pbrt_api as pa
+
+    class Keep(pa.Pbrt):
+        pass
+
+    p = Keep()
+    r = pa.parse(tp.SCENE, p)
+    try:
+        img, st = r.render("batched")
+        integ = dict(kind="direct", nx=2, ny=2, seed=0, light_samples=2)
+        ref = ol.render(p.scene(), p.camera_dict(), integ, nthreads=16, filt=p.filter_obj().args())
+        assert np.array_equal(img.view(np.uint32), ref["rgb"].view(np.uint32))
+        assert st["shadow_rays"] == ref["stats"][1] and img.mean() > 0.0
+    finally:
+        r.close()
