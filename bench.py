@@ -173,10 +173,6 @@ def main():
     mrays_p = rays / (ms_p * 1e-3) / 1e6
     log(args, "batched %.3f ms/frame (wall %.3f)  plain %.3f ms/frame" % (ms_b, wall_b * 1e3, ms_p))
 
-    # ---- CPU baseline (bounded sample of the same frame); it also yields the
-    # reference traversal's per-ray work used for the algorithmic figures ----
-    cpu, work = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
-    log(args, "cpu baseline %s" % json.dumps(cpu))
     serve = None
     if not args.no_serve:
         serve = serving(args, px, scenes, lambda m: log(args, m))
@@ -186,6 +182,13 @@ def main():
     render_serve = None
     if not args.no_render_serve:
         render_serve = render_serving(args, px, scenes, lambda m: log(args, m))
+
+    # ---- CPU baseline (bounded sample of the same frame, after the GPU
+    # sections so that its all-core run does not slow their host side); it
+    # also yields the reference traversal's per-ray work used for the
+    # algorithmic figures ----
+    cpu, work = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
+    log(args, "cpu baseline %s" % json.dumps(cpu))
 
     peaks = measured_peaks()
     log(args, "measured ceilings: %s" % peaks)
