@@ -58,6 +58,24 @@ __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, flo
     return !(t0 > t1);
 }
 
+// The same interval with each axis's planes given near-first by the sign of
+// 1/d.  For 1/d >= 0 (or NaN) the reference's compare-and-swap keeps lo, hi
+// in that order for any ray.  For 1/d < 0 it swaps them exactly when no slab
+// distance is NaN: with a finite 1/d, a finite origin and finite planes, (b -
+// o) * id is monotone in b and the swap result equals hi, lo up to the sign
+// of a zero distance, which no comparison here sees.  Callers take this form
+// only for such rays (cq_fast_ray in a scene passing cq_fast_scene).
+__device__ __forceinline__ bool slab_ord(float nx, float fx, float ny, float fy, float nz, float fz,
+                                         float ox, float oy, float oz, float ix, float iy, float iz,
+                                         float tmin, float tmax) {
+    const float ax = (nx - ox) * ix, bx = (fx - ox) * ix;
+    const float ay = (ny - oy) * iy, by = (fy - oy) * iy;
+    const float az = (nz - oz) * iz, bz = (fz - oz) * iz;
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, ax), ay), az);
+    const float t1 = fminf(fminf(fminf(tmax, bx), by), bz);
+    return !(t0 > t1);
+}
+
 struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz, tmin, tmax;
 };
@@ -870,6 +888,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[PWAVES][PSTK];
     __shared__ unsigned long long s_msk[PWAVES][PSTK];
+    const bool fast_scene = cq_fast_scene(sv);  // planes within 2^40: slab_ord may apply
     const uint32_t lane = __lane_id();
     const uint32_t w = threadIdx.x >> 6;
     // GEN (camera rays, their count known at launch): a fixed schedule with
@@ -917,6 +936,8 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
         uint32_t hprim = NO_HIT;
         float hb1 = 0.0f, hb2 = 0.0f;
         const unsigned long long alive_m = __ballot(alive);
+        // lanes that may not take the sign-ordered box test (slab_ord)
+        const unsigned long long slow_m = __ballot(alive && !cq_fast_ray(r));
         unsigned long long todo = __ballot(valid);
         while (todo) {
             // the next group: lanes with the signs of the first remaining lane
@@ -926,6 +947,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
             todo &= ~grp;
             const unsigned long long live = grp & alive_m;
             if (!live) continue;
+            const bool gfast = fast_scene && (live & slow_m) == 0ull;
             int sp = 0;
             int32_t cur = sv.qroot;
             unsigned long long cm = live;  // lanes that reached the current entry
@@ -938,15 +960,31 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                     const uint32_t meta = as_u(cload4(q, 7).x);
                     int32_t r0 = (int32_t)as_u(R6.x), r1 = (int32_t)as_u(R6.y), r2 = (int32_t)as_u(R6.z),
                             r3 = (int32_t)as_u(R6.w);
-                    float t;
-                    bool p0 = act && r0 != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
-                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                    bool p1 = act && r1 != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
-                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                    bool p2 = act && r2 != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
-                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                    bool p3 = act && r3 != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
-                                                          r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    bool p0, p1, p2, p3;
+                    if (gfast) {
+                        // planes ordered by the group's signs (wave-uniform selects)
+                        const float4 NX = (g & 1u) ? HX : LX, FX = (g & 1u) ? LX : HX;
+                        const float4 NY = (g & 2u) ? HY : LY, FY = (g & 2u) ? LY : HY;
+                        const float4 NZ = (g & 4u) ? HZ : LZ, FZ = (g & 4u) ? LZ : HZ;
+                        p0 = act && r0 != QEMPTY && slab_ord(NX.x, FX.x, NY.x, FY.x, NZ.x, FZ.x, r.ox, r.oy, r.oz,
+                                                             r.ix, r.iy, r.iz, r.tmin, r.tmax);
+                        p1 = act && r1 != QEMPTY && slab_ord(NX.y, FX.y, NY.y, FY.y, NZ.y, FZ.y, r.ox, r.oy, r.oz,
+                                                             r.ix, r.iy, r.iz, r.tmin, r.tmax);
+                        p2 = act && r2 != QEMPTY && slab_ord(NX.z, FX.z, NY.z, FY.z, NZ.z, FZ.z, r.ox, r.oy, r.oz,
+                                                             r.ix, r.iy, r.iz, r.tmin, r.tmax);
+                        p3 = act && r3 != QEMPTY && slab_ord(NX.w, FX.w, NY.w, FY.w, NZ.w, FZ.w, r.ox, r.oy, r.oz,
+                                                             r.ix, r.iy, r.iz, r.tmin, r.tmax);
+                    } else {
+                        float t;
+                        p0 = act && r0 != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy, r.oz,
+                                                         r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                        p1 = act && r1 != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy, r.oz,
+                                                         r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                        p2 = act && r2 != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy, r.oz,
+                                                         r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                        p3 = act && r3 != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy, r.oz,
+                                                         r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
+                    }
                     unsigned long long m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2), m3 = __ballot(p3);
                     // the group's visiting order: near half first, near child first
                     const bool nA = (g >> (meta & 3u)) & 1u;
@@ -990,9 +1028,16 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                     uint32_t pos = (uint32_t)(~cur);
                     const PrimRec64* pr = sv.prims64 + pos;
                     float4 c = cload4(pr, 2), d = cload4(pr, 3);
-                    float tn;
-                    const bool go = act && slab(c.y, d.x, c.z, d.y, c.w, d.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz,
-                                                r.tmin, r.tmax, &tn);
+                    bool go;
+                    if (gfast) {
+                        go = act && slab_ord((g & 1u) ? d.x : c.y, (g & 1u) ? c.y : d.x, (g & 2u) ? d.y : c.z,
+                                             (g & 2u) ? c.z : d.y, (g & 4u) ? d.z : c.w, (g & 4u) ? c.w : d.z,
+                                             r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax);
+                    } else {
+                        float tn;
+                        go = act && slab(c.y, d.x, c.z, d.y, c.w, d.z, r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin,
+                                         r.tmax, &tn);
+                    }
                     if (__ballot(go)) {
                         while (true) {
                             const float4 a = cload4(pr, 0), b = cload4(pr, 1);
