@@ -81,6 +81,14 @@ int pbrtx_bvh_export(const pbrtx_bvh* bvh, float* bounds, uint32_t* off, uint32_
    layout's 64-byte prim records (any pointer may be NULL). */
 int pbrtx_bvh_layout_info(const pbrtx_bvh* bvh, uint32_t* n_wide, uint32_t* n_compressed, uint32_t* n_prims);
 int pbrtx_bvh_export_layouts(const pbrtx_bvh* bvh, void* wide, void* compressed, void* prims64);
+/* The compressed occlusion records (64 B each, root first; leaves referenced
+   as in the compressed layout) that any-hit queries traverse: *n receives
+   their count (0 when the scene has none and any-hit queries use the
+   compressed layout) and *depth their record levels; records may be NULL.
+   No counterpart in the reference, whose any-hit query is the closest-hit
+   walk of the same binary tree (intersection.rs:62-64, bvh.rs:377-421); for
+   inspection and tests. */
+int pbrtx_bvh_occlusion_records(const pbrtx_bvh* bvh, uint32_t* n, uint32_t* depth, void* records);
 int pbrtx_bvh_free(pbrtx_bvh* bvh);
 
 /* ---- device scene -------------------------------------------------------- */

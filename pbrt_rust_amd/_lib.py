@@ -123,6 +123,7 @@ def lib():
     L.pbrtx_bvh_free.argtypes = [vp]
     L.pbrtx_bvh_layout_info.argtypes = [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u32)]
     L.pbrtx_bvh_export_layouts.argtypes = [vp, vp, vp, vp]
+    L.pbrtx_bvh_occlusion_records.argtypes = [vp, c.POINTER(u32), c.POINTER(u32), vp]
     L.pbrtx_scene_create.argtypes = [vp, c.POINTER(vp)]
     L.pbrtx_scene_destroy.argtypes = [vp]
     L.pbrtx_scene_device_bytes.argtypes = [vp, c.POINTER(u64)]

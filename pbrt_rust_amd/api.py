@@ -239,6 +239,16 @@ class BVHAccelerator:
                                              p.ctypes.data if p.size else None))
         return q, c, p
 
+    def occlusion_records(self):
+        """The compressed records any-hit queries traverse (CNODE_DTYPE, root
+        first; empty when they use the compressed layout) and their depth."""
+        n, d = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().pbrtx_bvh_occlusion_records(self._bvh, ctypes.byref(n), ctypes.byref(d), None))
+        r = np.zeros(n.value, CNODE_DTYPE)
+        if n.value:
+            check(lib().pbrtx_bvh_occlusion_records(self._bvh, ctypes.byref(n), ctypes.byref(d), r.ctypes.data))
+        return r, d.value
+
     def world_bound(self):
         b = self.nodes()[0]
         return b[0, :3].copy(), b[0, 3:].copy()

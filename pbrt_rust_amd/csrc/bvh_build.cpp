@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -209,13 +211,15 @@ bool cq_axis(float ulo, float uhi, const float* lo, const float* hi, const bool*
 // Builds the compressed layout from the four-wide records and the leaves of
 // the reference tree.  Returns false if some record cannot be represented
 // (non-finite bounds); the exact layout is used then.
-bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
-    out->cnodes.assign(out->qnodes.size(), CNode{});
+// Quantize four-wide records into compressed ones; false when some
+// record's bounds are not representable.
+static bool compress_records(const std::vector<QNode>& qn, std::vector<CNode>* cn) {
+    cn->assign(qn.size(), CNode{});
     std::atomic<bool> ok{true};
-    parallel_for(out->qnodes.size(), [&](size_t i0, size_t i1) {
+    parallel_for(qn.size(), [&](size_t i0, size_t i1) {
     for (size_t i = i0; i < i1 && ok.load(std::memory_order_relaxed); ++i) {
-        const QNode& q = out->qnodes[i];
-        CNode& c = out->cnodes[i];
+        const QNode& q = qn[i];
+        CNode& c = (*cn)[i];
         bool used[4];
         float u[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
         for (int k = 0; k < 4; ++k) {
@@ -236,7 +240,11 @@ bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
         c.emeta = ex | (ey << 8) | (ez << 16) | ((q.meta & 63u) << 24);
     }
     });
-    if (!ok.load()) return false;
+    return ok.load();
+}
+
+bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
+    if (!compress_records(out->qnodes, &out->cnodes)) return false;
     // prim records with their leaf's exact box
     out->prims64.assign(out->prims.size(), PrimRec64{});
     parallel_for(nodes.size(), [&](size_t i0, size_t i1) {
@@ -256,6 +264,126 @@ bool compress_layout(const std::vector<RNode>& nodes, HostBvh* out) {
     }
     });
     return true;
+}
+
+// Occlusion records: a second four-wide record tree over the same leaves,
+// for queries whose answer does not depend on the order in which leaves are
+// met.  The ordered records take a node's grandchildren, so a node with two
+// leaf children makes a record of two slots; here each record takes the cut
+// of its binary subtree (at most four nodes, each a leaf or the root of a
+// further record) that minimizes the summed surface area of all records
+// below it -- the expected number of record tests for rays spread evenly
+// over position and direction -- plus a small charge per record that breaks
+// ties towards fewer records.  Dynamic programming over the binary tree:
+// D[i][k-1] is the least cost of covering node i's subtree with at most k
+// slots, either i whole (a leaf, or a record of its own) or split between
+// its two children.  Leaves cost nothing (every cut holds each leaf once).
+// Empty when the tree's root is a leaf or a record is not representable.
+static void occlusion_records(const std::vector<RNode>& nodes, HostBvh* out) {
+    out->ocnodes.clear();
+    out->odepth = 0;
+    const size_t N = nodes.size();
+    auto is_leaf = [&](size_t i) { return (nodes[i].meta & 3u) == 3u; };
+    if (N == 0 || is_leaf(0)) return;
+    auto area = [&](size_t i) {
+        const double dx = (double)nodes[i].hi[0] - nodes[i].lo[0], dy = (double)nodes[i].hi[1] - nodes[i].lo[1],
+                     dz = (double)nodes[i].hi[2] - nodes[i].lo[2];
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    };
+    const double ra = area(0);
+    const double inv = ra > 0.0 && std::isfinite(ra) ? 1.0 / ra : 0.0;
+    const double REC = 1e-6;  // per-record charge (relative to the root's area)
+    std::vector<std::array<float, 4>> D(N);
+    // choice bits: [2k-4, 2k-2) for k = 2..4 the left share j of a split into
+    // at most k slots (0: whole); [6, 8) the left share of the record split
+    std::vector<uint8_t> ch(N, 0);
+    // children follow their parent in the depth-first layout: reverse index
+    // order meets every child before its parent
+    for (size_t ii = N; ii-- > 0;) {
+        if (is_leaf(ii)) { D[ii] = {0.0f, 0.0f, 0.0f, 0.0f}; continue; }
+        const size_t b = ii + 1, c = nodes[ii].off;
+        float S[5];
+        uint32_t js[5] = {0, 0, 0, 0, 0};
+        for (int k = 2; k <= 4; ++k) {
+            S[k] = INFINITY;
+            for (int j = 1; j < k; ++j) {
+                const float v = D[b][j - 1] + D[c][k - j - 1];
+                if (v < S[k]) { S[k] = v; js[k] = (uint32_t)j; }
+            }
+        }
+        double a = area(ii) * inv;
+        if (!std::isfinite(a)) a = 1.0;
+        const float R = (float)(a + REC) + S[4];
+        uint8_t bits = (uint8_t)(js[4] << 6);
+        D[ii][0] = R;
+        for (int k = 2; k <= 4; ++k) {
+            if (S[k] < R) { D[ii][k - 1] = S[k]; bits |= (uint8_t)(js[k] << (2 * k - 4)); }
+            else D[ii][k - 1] = R;
+        }
+        ch[ii] = bits;
+    }
+    std::vector<std::array<float, 4>>().swap(D);
+    // the slots of the record rooted at interior node a
+    auto slots_of = [&](size_t a, size_t s[4]) {
+        int n = 0;
+        struct Item { size_t i; uint32_t k; };
+        Item st[8];
+        int sp = 0;
+        const uint32_t j = (ch[a] >> 6) & 3u;
+        st[sp++] = {nodes[a].off, 4u - j};
+        st[sp++] = {a + 1, j};
+        while (sp > 0) {
+            const Item it = st[--sp];
+            const uint32_t jj = (it.k >= 2 && !is_leaf(it.i)) ? (ch[it.i] >> (2 * it.k - 4)) & 3u : 0u;
+            if (jj == 0) { s[n++] = it.i; continue; }
+            st[sp++] = {nodes[it.i].off, it.k - jj};
+            st[sp++] = {it.i + 1, jj};
+        }
+        for (int k = n; k < 4; ++k) s[k] = (size_t)-1;
+    };
+    // records in depth-first pre-order of the record tree
+    std::vector<int32_t> rec(N, -1);
+    std::vector<size_t> roots;
+    {
+        std::vector<std::pair<size_t, uint32_t>> stk;
+        stk.push_back({0, 1});
+        while (!stk.empty()) {
+            auto [a, dq] = stk.back();
+            stk.pop_back();
+            rec[a] = (int32_t)roots.size();
+            roots.push_back(a);
+            out->odepth = std::max(out->odepth, dq);
+            size_t s[4];
+            slots_of(a, s);
+            for (int k = 3; k >= 0; --k)
+                if (s[k] != (size_t)-1 && !is_leaf(s[k])) stk.push_back({s[k], dq + 1});
+        }
+    }
+    std::vector<QNode> oq(roots.size(), QNode{});
+    parallel_for(roots.size(), [&](size_t r0, size_t r1) {
+    for (size_t r = r0; r < r1; ++r) {
+        QNode& q = oq[r];
+        size_t s[4];
+        slots_of(roots[r], s);
+        q.meta = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (s[k] == (size_t)-1) {
+                q.ref[k] = QEMPTY;
+                q.lx[k] = q.ly[k] = q.lz[k] = FLT_MAX;
+                q.hx[k] = q.hy[k] = q.hz[k] = -FLT_MAX;
+                continue;
+            }
+            const RNode& c = nodes[s[k]];
+            q.ref[k] = is_leaf(s[k]) ? ~(int32_t)c.off : rec[s[k]];
+            q.lx[k] = c.lo[0]; q.ly[k] = c.lo[1]; q.lz[k] = c.lo[2];
+            q.hx[k] = c.hi[0]; q.hy[k] = c.hi[1]; q.hz[k] = c.hi[2];
+        }
+    }
+    });
+    if (!compress_records(oq, &out->ocnodes)) {
+        out->ocnodes.clear();
+        out->odepth = 0;
+    }
 }
 
 namespace {
@@ -776,6 +904,8 @@ bool build_bvh(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_
     if (!compress_layout(nodes, out)) {
         out->cnodes.clear();
         out->prims64.clear();
+    } else {
+        occlusion_records(nodes, out);
     }
     auto t1 = std::chrono::steady_clock::now();
     out->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

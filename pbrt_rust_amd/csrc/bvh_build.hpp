@@ -25,6 +25,8 @@ struct HostBvh {
     std::vector<QNode> qnodes;      // four-wide layout
     std::vector<CNode> cnodes;      // compressed four-wide layout (empty if not representable)
     std::vector<PrimRec64> prims64; // prim records of the compressed layout
+    std::vector<CNode> ocnodes;     // compressed occlusion records, root at 0 (empty: use cnodes)
+    uint32_t odepth = 0;            // record levels of the occlusion records
     int32_t qroot = 0;              // root ref of the four-wide layout
     uint32_t qdepth = 0;            // record levels of the four-wide layout
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};

@@ -112,10 +112,11 @@ int scene_from_host(const HostBvh& h, DeviceScene* s) {
     if ((rc = upload(&s->d_cnodes, h.cnodes.data(), h.cnodes.size() * sizeof(CNode)))) return rc;
     if ((rc = upload(&s->d_prims64, h.prims64.data(), h.prims64.size() * sizeof(PrimRec64)))) return rc;
     if ((rc = upload(&s->d_order, h.order.data(), h.order.size() * sizeof(uint32_t)))) return rc;
+    if ((rc = upload(&s->d_ocnodes, h.ocnodes.data(), h.ocnodes.size() * sizeof(CNode)))) return rc;
     s->bytes = h.rnodes.size() * sizeof(RNode) + h.qnodes.size() * sizeof(QNode) +
                h.prims.size() * sizeof(PrimRec) + h.spheres.size() * sizeof(SphereRec) +
                h.cnodes.size() * sizeof(CNode) + h.prims64.size() * sizeof(PrimRec64) +
-               h.order.size() * sizeof(uint32_t);
+               h.order.size() * sizeof(uint32_t) + h.ocnodes.size() * sizeof(CNode);
     SceneView& v = s->view;
     v.rnodes = (const RNode*)s->d_rnodes;
     v.qnodes = (const QNode*)s->d_qnodes;
@@ -126,14 +127,18 @@ int scene_from_host(const HostBvh& h, DeviceScene* s) {
     const bool cq = !h.cnodes.empty() && !h.prims64.empty();
     v.cnodes = cq ? (const CNode*)s->d_cnodes : nullptr;
     v.prims64 = cq ? (const PrimRec64*)s->d_prims64 : nullptr;
+    const bool oc = cq && !h.ocnodes.empty();
+    v.ocnodes = oc ? (const CNode*)s->d_ocnodes : v.cnodes;
     v.order = (const uint32_t*)s->d_order;
     v.n_rnodes = (uint32_t)h.rnodes.size();
     v.n_qnodes = (uint32_t)h.qnodes.size();
     v.n_prims = (uint32_t)h.prims.size();
     v.n_spheres = (uint32_t)h.spheres.size();
     v.qroot = h.qroot;
+    v.ocroot = oc ? 0 : h.qroot;
     v.depth = h.depth;
     v.qdepth = h.qdepth;
+    v.odepth = oc ? h.odepth : h.qdepth;
     for (int k = 0; k < 3; ++k) { v.root_lo[k] = h.root_lo[k]; v.root_hi[k] = h.root_hi[k]; }
     return PBRTX_OK;
 }
@@ -146,6 +151,7 @@ void scene_free(DeviceScene* s) {
     if (s->d_cnodes) (void)hipFree(s->d_cnodes);
     if (s->d_prims64) (void)hipFree(s->d_prims64);
     if (s->d_order) (void)hipFree(s->d_order);
+    if (s->d_ocnodes) (void)hipFree(s->d_ocnodes);
     memset(s, 0, sizeof(*s));
 }
 
@@ -232,6 +238,15 @@ int pbrtx_bvh_layout_info(const pbrtx_bvh* b, uint32_t* n_wide, uint32_t* n_comp
     if (n_wide) *n_wide = (uint32_t)b->h.qnodes.size();
     if (n_compressed) *n_compressed = (uint32_t)b->h.cnodes.size();
     if (n_prims) *n_prims = (uint32_t)b->h.prims.size();
+    return PBRTX_OK;
+}
+
+int pbrtx_bvh_occlusion_records(const pbrtx_bvh* b, uint32_t* n, uint32_t* depth, void* records) {
+    if (!b) return fail(PBRTX_ERR_ARG, "bvh is null");
+    const HostBvh& h = b->h;
+    if (n) *n = (uint32_t)h.ocnodes.size();
+    if (depth) *depth = h.odepth;
+    if (records) memcpy(records, h.ocnodes.data(), h.ocnodes.size() * sizeof(CNode));
     return PBRTX_OK;
 }
 

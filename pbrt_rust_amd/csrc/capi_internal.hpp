@@ -11,7 +11,7 @@ namespace px {
 
 struct DeviceScene {
     SceneView view;
-    void *d_rnodes, *d_qnodes, *d_prims, *d_spheres, *d_cnodes, *d_prims64, *d_order;
+    void *d_rnodes, *d_qnodes, *d_prims, *d_spheres, *d_cnodes, *d_prims64, *d_order, *d_ocnodes;
     uint64_t bytes;
 };
 

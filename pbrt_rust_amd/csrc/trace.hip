@@ -393,6 +393,8 @@ __device__ __forceinline__ bool prim_test64(const SceneView& sv, uint32_t pos, c
     return hit;
 }
 
+constexpr uint32_t LEAF_WAIT_PCT = 25;
+
 // SEG: the rays come from a segment table (served launch) instead of one array.
 // CQ: the compressed records (CNode, PrimRec64) instead of QNode / PrimRec.
 // Camera-ray packets dealt to one XCD at a time (see k_closest_packet).
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                     have_cur = slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1],
                                     sv.root_lo[2], sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz,
                                     r.tmin, r.tmax, &tn);
-                    cur = sv.qroot;
+                    cur = (ANY && CQ) ? sv.ocroot : sv.qroot;
                     has = true;
                 } else {
                     drained = true;
@@ -567,13 +569,24 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                     has = false;
                 }
             }
+            // A lane at a leaf waits while fewer than LEAF_WAIT_PCT % of the
+            // wave's tracing lanes are at one (and some are at records): every
+            // step that some lane spends on a leaf costs the whole wave the
+            // leaf code, so leaves are taken in bunches.  On the 1024x1024
+            // direct-lighting frame, occlusion 3.87 -> 3.45 ms at 25 % (15 %:
+            // 3.52, 35 %: 3.52, 50 %: 3.62); the path-tracing frame gains too.
+            const unsigned long long at_rec = __ballot(has && cur >= 0);
+            const unsigned long long at_leaf = __ballot(has && cur < 0);
+            const bool leaf_now = at_rec == 0ull || (uint32_t)__popcll(at_leaf) * 100u >=
+                                                        LEAF_WAIT_PCT * (uint32_t)__popcll(at_leaf | at_rec);
             if (has && cur >= 0) {
                 int4 RF;
                 uint32_t meta;
                 float t0, t1, t2, t3;
                 bool p0, p1, p2, p3;
                 if constexpr (CQ) {
-                    const float4* q = reinterpret_cast<const float4*>(sv.cnodes + cur);
+                    // occlusion reads the order-free records
+                    const float4* q = reinterpret_cast<const float4*>((ANY ? sv.ocnodes : sv.cnodes) + cur);
                     const float4 W0 = q[0], W1 = q[1], W2 = q[2], W3 = q[3];
                     const uint32_t em = as_u(W0.w);
                     // near / far bytes of all four slots by the ray's signs
@@ -681,7 +694,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                     have_cur = pend;
                     cur = pref;
                 }
-            } else if (has) {
+            } else if (has && leaf_now) {
                 // leaf: prims in order until the record flagged last
                 uint32_t pos = (uint32_t)(~cur);
                 bool hitnow = false;
@@ -1184,8 +1197,9 @@ static uint32_t plain_levels(const SceneView& s) {
     return need > (uint32_t)STK_PLAIN ? need - (uint32_t)STK_PLAIN : 0u;
 }
 static uint32_t wide_levels(const SceneView& s, int stack) {
-    // at most three pending slots per record level on the current path
-    uint32_t need = 3 * s.qdepth + 1;
+    // at most three pending slots per record level on the current path (of
+    // the ordered records or of the occlusion records)
+    uint32_t need = 3 * std::max(s.qdepth, s.odepth) + 1;
     return need > (uint32_t)stack ? need - (uint32_t)stack : 0u;
 }
 

@@ -13,12 +13,15 @@ struct SceneView {
     const PrimRec* prims;
     const SphereRec* spheres;
     const CNode* cnodes;       // compressed layout, null when the scene has none
+    const CNode* ocnodes;      // compressed occlusion records (order-free queries); cnodes when none
     const PrimRec64* prims64;
     const uint32_t* order;     // input prim id of each leaf-order position
     uint32_t n_rnodes, n_qnodes, n_prims, n_spheres;
     int32_t qroot;
+    int32_t ocroot;   // root ref of ocnodes
     uint32_t depth;   // levels of the reference tree
     uint32_t qdepth;  // levels of the four-wide records
+    uint32_t odepth;  // levels of ocnodes
     float root_lo[3], root_hi[3];
 };
 

@@ -168,3 +168,75 @@ def test_compressed_records_contain_exact_boxes(seed):
     v = np.concatenate([scenes.random_triangles(500, seed)[0] * 1e-3, scenes.random_triangles(500, seed + 9)[0] * 1e4])
     idx = np.arange(v.shape[0], dtype=np.uint32).reshape(-1, 3)
     check_compressed(build_product(v.astype(np.float32), idx, "middle", 4))
+
+
+def check_occlusion_records(bvh):
+    """The order-free records of any-hit queries: every binary leaf is the
+    slot of exactly one record, every record but the root the slot of exactly
+    one record, and every slot's quantized box contains the exact box of
+    what it holds (the leaf's box, or the union of the boxes below a
+    record), so the interior test is a conservative filter as in the ordered
+    layout.  Returns (records, ordered records)."""
+    q, c, p = bvh.layouts()
+    r, depth = bvh.occlusion_records()
+    if c.size == 0:
+        assert r.size == 0
+        return r, q
+    assert r.size > 0
+    b, off, meta, leaf, order = bvh.nodes()
+    empty = -2 ** 31
+    refs = r["ref"]
+    used = refs != empty
+    inner = used & (refs >= 0)
+    # each record below the root is referenced once; each leaf once
+    kids = np.sort(refs[inner])
+    assert np.array_equal(kids, np.arange(1, r.size))
+    leaf_offs = np.sort((~refs[used & (refs < 0)]).astype(np.int64))
+    assert np.array_equal(leaf_offs, np.sort(off[leaf.astype(bool)].astype(np.int64)))
+    # exact boxes of records, children first (a record's slots come after it)
+    d = decode_compressed(r)
+    lo = np.full((r.size, 3), np.inf)
+    hi = np.full((r.size, 3), -np.inf)
+    lvl = np.zeros(r.size, np.int64)
+    lvl[0] = 1
+    for i in range(r.size):
+        for k in range(4):
+            if inner[i, k]:
+                assert refs[i, k] > i
+                lvl[refs[i, k]] = lvl[i] + 1
+    assert lvl.max() == depth
+    for i in range(r.size - 1, -1, -1):
+        for k in range(4):
+            if not used[i, k]:
+                continue
+            if refs[i, k] >= 0:
+                elo, ehi = lo[refs[i, k]], hi[refs[i, k]]
+            else:
+                e = p[~refs[i, k], 9:15]
+                elo, ehi = e[:3], e[3:]
+            for a, ax in enumerate("xyz"):
+                assert d["l" + ax][i, k] <= elo[a] and d["h" + ax][i, k] >= ehi[a], (i, k, ax)
+            lo[i] = np.minimum(lo[i], elo)
+            hi[i] = np.maximum(hi[i], ehi)
+    # the root covers the whole tree
+    assert np.array_equal(lo[0].astype(np.float32), b[0, :3]) and np.array_equal(hi[0].astype(np.float32), b[0, 3:])
+    return r, q
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_occlusion_records_cover_the_tree(seed):
+    v, idx = scenes.random_triangles(3000, seed)
+    check_occlusion_records(build_product(v, idx, "sah", 1))
+    # on a tessellated surface the records are fuller than the ordered ones
+    # (each of those spans two levels), and fewer
+    sc = scenes.direct_lighting_scene(level=4, seed=seed)
+    r, q = check_occlusion_records(build_product(sc["verts"], sc["idx"], "sah", 1))
+    used_o = (r["ref"] != -2 ** 31).sum(1).mean()
+    used_q = (q["ref"] != -2 ** 31).sum(1).mean()
+    assert r.size < 0.9 * q.size and used_o > used_q + 0.3, (r.size, q.size, used_o, used_q)
+    v = np.concatenate([scenes.random_triangles(300, seed)[0] * 1e-3, scenes.random_triangles(300, seed + 9)[0] * 1e4])
+    idx = np.arange(v.shape[0], dtype=np.uint32).reshape(-1, 3)
+    check_occlusion_records(build_product(v.astype(np.float32), idx, "middle", 4))
+    # a root that is a leaf has no records
+    v, idx = scenes.random_triangles(1, seed)
+    assert build_product(v, idx, "sah", 1).occlusion_records()[0].size == 0

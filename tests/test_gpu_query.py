@@ -345,6 +345,9 @@ def test_packet_closest_scheme_matches_oracle(packet_closest):
     test_random_scene_parity("batched", "middle")
     test_edge_cases("batched")
     test_mixed_finite_and_infinite_inverse_directions("batched")
+    # groups with every ray inside the 2^40 limits take the sign-ordered box
+    # test, groups with one ray outside them the reference's swap
+    test_grazing_rays_on_box_planes("batched")
     test_mixed_triangles_and_spheres("batched")
     test_eight_spheres_prim_ids("batched", "equal")
     test_tetrahedron_maxt_side_effect("batched")
