@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32
+PEAK_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+SIMDS = 1024  # 256 CUs x 4 SIMDs
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # workload, scaled by a device-copy calibration (written next to its summary)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_direct1024.json")
@@ -65,19 +67,27 @@ def parse():
     ap.add_argument("--rs-batch", type=int, default=1 << 23, help="paths per render-serving batch")
     ap.add_argument("--rs-arena-gb", type=int, default=40)
     ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="only run one device copy of this many bytes (counter calibration) and exit")
+                    help="only run one device copy and one 64-byte gather over this many bytes (a power of two, "
+                         "counter calibration) and exit")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
 
 def copy_calibration(nbytes):
+    """one device copy of nbytes and one 64-byte gather over nbytes (2^k x
+    64), for the counter calibration passes"""
     import pbrt_rust_amd as px
     from pbrt_rust_amd._lib import check
+    L = px.lib()
     a = px.DeviceBuffer(nbytes)
     b = px.DeviceBuffer(nbytes)
-    check(px.lib().pbrtx_device_copy(b.ptr, a.ptr, nbytes, None))
+    check(L.pbrtx_device_copy(b.ptr, a.ptr, nbytes, None))
     px.synchronize()
-    print(json.dumps({"copy_bytes": nbytes}))
+    nt = int(L.pbrtx_gather_probe_threads())
+    out = px.DeviceBuffer(nt * 4)
+    check(L.pbrtx_gather_probe(a.ptr, nbytes, out.ptr, nt, None))
+    px.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "gather_bytes": nbytes}))
 
 
 def measured_peaks():
@@ -106,6 +116,14 @@ def measured_peaks():
     copy_ms = timed(lambda: check(L.pbrtx_device_copy(b.ptr, a.ptr, nbytes, None)), 5)
     a.free()
     b.free()
+    # scattered 64-byte reads of a 2 GiB buffer, each record once (the
+    # traversal's request size): the achievable rate of such reads
+    g = px.DeviceBuffer(nbytes)
+    gout = px.DeviceBuffer(int(L.pbrtx_gather_probe_threads()) * 4)
+    gather_ms = timed(lambda: check(L.pbrtx_gather_probe(g.ptr, nbytes, gout.ptr, int(L.pbrtx_gather_probe_threads()),
+                                                         None)), 5)
+    g.free()
+    gout.free()
     threads = int(L.pbrtx_fma_probe_threads())
     out = px.DeviceBuffer(threads * 4)
     flops = ctypes.c_uint64()
@@ -113,8 +131,10 @@ def measured_peaks():
     fma_ms = timed(lambda: check(L.pbrtx_fma_probe(out.ptr, threads, iters, ctypes.byref(flops), None)), 3)
     out.free()
     return {"hbm_copy_gbs": round(2 * nbytes / (copy_ms * 1e-3) / 1e9, 1),
+            "gather64_gbs": round(nbytes / (gather_ms * 1e-3) / 1e9, 1),
             "fp32_fma_tflops": round(flops.value / (fma_ms * 1e-3) / 1e12, 2),
-            "how": "2 GiB device copy (read + write); %d threads x 8 chains x %d fmas" % (threads, iters)}
+            "how": "2 GiB device copy (read + write); 2 GiB of 64-byte records read once each in a scattered "
+                   "order; %d threads x 8 chains x %d fmas" % (threads, iters)}
 
 
 def cpu_model():
@@ -169,6 +189,11 @@ def main():
 
     wall_b, sts_b = run("batched", args.steps, args.warmup)
     wall_p, sts_p = run("plain", args.plain_steps, 1)
+    # the headline frame once more with its image, outside the clock: the
+    # CPU-baseline leg checks a band of its rows against the oracle
+    gpu_img, _ = r.render("batched")
+    band = cpu_band(scene, cam, dict(kind="direct", nx=nx, ny=ny, seed=seed, light_samples=4), args.cpu_seconds)
+    gpu_samples = r.sample_radiance(band[1] * cam["xres"] * nx * ny)[band[0] * cam["xres"] * nx * ny:]
     st = sts_b[-1]
     rays = st["camera_rays"] + st["shadow_rays"]
     ms_b = float(np.mean([s["ms_total"] for s in sts_b]))
@@ -191,8 +216,10 @@ def main():
     # sections so that its all-core run does not slow their host side); it
     # also yields the reference traversal's per-ray work used for the
     # algorithmic figures ----
-    cpu, work = cpu_baseline(scene, cam, nx, ny, seed, args.cpu_seconds)
+    cpu, work, parity = cpu_baseline(scene, cam, nx, ny, seed, band, gpu_img, gpu_samples)
+    del gpu_samples
     log(args, "cpu baseline %s" % json.dumps(cpu))
+    log(args, "parity %s" % json.dumps(parity))
 
     peaks = measured_peaks()
     log(args, "measured ceilings: %s" % peaks)
@@ -201,9 +228,12 @@ def main():
     any_ms = float(np.mean([s["ms_trace_any"] / max(1, s["launches_any"]) for s in sts_b]))
     cl_ms = float(np.mean([s["ms_trace_closest"] / max(1, s["launches_closest"]) for s in sts_b]))
     dominant = "trace_any" if any_ms >= cl_ms else "trace_closest"
-    # algorithmic bytes per ray: 32 B per node popped + 36 B per triangle
-    # tested + 32 B ray in + 16 B hit out (1 bit for occlusion); flops per
-    # ray ~ 15 per node + 50 per triangle
+    # bytes touched per ray (the reference traversal's reads, counted on the
+    # oracle's traversal of a band of the same frame): 32 B per node popped +
+    # 36 B per triangle tested + 32 B ray in + 16 B hit out (1 bit for
+    # occlusion); the launch's sum counts every ray's reads, most of them
+    # served by L1, L2 and MALL, so it is NOT an HBM figure.  flops per ray
+    # ~ 15 per node + 50 per triangle.
     if dominant == "trace_any":
         nodes, tris, out_b, n_rays = work["any_nodes"], work["any_tris"], 1 / 8.0, st["shadow_rays"]
         kernel_ms = any_ms
@@ -212,27 +242,46 @@ def main():
         nodes, tris, out_b, n_rays = work["closest_nodes"], work["closest_tris"], 16.0, st["camera_rays"]
         kernel_ms = cl_ms
         kname = KERNEL_CLOSEST
-    bytes_per_ray = 32 * nodes + 36 * tris + 32 + out_b
-    per_launch_bytes = bytes_per_ray * n_rays
-    achieved = per_launch_bytes / (kernel_ms * 1e-3) / 1e9
+    touched_per_ray = 32 * nodes + 36 * tris + 32 + out_b
+    touched = touched_per_ray * n_rays
     flops_launch = (15 * nodes + 50 * tris) * n_rays
-    traffic = None
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes,
+    # reads scaled by the 64-byte gather calibration (the traversal's request
+    # size), writes by the copy calibration
+    traffic = prof_t = calib = None
     if os.path.exists(PROFILE_TRAFFIC):
         try:
-            traffic = json.load(open(PROFILE_TRAFFIC))["kernels"][kname]["hbm_bytes"]
+            tj = json.load(open(PROFILE_TRAFFIC))
+            prof_t = tj["kernels"][kname]
+            traffic = prof_t["hbm_bytes"]
+            calib = {k: tj[k] for k in ("fetch_factor", "write_factor", "copy_fetch_factor") if k in tj}
         except (KeyError, ValueError):
-            traffic = None
-    # share of the SIMDs' VALU issue slots the kernel used (SQ_INSTS_VALU
-    # against the dispatch's clock, one wave64 instruction per SIMD every 4
-    # cycles), from the committed counter pass
-    valu_issue = l2_hit = None
+            traffic = prof_t = None
+    # VALU issue: SQ_INSTS_VALU of the kernel's dispatch (committed counter
+    # pass) against the issue slots of the live launch time at the peak
+    # engine clock (each of the 1024 SIMDs issues at most one wave64 VALU
+    # instruction every 4 cycles), so the fraction cannot pass 1; lanes
+    # active per VALU instruction from SQ_THREAD_CYCLES_VALU over
+    # SQ_ACTIVE_INST_VALU x 64
+    issue = None
     if os.path.exists(PROFILE_VALU):
         try:
             kv = json.load(open(PROFILE_VALU))["kernels"][kname]
-            valu_issue = round(kv["valu_issue_frac"], 3)
-            l2_hit = None if kv.get("l2_hit_rate") is None else round(kv["l2_hit_rate"], 3)
+            slots = kernel_ms * 1e-3 * PEAK_CLOCK_HZ * SIMDS / 4.0
+            issue = {"valu_insts_per_launch": int(kv["valu"]), "salu_insts_per_launch": int(kv["salu"]),
+                     "slots_per_launch": int(slots), "frac": round(kv["valu"] / slots, 4),
+                     "lane_efficiency": None if kv.get("lane_efficiency") is None else round(kv["lane_efficiency"], 3),
+                     "clock_ghz_in_profile": None if kv.get("clock_ghz") is None else round(kv["clock_ghz"], 3),
+                     "l2_hit_rate": None if kv.get("l2_hit_rate") is None else round(kv["l2_hit_rate"], 3),
+                     "peak_clock_ghz": PEAK_CLOCK_HZ / 1e9}
         except (KeyError, ValueError):
-            valu_issue = None
+            issue = None
+    compulsory = n_rays * (32 + out_b)
+    achieved = None if traffic is None else traffic / (kernel_ms * 1e-3) / 1e9
+    hbm_frac = None if achieved is None else achieved / HBM_PEAK_GBS
+    binding = None
+    if hbm_frac is not None and issue is not None:
+        binding = "valu_issue" if issue["frac"] >= hbm_frac else "hbm"
 
     out = {
         "metric": "Mrays/s batched vs plain GPU kernel; p50/p99 request latency at fixed rate",
@@ -264,28 +313,37 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
-            "achieved": round(achieved, 1),
+            "kernel_symbol": kname,
+            "achieved": None if achieved is None else round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": None if hbm_frac is None else round(hbm_frac, 4),
             "traffic": None if traffic is None else int(traffic),
+            "traffic_source": "rocprofv3 FETCH_SIZE x 64-byte-gather calibration + WRITE_SIZE x copy calibration, "
+                              "per dispatch (profiles/traffic_direct1024.md)",
+            "calibration": calib if traffic is not None else None,
             "kernel_ms_per_launch": round(kernel_ms, 4),
-            "alg_bytes_per_ray": round(bytes_per_ray, 2),
             "rays_per_launch": int(n_rays),
-            "measured_hbm_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
+            "binding": binding,
+            "issue": issue,
+            "compulsory_bytes_per_launch": int(compulsory),
+            "bytes_touched": {"per_ray": round(touched_per_ray, 2), "per_launch": int(touched),
+                              "gb_per_s": round(touched / (kernel_ms * 1e-3) / 1e9, 1),
+                              "note": "the reference traversal's reads summed over the launch's rays, served by "
+                                      "L1/L2/MALL as well as HBM: not an HBM rate"},
+            "valu_flops": {"per_launch": int(flops_launch),
+                           "tflops": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3),
+                           "frac_of_fp32_peak": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
             "measured_peaks": peaks,
-            "l2_hit_rate": l2_hit,
-            "frac_of_measured_hbm": round(achieved / peaks["hbm_copy_gbs"], 4),
-            "valu": {"achieved": round(flops_launch / (kernel_ms * 1e-3) / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-                     "issue_frac": valu_issue,
-                     "frac_of_measured_fp32": round(flops_launch / (kernel_ms * 1e-3) / 1e12 / peaks["fp32_fma_tflops"], 4)},
+            "frac_of_measured_gather": None if achieved is None or not peaks.get("gather64_gbs") else
+            round(achieved / peaks["gather64_gbs"], 4),
             "work_per_ray": {k: round(v, 2) for k, v in work.items()},
         },
         "serving": serve,
         "render_serving": render_serve,
         "workloads": workloads,
         "cpu_baseline": cpu,
+        "parity": parity,
         "device": px.device_name(),
     }
     print(json.dumps(out), flush=True)
@@ -600,28 +658,53 @@ def render_serving(args, px, scenes, log_fn):
     }
 
 
-def cpu_baseline(scene, cam, nx, ny, seed, seconds):
-    """The oracle's render (CPU restatement of the same path) timed on the
-    host cores over a bounded band of image rows of the same frame.  Also
-    returns the reference traversal's work per ray on that band (nodes popped
-    and prims tested; occlusion queries counted up to their first hit), the
-    per-unit figures the roofline uses."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    # the host share given to this process (the pool sets OMP_NUM_THREADS),
-    # else every core
-    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
-    integ = dict(kind="direct", nx=nx, ny=ny, seed=seed, light_samples=4)
-    # calibrate: 2 rows first, then as many rows as fit the budget
+def cpu_cores():
+    """the host share given to this process (the pool sets
+    OMP_NUM_THREADS), else every core"""
+    return int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
+
+
+def cpu_band(scene, cam, integ, seconds):
+    """The rows [y0, y1) of the frame the CPU baseline renders: from the
+    middle row down, as many as fit the time budget (timed on two rows, the
+    scene build excluded)."""
+    ol = _oracle()
     y0 = cam["yres"] // 2
     t = time.perf_counter()
-    res = ol.render(scene, cam, integ, y0=y0, y1=y0 + 2, nthreads=cores)
+    res = ol.render(scene, cam, integ, y0=y0, y1=y0 + 2, nthreads=cpu_cores())
     first = time.perf_counter() - t
     build_s = res["stats"][3] * 1e-6
     per_row = max(1e-4, (first - build_s) / 2)
     rows = int(max(2, min(cam["yres"] - y0, (seconds - build_s) / per_row)))
-    res = ol.render(scene, cam, integ, y0=y0, y1=y0 + rows, nthreads=cores)
+    return y0, y0 + rows
+
+
+def cpu_baseline(scene, cam, nx, ny, seed, band, gpu_img, gpu_samples):
+    """The oracle's render (CPU restatement of the same path) timed on the
+    host cores over a bounded band of image rows of the same frame, and the
+    GPU frame's same rows checked against it bit for bit: every sample's
+    radiance and every film pixel.  Also returns the reference traversal's
+    work per ray on that band (nodes popped and prims tested; occlusion
+    queries counted up to their first hit), the per-unit figures the roofline
+    uses."""
+    ol = _oracle()
+    cores = cpu_cores()
+    integ = dict(kind="direct", nx=nx, ny=ny, seed=seed, light_samples=4)
+    y0, y1 = band
+    res = ol.render(scene, cam, integ, y0=y0, y1=y1, nthreads=cores, want_samples=True)
     st = [int(v) for v in res["stats"]]
+    rows = y1 - y0
+    spp = nx * ny
+    want_s = res["samples"].reshape(-1, 3)
+    got_s = gpu_samples.reshape(-1, 3)
+    bad_s = int(np.count_nonzero(np.any(want_s.view(np.uint32) != got_s.view(np.uint32), axis=1)))
+    got_f = gpu_img[y0:y1]
+    bad_f = int(np.count_nonzero(np.any(res["rgb"].view(np.uint32) != got_f.view(np.uint32), axis=2)))
+    parity = {"rows": [y0, y1], "samples": int(want_s.shape[0]), "pixels": int(rows * cam["xres"]),
+              "mismatches": bad_s + bad_f, "sample_mismatches": bad_s, "pixel_mismatches": bad_f,
+              "rmse": float(np.sqrt(np.mean((res["rgb"].astype(np.float64) - got_f.astype(np.float64)) ** 2))),
+              "against": "oracle render of the same rows (per-sample radiance and film, bit for bit)"}
+    assert want_s.shape[0] == rows * cam["xres"] * spp
     trace_s = st[4] * 1e-9
     rays = st[0] + st[1]
     # one thread on two rows, for the per-core rate
@@ -629,12 +712,18 @@ def cpu_baseline(scene, cam, nx, ny, seed, seconds):
     s1 = [int(v) for v in one["stats"]]
     cpu = {"value": round(rays / trace_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
            "sample": "oracle render of rows %d..%d of the same frame (%d camera + %d shadow rays, "
-                     "shading included), BVH build excluded" % (y0, y0 + rows, st[0], st[1]),
+                     "shading included), BVH build excluded" % (y0, y1, st[0], st[1]),
            "one_thread_mrays_per_s": round((s1[0] + s1[1]) / (s1[4] * 1e-9) / 1e6, 4),
            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     work = {"closest_nodes": st[5] / max(1, st[0]), "closest_tris": st[6] / max(1, st[0]),
             "any_nodes": st[7] / max(1, st[1]), "any_tris": st[8] / max(1, st[1])}
-    return cpu, work
+    return cpu, work, parity
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    return oracle_lib
 
 
 if __name__ == "__main__":

@@ -115,7 +115,8 @@ int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
    (batched path); refill_pct = idle-lane refill threshold (batched path);
    scheme 0 = a traversal stack per ray over the compressed records (the
    exact ones when a scene has none), 1 = one stack per wave of 64
-   consecutive rays (batched path; equal-sign groups for closest hit), 2 = a
+   consecutive rays in equal-sign groups (batched path, closest hit and
+   camera rays only: occlusion (kind 1) refuses it with PBRTX_ERR_ARG), 2 = a
    stack per ray over the exact records.  Process-wide setting. */
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme);
@@ -199,6 +200,18 @@ int pbrtx_filter_table(int32_t kind, float xw, float yw, float p0, float p1, flo
 /* renders one frame with the given traversal path; rgb_out (host, may be
    NULL) receives xres*yres*3 floats: the film's final rgb */
 int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);
+/* Film::write_image (film.rs:316-354, write_img :15-33) of the last frame:
+   the film's final rgb to `filename` by its extension -- .pfm (the floats),
+   .ppm (8-bit binary) or .png (8-bit RGB) -- the 8-bit forms with write_img's
+   encoding (255 p^(1/2.2) + 0.5, clamped to [0, 255], truncated).
+   PBRTX_ERR_STATE before the first frame, PBRTX_ERR_ARG for another
+   extension. */
+int pbrtx_renderer_write_image(pbrtx_renderer* r, const char* filename);
+/* the same for a host rgb image (xres*yres*3 floats, rows top to bottom);
+   host only, no device needed */
+int pbrtx_write_image(const char* filename, const float* rgb, uint32_t xres, uint32_t yres);
+/* write_img's per-channel encoding (film.rs:22-24) of n floats; host only */
+int pbrtx_image_encode8(const float* rgb, uint64_t n, uint8_t* out);
 /* radiance of the first `count` samples of the last frame (3 floats each;
    sample index = pixel * spp + sample, pixel = y * xres + x) */
 int pbrtx_renderer_sample_radiance(pbrtx_renderer* r, float* out, uint64_t count);
@@ -328,6 +341,15 @@ int pbrtx_device_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stre
    Time it with the timer below for the achievable non-packed FP32 rate. */
 uint64_t pbrtx_fma_probe_threads(void);
 int pbrtx_fma_probe(float* d_out, uint64_t out_floats, uint32_t iters, uint64_t* flops, void* stream);
+/* 64-byte gather probe: reads each 64-byte record of d_buf (bytes = 64 x 2^k,
+   k <= 32) exactly once, in a scattered order (neighbouring threads read
+   far-apart records, as traversal lanes do), four 16-byte loads a record.
+   The reference point for calibrating memory-traffic counters on the
+   traversal's request size: it reads exactly `bytes` and writes (almost)
+   nothing.  d_out: pbrtx_gather_probe_threads() words.  No counterpart in
+   the reference (measurement plumbing). */
+uint64_t pbrtx_gather_probe_threads(void);
+int pbrtx_gather_probe(const void* d_buf, uint64_t bytes, uint32_t* d_out, uint64_t out_words, void* stream);
 /* HIP-event timer on a stream: start, then stop returns elapsed ms */
 int pbrtx_timer_start(void* stream);
 int pbrtx_timer_stop(void* stream, float* ms);

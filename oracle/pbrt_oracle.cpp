@@ -1063,6 +1063,15 @@ static RGB li_direct(const RenderCtx& C, uint32_t pixel, uint32_t s, Ray r, uint
     return L;
 }
 
+// SpecularReflection::sample_f (specular.rs:34-40) with Fresnel::noop
+// (F = 1) in the BSDF's local frame: wi = (-wo.x, -wo.y, wo.z), pdf 1,
+// f = F * R / |cos theta_i| (abs_cos_theta = |wi.z|, bsdf/utils.rs).
+static void mirror_sample_f(V3 wo, RGB r, V3* wi, float* pdf, RGB* f) {
+    *wi = mk(-wo.x, -wo.y, wo.z);
+    *pdf = 1.0f;
+    *f = rgb_divs(r, std::fabs(wi->z));
+}
+
 // Path Li (deviation D1): pbrt-v2 PathIntegrator with one light sampled per
 // vertex (no MIS), Lambertian / mirror BSDF sampling, roulette when
 // bounce > rr_start, at most max_depth bounces.
@@ -1111,10 +1120,7 @@ static RGB li_path(const RenderCtx& C, uint32_t pixel, uint32_t s, Ray r, uint64
             f = rgb_muls(mat.r, 1.0f / PI_F);
             spec = false;
         } else {
-            // SpecularReflection::sample_f (specular.rs:34-40), Fresnel::noop
-            wil = mk(-wol.x, -wol.y, wol.z);
-            pdf = 1.0f;
-            f = rgb_divs(mat.r, std::fabs(wil.z));
+            mirror_sample_f(wol, mat.r, &wil, &pdf, &f);
             spec = true;
         }
         if (rgb_black(f) || pdf == 0.0f) break;
@@ -1356,6 +1362,14 @@ float orc_ld_vdc(uint32_t n, uint32_t scramble) { return ld_vdc(n, scramble); }
 float orc_ld_sobol2(uint32_t n, uint32_t scramble) { return ld_sobol2(n, scramble); }
 void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts) {
     ld_pixel_points(seed, pixel, spp, pts);
+}
+
+void orc_mirror_sample_f(const float wo[3], const float r[3], float wi[3], float* pdf, float f[3]) {
+    V3 w;
+    RGB fr;
+    mirror_sample_f(mk(wo[0], wo[1], wo[2]), RGB{r[0], r[1], r[2]}, &w, pdf, &fr);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    f[0] = fr.r; f[1] = fr.g; f[2] = fr.b;
 }
 
 int orc_filter_eval(int kind, float xw, float yw, float p0, float p1, float x, float y, float* out) {

@@ -163,6 +163,9 @@ void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts
    tested by closest-hit queries (camera and extension rays), [7] nodes
    visited and [8] prims tested by occlusion queries up to their first hit
    (the work of an early-exit any-hit query).  nthreads >= 1. */
+/* SpecularReflection::sample_f (specular.rs:34-40) with Fresnel::noop, in the
+   BSDF's local frame: wi, pdf and f for reflectance r. */
+void orc_mirror_sample_f(const float wo[3], const float r[3], float wi[3], float* pdf, float f[3]);
 /* Filter::evaluate (filter.rs:89-124) and the film's 16x16 weight table
    (film.rs:89-97) of a filter given as in orc_render_desc. */
 int orc_filter_eval(int kind, float xw, float yw, float p0, float p1, float x, float y, float* out);

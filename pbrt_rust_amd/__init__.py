@@ -9,11 +9,11 @@ from ._lib import (HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, Pinne
 from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, Filter, LDSampler, PathIntegrator, Primitive,
                   RayBatch,
                   RayServer, RenderServer, SamplerRenderer, Sphere, StratifiedSampler, Transform, TriangleMesh,
-                  reset_tuning, set_tuning)
+                  encode8, reset_tuning, set_tuning, write_image)
 
 __all__ = [
     "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "Filter", "LDSampler", "PathIntegrator", "SamplerRenderer",
     "StratifiedSampler", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "PinnedArray", "Primitive",
-    "RAY_DTYPE", "RayBatch", "RayServer", "RenderServer", "Sphere", "Transform", "TriangleMesh", "device_name", "lib", "reset_tuning", "set_tuning",
-    "synchronize",
+    "RAY_DTYPE", "RayBatch", "RayServer", "RenderServer", "Sphere", "Transform", "TriangleMesh", "device_name", "encode8", "lib", "reset_tuning", "set_tuning",
+    "synchronize", "write_image",
 ]

@@ -140,6 +140,9 @@ def lib():
     L.pbrtx_fma_probe_threads.argtypes = []
     L.pbrtx_fma_probe_threads.restype = u64
     L.pbrtx_fma_probe.argtypes = [vp, u64, ctypes.c_uint32, ctypes.POINTER(u64), vp]
+    L.pbrtx_gather_probe_threads.argtypes = []
+    L.pbrtx_gather_probe_threads.restype = u64
+    L.pbrtx_gather_probe.argtypes = [vp, u64, vp, u64, vp]
     L.pbrtx_device_malloc.argtypes = [c.POINTER(vp), u64]
     L.pbrtx_device_free.argtypes = [vp]
     L.pbrtx_memcpy_htod.argtypes = [vp, vp, u64]
@@ -167,6 +170,9 @@ def lib():
         L.pbrtx_renderer_render.argtypes = [vp, i32, vp, c.POINTER(RenderStats)]
         L.pbrtx_renderer_destroy.argtypes = [vp]
         L.pbrtx_renderer_sample_radiance.argtypes = [vp, vp, u64]
+        L.pbrtx_renderer_write_image.argtypes = [vp, c.c_char_p]
+        L.pbrtx_write_image.argtypes = [c.c_char_p, vp, u32, u32]
+        L.pbrtx_image_encode8.argtypes = [vp, u64, vp]
         L.pbrtx_renderer_set_chunk.argtypes = [vp, u32]
         L.pbrtx_renderer_set_graph.argtypes = [vp, i32]
         L.pbrtx_renderer_set_octant_sort.argtypes = [vp, i32]

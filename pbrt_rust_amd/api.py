@@ -12,6 +12,7 @@ each hit ray's tmax to the hit t, the reference's maxt side effect
 (src/primitive/geometric.rs:64).
 """
 import ctypes
+import os
 import warnings
 
 import numpy as np
@@ -484,7 +485,8 @@ def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray
     renderer camera-ray ("camera") kernel of a path, and its traversal scheme: "ray" (a stack per ray, the
     compressed 64-byte records when the scene has them), "ray_exact" (a stack
     per ray on the exact 128-byte records) or "packet" (a stack per wave of 64
-    consecutive rays; batched occlusion).  stack: LDS stack entries per lane
+    consecutive rays in equal-sign groups; batched closest hit and camera rays
+    only, refused for occlusion).  stack: LDS stack entries per lane
     of the per-ray kernel (8, 16 or 24; None keeps the current depth)."""
     k = KINDS[kind] if isinstance(kind, str) else int(kind)
     sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
@@ -645,6 +647,24 @@ class Filter:
         return t.reshape(16, 16)
 
 
+def write_image(filename, rgb):
+    """Film::write_image's file output (film.rs:15-33, 316-354) of an rgb
+    image (yres x xres x 3 floats): .pfm, .ppm or .png by extension; host
+    only."""
+    a = np.ascontiguousarray(rgb, F32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("rgb must be yres x xres x 3")
+    check(lib().pbrtx_write_image(os.fsencode(filename), a.ctypes.data, a.shape[1], a.shape[0]))
+
+
+def encode8(rgb):
+    """write_img's per-channel 8-bit encoding (film.rs:22-24) of floats."""
+    a = np.ascontiguousarray(rgb, F32)
+    out = np.zeros(a.shape, np.uint8)
+    check(lib().pbrtx_image_encode8(a.ctypes.data, a.size, out.ctypes.data))
+    return out
+
+
 class SamplerRenderer:
     """Renderer::render for a generated scene dict (see scenes.py), a camera
     dict (scenes.pinhole) and an integrator; the frame runs on the GPU.
@@ -689,6 +709,11 @@ class SamplerRenderer:
                                           ctypes.byref(st)))
         stats = {f: getattr(st, f) for f, _ in st._fields_}
         return out, stats
+
+    def write_image(self, filename):
+        """Film::write_image of the last frame: .pfm (floats), .ppm or .png
+        (8-bit, write_img's encoding)."""
+        check(lib().pbrtx_renderer_write_image(self._h, os.fsencode(filename)))
 
     def sample_radiance(self, count=None):
         n = self.xres * self.yres * self.spp if count is None else int(count)

@@ -47,9 +47,15 @@ def sweep(renderer, space=None, repeats=2, log=None):
     return out
 
 
-def sweep_kind(renderer, kind, stage, space=None, repeats=2, log=None, packets=True):
-    """Tune one kernel kind on a renderer's frames, timing `stage`."""
+def sweep_kind(renderer, kind, stage, space=None, repeats=2, log=None, packets=None):
+    """Tune one kernel kind on a renderer's frames, timing `stage`.  The
+    packet scheme is swept for camera rays only (packets=None); occlusion has
+    no packet kernel and the library refuses it."""
     space = space or DEFAULT_SPACE
+    if packets is None:
+        packets = kind == "camera"
+    if packets and kind == "any":
+        raise ValueError("occlusion has no packet scheme")
     cands = [(b, c, f, "ray", k) for b, c, f, k in
              itertools.product(space["block"], space["blocks_per_cu"], space["refill_pct"], space["stack"])]
     if packets:

@@ -319,6 +319,9 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
     if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET && scheme != SCHEME_RAY_EXACT)
         return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
     if (scheme == SCHEME_PACKET && path != 1) return fail(PBRTX_ERR_ARG, "the packet scheme exists for the batched path only");
+    if (scheme == SCHEME_PACKET && kind == 1)
+        return fail(PBRTX_ERR_ARG, "occlusion has no packet scheme: its kernel was removed after it hung once with the "
+                                   "cause not found");
     TraceTuning t{block, blocks_per_cu, refill_pct, scheme};
     t.stack = g_tuning[path][kind].stack;
     g_tuning[path][kind] = t;
@@ -410,6 +413,18 @@ int pbrtx_fma_probe(float* d_out, uint64_t out_floats, uint32_t iters, uint64_t*
     return PBRTX_OK;
 }
 uint64_t pbrtx_fma_probe_threads(void) { return fma_chain_threads(); }
+
+int pbrtx_gather_probe(const void* d_buf, uint64_t bytes, uint32_t* d_out, uint64_t out_words, void* stream) {
+    const uint64_t recs = bytes / 64;
+    if (!d_buf || !d_out || bytes % 64 != 0 || recs == 0 || (recs & (recs - 1)) != 0 || recs > (1ull << 32))
+        return fail(PBRTX_ERR_ARG, "gather probe needs a device buffer of 64 x 2^k bytes (k <= 32)");
+    if (out_words < gather_probe_threads())
+        return fail(PBRTX_ERR_ARG, "gather probe needs pbrtx_gather_probe_threads() output words");
+    hipError_t e = gather_probe(d_buf, recs, d_out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gather probe");
+    return PBRTX_OK;
+}
+uint64_t pbrtx_gather_probe_threads(void) { return gather_probe_threads(); }
 
 int pbrtx_timer_start(void* stream) {
     std::lock_guard<std::mutex> lk(g_ev_mu);

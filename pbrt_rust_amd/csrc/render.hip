@@ -1464,6 +1464,7 @@ struct Renderer {
     hipStream_t stream = nullptr;
     EventPool events;
     bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
+    bool have_frame = false;   // rgb holds a finished frame
     // graph mode, per path: warm after one eager frame; the captured frame
     // and the tuning generation it was captured under
     bool use_graph = false;
@@ -1681,6 +1682,7 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         stats->extension_rays = (uint64_t)c[8] | ((uint64_t)c[9] << 32);
         stats->build_ms = R->sc.build_ms;
     }
+    R->have_frame = true;
     return PBRTX_OK;
 }
 
@@ -1795,6 +1797,16 @@ int pbrtx_renderer_render(pbrtx_renderer* h, int32_t path, float* rgb_out, pbrtx
         if (e != hipSuccess) return hip_fail(e, "hipMemcpy(rgb)");
     }
     return PBRTX_OK;
+}
+
+int pbrtx_renderer_write_image(pbrtx_renderer* h, const char* filename) {
+    if (!h || !filename) return fail(PBRTX_ERR_ARG, "null argument");
+    Renderer& R = h->r;
+    if (!R.have_frame) return fail(PBRTX_ERR_STATE, "no frame rendered yet");
+    std::vector<float> rgb((size_t)R.P.xres * R.P.yres * 3);
+    hipError_t e = hipMemcpy(rgb.data(), R.rgb.p, rgb.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(rgb)");
+    return pbrtx_write_image(filename, rgb.data(), R.P.xres, R.P.yres);
 }
 
 int pbrtx_renderer_sample_radiance(pbrtx_renderer* h, float* out, uint64_t count) {

@@ -83,9 +83,9 @@ struct Server {
     int device = 0;
     Clock::time_point t0;
     Arena arena;
-    std::vector<const DeviceScene*> scenes;
+    std::vector<const DeviceScene*> scenes;  // guarded by mu (add_scene runs on client threads)
     uint64_t scene_bytes = 0;
-    size_t spill_need = 0;
+    size_t spill_need = 0;                   // guarded by mu
     // compute, upload and download streams (all default priority)
     hipStream_t s_comp = nullptr, s_up = nullptr, s_down = nullptr;
     TraceSeg* h_segs = nullptr;  // pinned staging of the segment tables, two halves
@@ -218,6 +218,14 @@ int Server::launch_step() {
         ++nseg;
     }
     if (groups.empty()) return 0;
+    // pbrtx_server_add_scene may grow the scene table meanwhile (under mu)
+    std::vector<const DeviceScene*> scn;
+    size_t spill;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        scn = scenes;
+        spill = spill_need;
+    }
     Step st;
     if (!spare.empty()) {
         st = spare.back();
@@ -237,7 +245,7 @@ int Server::launch_step() {
     uint32_t si = 0;
     std::vector<Request*> last;
     for (auto& g : groups) {
-        const DeviceScene* sc = scenes[g.first.first];
+        const DeviceScene* sc = scn[g.first.first];
         const bool any = g.first.second == 1;
         const uint32_t s0 = si;
         uint32_t start = 0;
@@ -259,7 +267,7 @@ int Server::launch_step() {
         if (e != hipSuccess) return hip_fail(e, "segment upload");
         const TraceTuning& tun = g_tuning[PATH_BATCHED][any ? 1 : 0];
         TraceScratch* scr;
-        int rc = stream_scratch(s_comp, spill_need, &scr);
+        int rc = stream_scratch(s_comp, spill, &scr);
         if (rc) return rc;
         e = trace_segments(sc->view, any, ds + s0, si - s0, start, tun, scr, s_comp);
         if (e != hipSuccess) return hip_fail(e, "trace_segments");
@@ -303,7 +311,11 @@ int Server::retire_steps(bool wait_oldest) {
 int Server::serve_plain(Request* r) {
     hipError_t e = hipEventSynchronize(r->ready);
     if (e != hipSuccess) return hip_fail(e, "ray upload");
-    const DeviceScene* sc = scenes[r->scene];
+    const DeviceScene* sc;
+    {
+        std::lock_guard<std::mutex> lk(mu);  // add_scene may grow the table
+        sc = scenes[r->scene];
+    }
     Step st;
     if (!spare.empty()) {
         st = spare.back();

@@ -745,17 +745,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
     }
 }
 
-// Batched occlusion, wave-coherent ("packet") form.  A wave takes 64
-// consecutive rays (neighbouring samples of one pixel in a render queue) and
-// walks the four-wide records with ONE stack whose entries carry the 64-bit
-// mask of lanes whose box test passed; every record and prim record is read
-// once per wave from a wave-uniform address instead of once per lane.
-// Exactness: for an occlusion query tmax is the ray's own until its answer
-// is known, and a lane reaches a leaf exactly when every box above it passes
-// at that tmax (its bit survives each ballot), as in the reference; the order
-// in which leaves are reached does not change "is any prim hit".  A lane
-// drops out at its first hit.
-constexpr int PSTK = 96;               // stack entries per wave
+constexpr int PSTK = 96;  // stack entries per wave of the packet closest-hit kernel below
 constexpr int PWAVES = MAX_BLOCK / 64;
 
 // Read-only scene data seen through the constant address space: a load from
@@ -765,115 +755,6 @@ typedef __attribute__((address_space(4))) const float cfloat;
 __device__ __forceinline__ float4 cload4(const void* p, uint32_t k) {
     cfloat* q = (cfloat*)p + 4 * k;
     return make_float4(q[0], q[1], q[2], q[3]);
-}
-
-__global__ __launch_bounds__(MAX_BLOCK) void k_occlude_packet(SceneView sv, const Ray* rays, uint32_t n_host,
-                                                              const uint32_t* n_dev, uint32_t* occ,
-                                                              uint32_t* counter, uint32_t* err) {
-    const uint32_t n = n_dev ? *n_dev : n_host;
-    __shared__ int32_t s_ref[PWAVES][PSTK];
-    __shared__ unsigned long long s_msk[PWAVES][PSTK];
-    const uint32_t lane = __lane_id();
-    const uint32_t w = threadIdx.x >> 6;
-    while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(counter, 64u);
-        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-        if (base >= n) break;
-        const uint32_t i = base + lane;
-        RayState r{};
-        bool alive = false;
-        if (i < n) {
-            load_ray(rays, i, r);
-            float tn;
-            alive = slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2],
-                         sv.root_hi[2], r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &tn);
-        }
-        unsigned long long live = __ballot(alive);
-        bool hit = false;
-        // visiting order: the reference's near-first order for the signs of
-        // the first live ray (a heuristic here: any order gives the same bits)
-        uint32_t sgn = 0;
-        if (live) {
-            const uint32_t lead = (uint32_t)__builtin_ctzll(live);
-            sgn = (uint32_t)((__ballot(r.ix < 0.0f) >> lead) & 1ull) |
-                  ((uint32_t)((__ballot(r.iy < 0.0f) >> lead) & 1ull) << 1) |
-                  ((uint32_t)((__ballot(r.iz < 0.0f) >> lead) & 1ull) << 2);
-            sgn = __builtin_amdgcn_readfirstlane(sgn);
-        }
-        int sp = 0;
-        auto push = [&](int32_t ref, unsigned long long m) {
-            if (sp < PSTK) {
-                s_ref[w][sp] = ref;
-                s_msk[w][sp] = m;
-                ++sp;
-            } else {
-                atomicOr(err, 1u);  // deeper than the launcher's bound: never expected
-            }
-        };
-        if (live) push(sv.qroot, live);
-        while (sp > 0) {
-            --sp;
-            const int32_t ref = __builtin_amdgcn_readfirstlane(s_ref[w][sp]);
-            const unsigned long long m = s_msk[w][sp] & live;
-            if (m == 0) continue;
-            const bool act = (m >> lane) & 1ull;
-            if (ref >= 0) {
-                const QNode* q = sv.qnodes + ref;
-                const float4 LX = cload4(q, 0), HX = cload4(q, 1), LY = cload4(q, 2), HY = cload4(q, 3),
-                             LZ = cload4(q, 4), HZ = cload4(q, 5), R6 = cload4(q, 6);
-                const int4 RF = make_int4((int)as_u(R6.x), (int)as_u(R6.y), (int)as_u(R6.z), (int)as_u(R6.w));
-                const uint32_t meta = as_u(cload4(q, 7).x);
-                float t;
-                const bool p0 = act && RF.x != QEMPTY && slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, r.ox, r.oy,
-                                                              r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                const bool p1 = act && RF.y != QEMPTY && slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, r.ox, r.oy,
-                                                              r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                const bool p2 = act && RF.z != QEMPTY && slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, r.ox, r.oy,
-                                                              r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                const bool p3 = act && RF.w != QEMPTY && slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, r.ox, r.oy,
-                                                              r.oz, r.ix, r.iy, r.iz, r.tmin, r.tmax, &t);
-                const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2),
-                                         m3 = __ballot(p3);
-                const uint32_t aA = meta & 3u, aB = (meta >> 2) & 3u, aC = (meta >> 4) & 3u;
-                const bool nA = (sgn >> aA) & 1u, nB = (sgn >> aB) & 1u, nC = (sgn >> aC) & 1u;
-                const int b0 = nB ? 1 : 0, b1 = nB ? 0 : 1, c0 = nC ? 3 : 2, c1 = nC ? 2 : 3;
-                const int o0 = nA ? c0 : b0, o1 = nA ? c1 : b1, o2 = nA ? b0 : c0, o3 = nA ? b1 : c1;
-                // farthest pushed first, so the nearest pops next
-#define PX_PUSH(O)                                                                             \
-    {                                                                                           \
-        const unsigned long long mo = O == 0 ? m0 : (O == 1 ? m1 : (O == 2 ? m2 : m3));         \
-        if (mo) push(sel4i(O, RF.x, RF.y, RF.z, RF.w), mo);                                     \
-    }
-                PX_PUSH(o3)
-                PX_PUSH(o2)
-                PX_PUSH(o1)
-                PX_PUSH(o0)
-#undef PX_PUSH
-            } else {
-                // leaf: its prims in order until the record flagged last
-                uint32_t pos = (uint32_t)(~ref);
-                while (true) {
-                    const PrimRec* p = sv.prims + pos;
-                    const float4 a = cload4(p, 0), b = cload4(p, 1), c = cload4(p, 2);
-                    const uint32_t kind = as_u(c.w);
-                    if (act && !hit) {
-                        float t, u, v;
-                        if ((kind & 0xFFu) == 0u) hit = tri_test(a, b, c, r, &t, &u, &v);
-                        else hit = sphere_test(sv.spheres[as_u(c.y)], r, &t);
-                    }
-                    if (kind & 0x100u) break;  // last prim of the leaf
-                    ++pos;
-                }
-                live &= ~__ballot(hit);
-                if (!live) break;
-            }
-        }
-        // base is a multiple of 64: this wave owns two whole bit words
-        const unsigned long long hm = __ballot(hit);
-        if (lane == 0) occ[base >> 5] = (uint32_t)hm;
-        if (lane == 32 && base + 32 < n) occ[(base >> 5) + 1] = (uint32_t)(hm >> 32);
-    }
 }
 
 // Batched closest hit, wave-coherent ("packet") form, for coherent rays such
@@ -1110,6 +991,25 @@ __global__ void k_copy16(uint4* dst_, const uint4* src_, size_t n) {
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// Random 64-byte gathers: thread i reads record (i * GATHER_MUL) mod n of a
+// buffer of n = 2^k records (an odd multiplier permutes [0, n), so every
+// record is read exactly once and neighbouring lanes read far-apart records,
+// as traversal lanes read records), four 16-byte loads each.  The loaded
+// words are folded into one value that is stored only if it equals a
+// constant no folded data here can give by chance often enough to matter
+// (the store keeps the loads alive; its slot is the thread's own).
+constexpr uint32_t GATHER_MUL = 0x9E3779B1u;
+__global__ __launch_bounds__(256) void k_gather64(const uint4* buf, uint32_t n_mask, uint32_t n, uint32_t* out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4* p = buf + 4ull * ((i * GATHER_MUL) & n_mask);
+        const uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+        acc += (a.x ^ a.y ^ a.z ^ a.w) + (b.x ^ b.y ^ b.z ^ b.w) + (c.x ^ c.y ^ c.z ^ c.w) + (d.x ^ d.y ^ d.z ^ d.w);
+    }
+    if (acc == 0x5EEDF00Du) out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 // Eight independent fma chains a thread: a measure of the achievable FP32
 // vector rate (one fma per lane per instruction; no packed math).
 __global__ __launch_bounds__(256) void k_fma_chain(float* out, uint32_t iters, float a, float b) {
@@ -1172,6 +1072,15 @@ hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st) 
     const size_t cap = (size_t)device_cu_count() * 16;
     if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n);
+    return hipGetLastError();
+}
+
+uint64_t gather_probe_threads() { return (uint64_t)device_cu_count() * 8 * 256; }
+
+hipError_t gather_probe(const void* buf, uint64_t records, uint32_t* out, hipStream_t st) {
+    const unsigned blocks = (unsigned)(gather_probe_threads() / 256);
+    hipLaunchKernelGGL(k_gather64, dim3(blocks), dim3(256), 0, st, (const uint4*)buf, (uint32_t)(records - 1),
+                       (uint32_t)records, out);
     return hipGetLastError();
 }
 
@@ -1262,13 +1171,8 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
                            (uint32_t*)scr->spill, lv, err);
         return hipGetLastError();
     }
-    if (ANY && tun.scheme == SCHEME_PACKET && packet_fits(s)) {
-        hipError_t e = fill_u32(scr->counter, 0u, 1, st);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_occlude_packet, dim3(wide_grid(k_occlude_packet, tun)), dim3(block), 0, st, s,
-                           rays, n, n_dev, occ, scr->counter, err);
-        return hipGetLastError();
-    }
+    // occlusion has no packet form (pbrtx_set_tuning refuses it)
+    if (ANY && tun.scheme == SCHEME_PACKET) return hipErrorNotSupported;
     if (!ANY && tun.scheme == SCHEME_PACKET && packet_fits(s) && s.prims64) {
         hipError_t e = fill_u32(scr->counter, 0u, 1, st);
         if (e != hipSuccess) return e;

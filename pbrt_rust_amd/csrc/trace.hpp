@@ -89,6 +89,11 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
 // streaming 16-byte-vector copy (bytes a multiple of 16)
 hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st);
+// 64-byte gather probe over a buffer of `records` (a power of two, at most
+// 2^32) 64-byte records, each read once in a scattered order;
+// gather_probe_threads() threads, out holds one word per thread
+uint64_t gather_probe_threads();
+hipError_t gather_probe(const void* buf, uint64_t records, uint32_t* out, hipStream_t st);
 // fma-chain probe: fma_chain_threads() threads, 8 chains of `iters` fmas
 // each; out holds one float per thread
 uint64_t fma_chain_threads();

@@ -403,13 +403,22 @@ class Pbrt:
             return api.PathIntegrator(int(_param(p, "maxdepth", 5)), int(_param(p, "rrstart", 3)))
         return api.DirectLightingIntegrator(int(_param(p, "lightsamples", 1)))
 
+    def film_filename(self):
+        """The Film's "filename" (the image write_image writes); pbrt-v2's
+        default names an EXR file, which this build does not write, so the
+        default here is "pbrt.png"."""
+        return str(_param(self.film_params, "filename", "pbrt.png"))
+
     def world_end(self, chunk=None):
-        """The renderer of the described scene (WorldEnd, main.rs:774-800)."""
+        """The renderer of the described scene (WorldEnd, main.rs:774-800),
+        with the Film's file name as its `filename`."""
         if self.stack:
             warnings.warn("missing AttributeEnd/TransformEnd at WorldEnd")
         self.in_world = False
-        return api.SamplerRenderer(self.scene(), self.camera_dict(), self.sampler_obj(), self.integrator_obj(),
-                                   chunk=chunk, filter=self.filter_obj())
+        r = api.SamplerRenderer(self.scene(), self.camera_dict(), self.sampler_obj(), self.integrator_obj(),
+                                chunk=chunk, filter=self.filter_obj())
+        r.filename = self.film_filename()
+        return r
 
 
 # ---- scene files -------------------------------------------------------------
@@ -521,3 +530,21 @@ def load(path, pbrt=None):
     """parse() of a scene file."""
     with open(path) as f:
         return parse(f.read(), pbrt)
+
+
+def run(filenames, path="batched"):
+    """Pbrt::run (main.rs:828-842): each scene file is read, its world
+    rendered and the image written to the Film's file name (relative names
+    relative to the working directory).  Returns the image files written."""
+    written = []
+    for fn in filenames:
+        r = load(fn)
+        if r is None:
+            raise ValueError("%s: no world block" % fn)
+        try:
+            r.render(path, want_image=False)
+            r.write_image(r.filename)
+            written.append(r.filename)
+        finally:
+            r.close()
+    return written

@@ -78,6 +78,8 @@ def lib():
                                   ctypes.c_float, ctypes.c_float, fp]
     L.orc_filter_table.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    F32P]
+    L.orc_mirror_sample_f.argtypes = [F32P, F32P, F32P, F32P, F32P]
+    L.orc_mirror_sample_f.restype = None
     L.orc_ld_draw.argtypes = [ctypes.c_uint32] * 3
     L.orc_ld_draw.restype = ctypes.c_uint64
     L.orc_ld_vdc.argtypes = [ctypes.c_uint32, ctypes.c_uint32]

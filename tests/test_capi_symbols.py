@@ -52,6 +52,11 @@ def test_errors_are_codes_not_aborts():
     assert L.pbrtx_set_tuning(1, 3, 256, 0, 50, 0) < 0
     assert L.pbrtx_set_tuning(1, 1, 256, 0, 101, 0) < 0
     assert L.pbrtx_set_tuning(0, 1, 256, 0, 10, 1) < 0   # packet: batched only
+    # occlusion has no packet kernel; closest hit and camera rays keep theirs
+    assert L.pbrtx_set_tuning(1, 1, 256, 0, 0, 1) < 0 and b"no packet scheme" in L.pbrtx_last_error()
+    assert L.pbrtx_set_tuning(1, 2, 256, 16, 0, 1) == 0
+    assert L.pbrtx_set_tuning(1, 0, 256, 0, 0, 1) == 0
+    assert L.pbrtx_reset_tuning() == 0
     assert L.pbrtx_set_tuning(1, 1, 256, 0, 10, 3) < 0
     assert L.pbrtx_intersect(None, None, 1, None, 0, None) < 0
     assert L.pbrtx_set_stack_depth(1, 1, 12) < 0 and b"stack depth" in L.pbrtx_last_error()

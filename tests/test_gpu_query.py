@@ -279,8 +279,9 @@ def packet_occlusion():
 
 
 @pytest.mark.skip(reason="hung once on the GPU (no output for 180 s) while a build with early block exits and "
-                         "multi-packet work claims in the traversal kernels ran it; that code is taken out, the "
-                         "cause is not shown, so the case stays off the GPU")
+                         "multi-packet work claims in the traversal kernels ran it; the cause was not shown, so the "
+                         "packet occlusion kernel was removed and pbrtx_set_tuning refuses the scheme for "
+                         "occlusion (tests/test_capi_symbols.py checks the refusal)")
 def test_packet_occlusion_scheme_matches_oracle(packet_occlusion):
     # one stack per wave of 64 rays: incoherent rays, the point-light shadow
     # segments of the ray-query workload, the 8-sphere fixture, ragged counts

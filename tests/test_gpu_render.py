@@ -309,3 +309,78 @@ def test_scene_file_renders_as_the_oracle():
         assert st["shadow_rays"] == ref["stats"][1] and img.mean() > 0.0
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_mirror_closed_form_scenes_match_oracle(path):
+    # the scenes whose oracle radiance tests/test_oracle_mirror.py pins by
+    # closed forms: mirror -> Lambertian under a point light, and mirror ->
+    # area light (emission after a specular bounce); the device gives the
+    # oracle's samples bit for bit
+    import test_oracle_mirror as tm
+    v, idx = tm._two_quads(False)
+    point = dict(verts=v, idx=idx, face_mat=np.array([1, 1, 0, 0], np.uint32),
+                 mats=np.array([[0, 0.6, 0.6, 0.6], [1, 0.9, 0.9, 0.9]], np.float32),
+                 face_light=np.full(4, -1, np.int32), area_le=np.zeros((0, 3), np.float32),
+                 point_lights=np.array([[0, 4, 5, 20, 10, 5]], np.float32))
+    emit = dict(point, face_light=np.array([-1, -1, 0, 0], np.int32),
+                area_le=np.array([[4, 2, 1]], np.float32), point_lights=np.zeros((0, 6), np.float32))
+    for scene in (point, emit):
+        for integ in (tm.PATH1, dict(tm.PATH1, max_depth=3, rr_start=1)):
+            ref = oracle_render(scene, tm.CAM, integ)
+            img, samples, _ = gpu_render(scene, tm.CAM, integ, path)
+            assert_same(ref, img, samples)
+            assert img.max() > 0
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_headline_settings_on_a_band(path):
+    # the direct-lighting headline's settings -- stratified 4x4 spp, 4 light
+    # samples per hit, the quad area light, fBm-displaced icosphere and ground
+    # -- at level 6 (81,920 sphere triangles) and 256x256: a band of rows of
+    # the GPU frame equals the oracle's render of those rows, sample for
+    # sample and pixel for pixel
+    scene = scenes.direct_lighting_scene(level=6)
+    cam = scenes.pinhole(256, 256)
+    integ = dict(kind="direct", nx=4, ny=4, seed=1, light_samples=4)
+    y0, y1 = 112, 144
+    ref = ol.render(scene, cam, integ, y0=y0, y1=y1, nthreads=16, want_samples=True)
+    sampler = px.StratifiedSampler(4, 4, 1)
+    r = px.SamplerRenderer(scene, cam, sampler, px.DirectLightingIntegrator(4))
+    img, st = r.render(path)
+    spp = 16
+    samples = r.sample_radiance(y1 * 256 * spp)[y0 * 256 * spp:].reshape(y1 - y0, 256, spp, 3)
+    r.close()
+    assert np.array_equal(samples.view(np.uint32), ref["samples"].view(np.uint32))
+    assert np.array_equal(img[y0:y1].view(np.uint32), ref["rgb"].view(np.uint32))
+    assert st["camera_rays"] == 256 * 256 * spp and st["shadow_rays"] > 0
+
+
+def test_write_image_and_scene_file_run(tmp_path):
+    # Film::write_image of a rendered frame: the .pfm holds the frame's rgb
+    # bit for bit, the .png write_img's encoding of it; pbrt_api.run renders
+    # a scene file and writes the Film's "filename"
+    import test_image_out as ti
+    import test_pbrt_api as tp
+    from pbrt_rust_amd import pbrt_api as pa
+    scene = scenes.direct_lighting_scene(level=3)
+    cam = scenes.pinhole(40, 24)
+    r = px.SamplerRenderer(scene, cam, px.StratifiedSampler(2, 2, 3), px.DirectLightingIntegrator(2))
+    with pytest.raises(px.PbrtxError):
+        r.write_image(str(tmp_path / "early.pfm"))  # no frame yet
+    img, _ = r.render("batched")
+    r.write_image(str(tmp_path / "f.pfm"))
+    r.write_image(str(tmp_path / "f.png"))
+    r.close()
+    assert np.array_equal(ti.read_pfm(str(tmp_path / "f.pfm")).view(np.uint32), img.view(np.uint32))
+    assert np.array_equal(ti.read_png(str(tmp_path / "f.png")), ti.want8(img))
+    out = str(tmp_path / "scene.png")
+    text = tp.SCENE.replace('Film "image"', 'Film "image" "string filename" "%s"' % out)
+    fn = tmp_path / "scene.pbrt"
+    fn.write_text(text)
+    assert pa.run([str(fn)]) == [out]
+    p = pa.Pbrt()
+    pa.parse(text, p).close()
+    want = ol.render(p.scene(), p.camera_dict(), dict(kind="direct", nx=2, ny=2, seed=0, light_samples=2),
+                     nthreads=16, filt=p.filter_obj().args())
+    assert np.array_equal(ti.read_png(out), ti.want8(want["rgb"]))

@@ -147,3 +147,26 @@ def test_pinned_buffers(world):
     res, _ = srv.wait(srv.submit(2, "any", pin.array, out.array))
     assert np.array_equal(res, alone(world, 2, "any", rays))
     srv.close()
+
+
+def test_scene_added_while_serving(world):
+    # pbrtx_server_add_scene while the worker is tracing: the worker reads the
+    # scene table under the server's lock, so requests in flight and requests
+    # on the new scene all give their results alone
+    srv = px.RayServer(arena_bytes=64 << 20, batch_rays=1 << 16)
+    srv.add_scene(world[0][0])
+    reqs = make_requests(world[:1], [200000, 150000, 90000], 21)
+    tickets = [srv.submit(s, kind, rays) for s, kind, rays in reqs]
+    late = []
+    for k in (1, 2):
+        assert srv.add_scene(world[k][0]) == k
+        o, d = scene_rays(world[k][0], 50000, 70 + k)
+        rays = px.RayBatch(o, d)
+        late.append((k, "closest", rays, srv.submit(k, "closest", rays)))
+    for (s, kind, rays), t in zip(reqs, tickets):
+        out, _ = srv.wait(t)
+        assert same(kind, out, alone(world, s, kind, rays))
+    for s, kind, rays, t in late:
+        out, _ = srv.wait(t)
+        assert same(kind, out, alone(world, s, kind, rays))
+    srv.close()
