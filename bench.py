@@ -52,8 +52,9 @@ def parse():
                     help="skip the ray-query and path-tracing measurements")
     ap.add_argument("--pt-spp-side", type=int, default=8, help="path-tracing workload: sqrt(spp)")
     ap.add_argument("--serve-pool", type=int, default=48, help="distinct requests in the serving pool")
-    ap.add_argument("--serve-requests", type=int, default=600, help="requests in the fixed-rate run")
-    ap.add_argument("--serve-load", type=float, default=0.5,
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="arrival span of the fixed-rate ray-serving run (requests = rate x span)")
+    ap.add_argument("--serve-load", type=float, default=0.7,
                     help="fixed-rate ray-serving load as a fraction of the batched server's saturated rate")
     ap.add_argument("--serve-passes", type=int, default=5,
                     help="timed saturated passes over the ray-serving pool per server (the median is reported)")
@@ -407,11 +408,11 @@ def serving(args, px, scenes, log_fn):
     shadow segments, origins uniform in the scene box and directions uniform
     on the sphere.  Saturated throughput with every request queued at once
     (the median of several passes; the first, cold pass runs about a fifth
-    slower), then p50/p99 latency under Poisson arrivals at half the batched
-    server's saturated rate (--serve-load), for the continuous-batching
-    server and for the one-plain-kernel-per-request server at the same rate.
-    Under arrivals the batched server sustains about 60% of its saturated
-    rate (requests arrive one at a time, so batches are smaller)."""
+    slower), then p50/p99 latency under Poisson arrivals at 70% of the
+    batched server's saturated rate (--serve-load) over --serve-seconds of
+    arrivals, for the continuous-batching server and for the
+    one-plain-kernel-per-request server at the same rate; achieved = rays
+    served / time until the last request is done, beside the offered rate."""
     import threading
     rng = np.random.default_rng(0x5E4E)
     counts = np.exp(rng.uniform(np.log(1e4), np.log(2e6), 8)).astype(np.int64)
@@ -506,9 +507,11 @@ def serving(args, px, scenes, log_fn):
     log_fn("serving saturated: batched %.1f Mrays/s, plain %.1f Mrays/s" % (sat_b, sat_p))
     rate = args.serve_load * sat_b
     req_per_s = rate * 1e6 / mean_rays
-    nreq = args.serve_requests
+    nreq = max(len(pool), int(round(args.serve_seconds * req_per_s)))
     lat_b = at_rate("batched", req_per_s, nreq)
     lat_p = at_rate("plain", req_per_s, nreq)
+    for lat in (lat_b, lat_p):
+        lat["achieved_over_offered"] = round(lat["achieved_mrays_per_s"] / rate, 3)
     log_fn("serving at %.1f Mrays/s: batched %s plain %s" % (rate, lat_b, lat_p))
     return {
         "workload": "8 resident scenes of random triangles (%d tris total), %d-request pool of 1K-4M rays "

@@ -92,6 +92,20 @@ int pbrtx_bvh_occlusion_records(const pbrtx_bvh* bvh, uint32_t* n, uint32_t* dep
 int pbrtx_bvh_free(pbrtx_bvh* bvh);
 
 /* ---- device scene -------------------------------------------------------- */
+/* A scene whose tree is built on the GPU: a linear BVH (Morton-ordered
+   primitives, Karras radix tree) over mesh faces then spheres, laid out for
+   both traversal paths.  Replaces BVHAccelerator::new's host build
+   (bvh.rs:189-258) where admission time matters (serving): its tree is not
+   the reference's, so closest-hit queries return the reference's hit except
+   on rays where the reference's own answer depends on its visiting order:
+   several primitives sharing the closest t, or a hit on its box's boundary
+   that the pop-time box test (bvh.rs:390-395) culls at a smaller maxt.
+   Occlusion bits are the reference's exactly.  No
+   host tree is kept (pbrtx_bvh_* inspection does not apply).  *build_ms (may
+   be NULL) receives the build's wall time, uploads included.  A render
+   description selects it with split = 3. */
+int pbrtx_scene_build_gpu(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces,
+                          const pbrtx_sphere* spheres, uint32_t nspheres, pbrtx_scene** out, double* build_ms);
 /* Copies both node layouts and the primitive records to HBM.  The handle is
    immutable afterwards and may be shared by host threads. */
 int pbrtx_scene_create(const pbrtx_bvh* bvh, pbrtx_scene** out);
@@ -131,7 +145,7 @@ int pbrtx_reset_tuning(void);
 
 /* ---- renderer: SamplerRenderer::render (sampler_renderer.rs:146-206) ------ */
 /* Scene + camera + sampler + integrator description.  Triangles (world-space
-   vertices, index triples) with a material per face: mats holds 4 floats per
+   vertices, index triples) and spheres, each with a material: mats holds 4 floats per
    material (kind 0 = matte / Lambertian, 1 = mirror; rgb reflectance).
    face_light: area-light id of an emitting face or -1; area_le: radiance of
    each area light (3 floats); point_lights: 6 floats each (position,
@@ -153,12 +167,20 @@ typedef struct pbrtx_render_desc {
     uint32_t light_samples;
     uint32_t max_depth;
     uint32_t rr_start;
-    int32_t split; uint32_t max_prims;
+    int32_t split; uint32_t max_prims;  /* split 3: the tree built on the GPU (pbrtx_scene_build_gpu) */
     /* 0 StratifiedSampler (nx x ny strata, jittered; stratified.rs:60-127),
        1 LDSampler (nx*ny samples per pixel rounded up to a power of two;
        lds.rs:17-70), its scrambles and shuffles drawn from the counter-based
-       generator.  Requests of the render server are stratified. */
+       generator, 2 HaltonSampler (nx*ny samples per pixel on average;
+       halton.rs:18-107) over the image as one window.  Requests of the
+       render server are stratified. */
     int32_t sampler;
+    /* spheres (Shape::sphere, main.rs:314-323, sphere.rs:28-44) after the
+       faces: prim id nfaces + k, material sphere_mat[k]; shaded with the
+       sphere's differential geometry (sphere.rs:135-175, helpers.rs:17-43);
+       not emitters.  NULL / 0 for none. */
+    const pbrtx_sphere* spheres; uint32_t nspheres;
+    const uint32_t* sphere_mat;
 } pbrtx_render_desc;
 
 typedef struct pbrtx_render_stats {
@@ -200,6 +222,12 @@ int pbrtx_filter_table(int32_t kind, float xw, float yw, float p0, float p1, flo
 /* renders one frame with the given traversal path; rgb_out (host, may be
    NULL) receives xres*yres*3 floats: the film's final rgb */
 int pbrtx_renderer_render(pbrtx_renderer* r, int32_t path, float* rgb_out, pbrtx_render_stats* stats);
+/* Sample slots per pixel of the renderer's frames: nx*ny for the stratified
+   sampler, that rounded up to a power of two for LDSampler, and for
+   HaltonSampler the most samples any pixel receives (a pixel's samples take
+   its first slots in sequence order; its other slots are empty: zero
+   radiance, no film weight). */
+int pbrtx_renderer_sample_slots(pbrtx_renderer* r, uint32_t* slots_per_pixel);
 /* Film::write_image (film.rs:316-354, write_img :15-33) of the last frame:
    the film's final rgb to `filename` by its extension -- .pfm (the floats),
    .ppm (8-bit binary) or .png (8-bit RGB) -- the 8-bit forms with write_img's
@@ -213,7 +241,8 @@ int pbrtx_write_image(const char* filename, const float* rgb, uint32_t xres, uin
 /* write_img's per-channel encoding (film.rs:22-24) of n floats; host only */
 int pbrtx_image_encode8(const float* rgb, uint64_t n, uint8_t* out);
 /* radiance of the first `count` samples of the last frame (3 floats each;
-   sample index = pixel * spp + sample, pixel = y * xres + x) */
+   sample index = pixel * spp + sample, pixel = y * xres + x, spp as
+   pbrtx_renderer_sample_slots gives it) */
 int pbrtx_renderer_sample_radiance(pbrtx_renderer* r, float* out, uint64_t count);
 int pbrtx_renderer_destroy(pbrtx_renderer* r);
 

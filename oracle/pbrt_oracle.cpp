@@ -924,22 +924,52 @@ static bool trace_any(const Scene& sc, const Ray& r0, uint64_t* st) {
 struct HitGeom {
     V3 p, nn, dpdu;
     float eps;
-    uint32_t face;
+    uint32_t mat;   // material index
+    int32_t light;  // area light of an emitting face, or -1
 };
+// The shading geometry of a sphere hit (Sphere::intersect, sphere.rs:135-175,
+// and compute_dg, helpers.rs:17-43): the object-space hit of the world ray at
+// t, dp/du and dp/dv there (theta from acos, sin theta from sin, as the
+// reference's libm calls), the world point o2w(p_hit), the world partials
+// and dg.nn = normalize(dpdu x dpdv) flipped when reverse_orientation ^
+// transform_swaps_handedness (diff_geom.rs:52-60).  dn/du, dn/dv and u, v
+// feed nothing the renderer computes and are left out.
+static void sphere_frame(const Sphere& s, const Ray& r, float t, V3* pw, V3* nn, V3* dpdu_w) {
+    const Ray ray = ray_to_object(s.w2o, r);
+    const V3 ph = add(ray.o, mul(ray.d, t));  // ray.point_at (sphere.rs:142)
+    const float theta = std::acos(std::min(std::max(ph.z / s.radius, -1.0f), 1.0f));
+    const float zradius = std::sqrt(ph.x * ph.x + ph.y * ph.y);
+    const float inv_z = 1.0f / zradius;
+    const float cos_phi = ph.x * inv_z, sin_phi = ph.y * inv_z;
+    const V3 dpdu = mk(-s.phimax * ph.y, s.phimax * ph.x, 0.0f);
+    const V3 dpdv = mul(mk(ph.z * cos_phi, ph.z * sin_phi, -s.radius * std::sin(theta)), s.thetamax - s.thetamin);
+    *pw = xf_point(s.o2w.m, ph);
+    *dpdu_w = xf_vector(s.o2w.m, dpdu);
+    const V3 dpdv_w = xf_vector(s.o2w.m, dpdv);
+    V3 n = normalize(cross(*dpdu_w, dpdv_w));
+    if (s.ro != s.swaps) n = mul(n, -1.0f);
+    *nn = n;
+}
 static HitGeom hit_geom(const RenderCtx& C, const Ray& r, uint32_t prim, float t) {
     const Prim& pr = C.sc.prims[prim];
-    const Mesh& m = C.sc.meshes[pr.mesh];
     HitGeom h;
+    h.eps = t * 5e-4f;  // mesh.rs:262, sphere.rs:174
+    if (pr.kind == 1) {
+        sphere_frame(C.sc.spheres[pr.sphere], r, t, &h.p, &h.nn, &h.dpdu);
+        h.mat = C.d->sphere_mat ? C.d->sphere_mat[pr.sphere] : 0;
+        h.light = -1;
+        return h;
+    }
+    const Mesh& m = C.sc.meshes[pr.mesh];
     h.p = add(r.o, mul(r.d, t));  // Ray::point_at, ray.rs:60
     tri_frame(m.p[pr.v[0]], m.p[pr.v[1]], m.p[pr.v[2]], &h.nn, &h.dpdu);
-    h.eps = t * 5e-4f;  // mesh.rs:262
-    h.face = prim;
+    h.mat = C.d->face_mat[prim];
+    h.light = C.d->face_light ? C.d->face_light[prim] : -1;
     return h;
 }
 static RGB emitted(const RenderCtx& C, const HitGeom& h, V3 wo) {
-    int32_t l = C.d->face_light ? C.d->face_light[h.face] : -1;
-    if (l < 0) return rgb(0.0f);
-    if (dot(h.nn, wo) > 0.0f) return C.areas[(size_t)l].le;
+    if (h.light < 0) return rgb(0.0f);
+    if (dot(h.nn, wo) > 0.0f) return C.areas[(size_t)h.light].le;
     return rgb(0.0f);
 }
 
@@ -1017,21 +1047,90 @@ static uint32_t next_pow2(uint32_t v) {
     while (p < v) p <<= 1;
     return p;
 }
-// The sampler of a render: stratified (nx x ny, jittered) or low-discrepancy
-// (spp rounded up to a power of two, LDSampler::new lds.rs:17-24).
-struct SamplerDesc {
-    int kind;  // 0 stratified, 1 low-discrepancy
-    uint32_t seed, xres, nx, ny, spp;
+// radical_inverse (montecarlo.rs:8-21), in f64 as the reference computes it
+static double radical_inverse(uint64_t n, uint64_t b) {
+    double v = 0.0;
+    uint64_t num = n;
+    const double inv_base = 1.0 / (double)b;
+    double aib = 1.0;
+    while (num > 0) {
+        const double d = (double)(num % b);
+        num /= b;
+        aib *= inv_base;
+        v += d * aib;
+    }
+    return v;
+}
+// Lerp for f32 (utils/mod.rs:15-18): a * (1 - t) + b * t
+static float lerp_f(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// HaltonSampler::get_more_samples (halton.rs:48-107) over the whole image as
+// one window [0, xres) x [0, yres) (the reference's task windows depend on
+// the host's core count, so the build samples the image as one window):
+// sample k of the sequence, k < max(xres, yres)^2 * spp, lies at
+// (lerp(0, delta, rinv(k, 3)), lerp(0, delta, rinv(k, 2))) with delta =
+// max(dx, dy), and is kept when inside the image.  The frame's samples are
+// grouped by the pixel they fall in, in sequence order within a pixel; every
+// pixel has K slots (K = the most samples any pixel received) and slots past
+// its count are empty (no ray, no film weight).
+struct HaltonTable {
+    uint32_t K = 0;
+    std::vector<uint32_t> cnt;  // samples of each pixel
+    std::vector<float> pos;     // image position of slot s of pixel p at 2 (p K + s)
 };
-static void sample_at(const SamplerDesc& S, uint32_t px, uint32_t py, uint32_t s, float* ix, float* iy) {
+static void halton_table(uint32_t xres, uint32_t yres, uint32_t spp, HaltonTable* T) {
+    const int32_t x0 = 0, x1 = (int32_t)xres, y0 = 0, y1 = (int32_t)yres;
+    const float dx = (float)(x1 - x0), dy = (float)(y1 - y0);
+    const float delta = dy > dx ? dy : dx;  // dy.max(dx)
+    const uint64_t side = xres > yres ? xres : yres;
+    const uint64_t wanted = side * side * spp;
+    std::vector<std::vector<std::pair<float, float>>> per((size_t)xres * yres);
+    for (uint64_t k = 0; k < wanted; ++k) {
+        const float u = (float)radical_inverse(k, 3), v = (float)radical_inverse(k, 2);
+        const float ix = lerp_f((float)x0, (float)x0 + delta, u);
+        const float iy = lerp_f((float)y0, (float)y0 + delta, v);
+        if (ix >= (float)x1 || iy >= (float)y1) continue;
+        per[(size_t)(uint32_t)iy * xres + (uint32_t)ix].push_back({ix, iy});
+    }
+    T->K = 0;
+    for (auto& v : per) T->K = std::max<uint32_t>(T->K, (uint32_t)v.size());
+    T->cnt.assign(per.size(), 0);
+    T->pos.assign(per.size() * T->K * 2, 0.0f);
+    for (size_t p = 0; p < per.size(); ++p) {
+        T->cnt[p] = (uint32_t)per[p].size();
+        for (size_t s = 0; s < per[p].size(); ++s) {
+            T->pos[2 * (p * T->K + s)] = per[p][s].first;
+            T->pos[2 * (p * T->K + s) + 1] = per[p][s].second;
+        }
+    }
+}
+
+// The sampler of a render: stratified (nx x ny, jittered), low-discrepancy
+// (spp rounded up to a power of two, LDSampler::new lds.rs:17-24) or Halton
+// (K slots a pixel, see HaltonTable).
+struct SamplerDesc {
+    int kind;  // 0 stratified, 1 low-discrepancy, 2 Halton
+    uint32_t seed, xres, nx, ny, spp;
+    const HaltonTable* H;
+};
+// false: an empty Halton slot
+static bool sample_at(const SamplerDesc& S, uint32_t px, uint32_t py, uint32_t s, float* ix, float* iy) {
     if (S.kind == 0) {
         sample_pos(S.seed, S.xres, S.nx, S.ny, true, px, py, s, ix, iy);
-        return;
+        return true;
+    }
+    if (S.kind == 2) {
+        const size_t p = (size_t)py * S.xres + px;
+        if (s >= S.H->cnt[p]) return false;
+        *ix = S.H->pos[2 * (p * S.H->K + s)];
+        *iy = S.H->pos[2 * (p * S.H->K + s) + 1];
+        return true;
     }
     std::vector<float> pts(2 * (size_t)S.spp);
     ld_pixel_points(S.seed, py * S.xres + px, S.spp, pts.data());
     *ix = (float)px + pts[2 * s];  // CameraSample::new(x_pos as f32 + image_samples[2i], ..) lds.rs / utils.rs:140-146
     *iy = (float)py + pts[2 * s + 1];
+    return true;
 }
 
 // Direct lighting Li (deviation D1): emitted light plus, for every light,
@@ -1044,7 +1143,7 @@ static RGB li_direct(const RenderCtx& C, uint32_t pixel, uint32_t s, Ray r, uint
     HitGeom h = hit_geom(C, r, prim, t);
     V3 wo = neg(r.d);
     RGB L = emitted(C, h, wo);
-    const Mat& mat = C.mats[C.d->face_mat[h.face]];
+    const Mat& mat = C.mats[h.mat];
     uint32_t ns_area = C.d->light_samples;
     for (uint32_t li = 0; li < C.nlights; ++li) {
         bool is_point = li < C.points.size();
@@ -1085,7 +1184,7 @@ static RGB li_path(const RenderCtx& C, uint32_t pixel, uint32_t s, Ray r, uint64
         HitGeom h = hit_geom(C, r, prim, t);
         V3 wo = neg(r.d);
         if (bounce == 0 || specular) L = rgb_add(L, rgb_mul(beta, emitted(C, h, wo)));
-        const Mat& mat = C.mats[C.d->face_mat[h.face]];
+        const Mat& mat = C.mats[h.mat];
         float ua[4], ub[4];
         rng4(C.d->seed, pixel, s, 1 + 2 * bounce, ua);
         rng4(C.d->seed, pixel, s, 2 + 2 * bounce, ub);
@@ -1149,11 +1248,23 @@ static bool setup_render(RenderCtx& C, const orc_render_desc* d) {
     for (uint32_t i = 0; i < d->nverts; ++i)
         m.p[i] = mk(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]);
     C.sc.meshes.push_back(std::move(m));
-    C.sc.prims.resize(d->nfaces);
+    C.sc.prims.resize(d->nfaces + d->nspheres);
     for (uint32_t f = 0; f < d->nfaces; ++f) {
         Prim& p = C.sc.prims[f];
         p.kind = 0; p.mesh = 0; p.sphere = 0;
         p.v[0] = d->idx[3 * f + 2]; p.v[1] = d->idx[3 * f + 1]; p.v[2] = d->idx[3 * f];
+    }
+    // spheres after the faces (prim ids nfaces + k), as Shape::sphere makes
+    // them (main.rs:314-323, sphere.rs:28-44)
+    for (uint32_t k = 0; k < d->nspheres; ++k) {
+        const orc_sphere_desc& q = d->spheres[k];
+        Xf o2w{m4_from(q.o2w), m4_from(q.o2w_inv)};
+        C.sc.spheres.push_back(sphere_new(o2w, xf_inverse(o2w), q.reverse_orientation != 0, q.radius, q.z0, q.z1,
+                                          q.phimax_deg));
+        Prim& p = C.sc.prims[d->nfaces + k];
+        p.kind = 1; p.mesh = 0; p.sphere = k;
+        p.v[0] = p.v[1] = p.v[2] = 0;
+        if (d->sphere_mat && d->sphere_mat[k] >= d->nmats) return false;
     }
     for (uint32_t i = 0; i < d->nmats; ++i)
         C.mats.push_back(Mat{(int)d->mats[4 * i], RGB{d->mats[4 * i + 1], d->mats[4 * i + 2], d->mats[4 * i + 3]}});
@@ -1364,6 +1475,18 @@ void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts
     ld_pixel_points(seed, pixel, spp, pts);
 }
 
+double orc_radical_inverse(uint64_t n, uint64_t b) { return radical_inverse(n, b); }
+
+int orc_halton_table(uint32_t xres, uint32_t yres, uint32_t spp, uint32_t* K, uint32_t* cnt, float* pos) {
+    if (xres == 0 || yres == 0 || spp == 0 || !K) return -1;
+    HaltonTable T;
+    halton_table(xres, yres, spp, &T);
+    *K = T.K;
+    if (cnt) std::copy(T.cnt.begin(), T.cnt.end(), cnt);
+    if (pos) std::copy(T.pos.begin(), T.pos.end(), pos);
+    return 0;
+}
+
 void orc_mirror_sample_f(const float wo[3], const float r[3], float wi[3], float* pdf, float f[3]) {
     V3 w;
     RGB fr;
@@ -1520,6 +1643,26 @@ int orc_intersect_brute(const orc_scene* s, uint32_t n, const float* o, const fl
             }
         }
         prim[i] = best; t[i] = bt; ties[i] = nt;
+    }
+    return 0;
+}
+
+int orc_hit_box_misses(const orc_scene* s, uint32_t n, const float* o, const float* d, const float* mint,
+                       const float* maxt, uint32_t* out) {
+    if (!s) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r{mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]), mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]), mint[i], maxt[i]};
+        uint32_t cnt = 0;
+        for (uint32_t k = 0; k < s->sc.prims.size(); ++k) {
+            Ray q = r;
+            float tt, bb1, bb2;
+            if (!prim_intersect(s->sc, s->sc.prims[k], q, &tt, &bb1, &bb2)) continue;
+            // the prim's own box at maxt = its hit t (bvh.rs:389-392 at pop time)
+            Ray rb = r;
+            rb.maxt = tt;
+            if (!bbox_intersect(prim_world_bound(s->sc, s->sc.prims[k]), rb, nullptr, nullptr)) ++cnt;
+        }
+        out[i] = cnt;
     }
     return 0;
 }
@@ -1691,9 +1834,13 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
     if (!perspective_r2c(d->xres, d->yres, d->screen, d->fov_deg, C.r2c)) return -4;
     C.c2w = m4_from(d->cam_to_world);
     const uint32_t xres = d->xres, yres = d->yres;
-    if (d->sampler_kind != 0 && d->sampler_kind != 1) return -6;
+    if (d->sampler_kind < 0 || d->sampler_kind > 2) return -6;
+    HaltonTable H;
+    if (d->sampler_kind == 2) halton_table(xres, yres, d->nx * d->ny, &H);
     const SamplerDesc S{d->sampler_kind, d->seed, xres, d->nx, d->ny,
-                        d->sampler_kind == 1 ? next_pow2(d->nx * d->ny) : d->nx * d->ny};
+                        d->sampler_kind == 1 ? next_pow2(d->nx * d->ny)
+                                             : (d->sampler_kind == 2 ? H.K : d->nx * d->ny),
+                        &H};
     const uint32_t spp = S.spp;
     OFilter F;
     if (d->filter_xw == 0.0f && d->filter_yw == 0.0f) make_filter(0, 0.5f, 0.5f, 0.0f, 0.0f, &F);
@@ -1712,7 +1859,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
         for (uint32_t x = 0; x < xres; ++x)
             for (uint32_t s = 0; s < spp; ++s) {
                 float ix, iy;
-                sample_at(S, x, yy, s, &ix, &iy);
+                if (!sample_at(S, x, yy, s, &ix, &iy)) continue;  // empty slot
                 int32_t a, b;
                 sample_extent(iy, F.yw, (int32_t)yres, &a, &b);
                 bool in = !(b < (int32_t)y0 || a >= (int32_t)y1);
@@ -1749,7 +1896,7 @@ int orc_render(const orc_render_desc* d, uint32_t y0, uint32_t y1, uint32_t nthr
                     if (qx < 0 || qx >= (int32_t)xres) continue;
                     for (uint32_t s = 0; s < spp; ++s) {
                         float ix, iy;
-                        sample_at(S, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
+                        if (!sample_at(S, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy)) continue;
                         int32_t xa, xb, ya_, yb_;
                         sample_extent(ix, F.xw, (int32_t)xres, &xa, &xb);
                         sample_extent(iy, F.yw, (int32_t)yres, &ya_, &yb_);

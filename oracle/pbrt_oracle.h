@@ -81,6 +81,12 @@ int orc_intersect_p(const orc_scene* s, uint32_t n, const float* o, const float*
 int orc_intersect_brute(const orc_scene* s, uint32_t n, const float* o, const float* d,
                         const float* mint, const float* maxt, uint32_t* prim, float* t,
                         uint32_t* ties);
+/* For each ray: how many prims it hits (full interval) whose own box the
+   slab test rejects at maxt = that hit's t -- rays where the traversal's
+   pop-time box test at the current maxt can cull a hit, so that its result
+   depends on the order in which it meets the tree's nodes. */
+int orc_hit_box_misses(const orc_scene* s, uint32_t n, const float* o, const float* d, const float* mint,
+                       const float* maxt, uint32_t* out);
 /* Sphere::intersect full hit record (sphere.rs:135-175 + helpers.rs:17-43):
    out[0]=t, [1]=ray_epsilon, [2..4]=p, [5..7]=nn, [8]=u, [9]=v, [10..12]=dpdu,
    [13..15]=dpdv.  returns 1 on hit */
@@ -124,6 +130,14 @@ void orc_pixel_samples(uint32_t seed, uint32_t xres, uint32_t px, uint32_t py,
    product header too).  materials: 4 floats each (kind 0 = matte/Lambertian,
    1 = mirror; rgb); face_light: area-light id per face or -1; area_le: 3 floats
    per area light; point_lights: 6 floats each (pos, I). */
+/* a sphere (Shape::sphere, sphere.rs:28-44): object-to-world m and m_inv
+   (row-major), radius, z range, phi max in degrees, reverse orientation */
+typedef struct orc_sphere_desc {
+    float o2w[16], o2w_inv[16];
+    float radius, z0, z1, phimax_deg;
+    int32_t reverse_orientation;
+} orc_sphere_desc;
+
 typedef struct orc_render_desc {
     const float* verts; uint32_t nverts;
     const uint32_t* idx; uint32_t nfaces;
@@ -142,8 +156,14 @@ typedef struct orc_render_desc {
        mean Filter::mean(0.5, 0.5) */
     int32_t filter_kind; float filter_xw, filter_yw, filter_p0, filter_p1;
     /* 0 StratifiedSampler (nx x ny, jittered), 1 LDSampler with nx*ny samples
-       per pixel rounded up to a power of two */
+       per pixel rounded up to a power of two, 2 HaltonSampler with nx*ny
+       samples per pixel over the image as one window (K slots a pixel, see
+       orc_halton_table) */
     int32_t sampler_kind;
+    /* spheres after the faces (prim ids nfaces + k), each with a material;
+       not emitters */
+    const orc_sphere_desc* spheres; uint32_t nspheres;
+    const uint32_t* sphere_mat;
 } orc_render_desc;
 
 /* LDSampler's pieces (sampler/utils.rs:5-32, lds.rs): van_der_corput and
@@ -163,6 +183,13 @@ void orc_ld_pixel_points(uint32_t seed, uint32_t pixel, uint32_t spp, float* pts
    tested by closest-hit queries (camera and extension rays), [7] nodes
    visited and [8] prims tested by occlusion queries up to their first hit
    (the work of an early-exit any-hit query).  nthreads >= 1. */
+/* radical_inverse (montecarlo.rs:8-21) */
+double orc_radical_inverse(uint64_t n, uint64_t b);
+/* HaltonSampler (halton.rs:48-107) over the whole image: *K = the most
+   samples any pixel received; cnt[xres*yres] = each pixel's samples; pos[2 *
+   (pixel*K + s)] = image position of its s-th sample in sequence order (pass
+   NULL to get K first). */
+int orc_halton_table(uint32_t xres, uint32_t yres, uint32_t spp, uint32_t* K, uint32_t* cnt, float* pos);
 /* SpecularReflection::sample_f (specular.rs:34-40) with Fresnel::noop, in the
    BSDF's local frame: wi, pdf and f for reflectance r. */
 void orc_mirror_sample_f(const float wo[3], const float r[3], float wi[3], float* pdf, float f[3]);

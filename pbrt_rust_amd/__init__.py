@@ -6,13 +6,14 @@ aggregate / renderer interfaces.
 """
 from ._lib import (HIT_DTYPE, NO_HIT, RAY_DTYPE, DeviceBuffer, PbrtxError, PinnedArray, device_name, lib,
                    synchronize)
-from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, Filter, LDSampler, PathIntegrator, Primitive,
+from .api import (Aggregate, BVHAccelerator, DirectLightingIntegrator, Filter, HaltonSampler, LDSampler,
+                  PathIntegrator, Primitive,
                   RayBatch,
                   RayServer, RenderServer, SamplerRenderer, Sphere, StratifiedSampler, Transform, TriangleMesh,
                   encode8, reset_tuning, set_tuning, write_image)
 
 __all__ = [
-    "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "Filter", "LDSampler", "PathIntegrator", "SamplerRenderer",
+    "Aggregate", "BVHAccelerator", "DirectLightingIntegrator", "Filter", "HaltonSampler", "LDSampler", "PathIntegrator", "SamplerRenderer",
     "StratifiedSampler", "DeviceBuffer", "HIT_DTYPE", "NO_HIT", "PbrtxError", "PinnedArray", "Primitive",
     "RAY_DTYPE", "RayBatch", "RayServer", "RenderServer", "Sphere", "Transform", "TriangleMesh", "device_name", "encode8", "lib", "reset_tuning", "set_tuning",
     "synchronize", "write_image",

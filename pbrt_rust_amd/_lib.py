@@ -43,6 +43,8 @@ class RenderDesc(ctypes.Structure):
         ("max_depth", ctypes.c_uint32), ("rr_start", ctypes.c_uint32),
         ("split", ctypes.c_int32), ("max_prims", ctypes.c_uint32),
         ("sampler", ctypes.c_int32),
+        ("spheres", ctypes.c_void_p), ("nspheres", ctypes.c_uint32),
+        ("sphere_mat", ctypes.c_void_p),
     ]
 
 
@@ -127,6 +129,7 @@ def lib():
     L.pbrtx_scene_create.argtypes = [vp, c.POINTER(vp)]
     L.pbrtx_scene_destroy.argtypes = [vp]
     L.pbrtx_scene_device_bytes.argtypes = [vp, c.POINTER(u64)]
+    L.pbrtx_scene_build_gpu.argtypes = [vp, u32, vp, u32, vp, u32, c.POINTER(vp), c.POINTER(c.c_double)]
     L.pbrtx_intersect.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_intersect_p.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
@@ -171,6 +174,7 @@ def lib():
         L.pbrtx_renderer_destroy.argtypes = [vp]
         L.pbrtx_renderer_sample_radiance.argtypes = [vp, vp, u64]
         L.pbrtx_renderer_write_image.argtypes = [vp, c.c_char_p]
+        L.pbrtx_renderer_sample_slots.argtypes = [vp, c.POINTER(u32)]
         L.pbrtx_write_image.argtypes = [c.c_char_p, vp, u32, u32]
         L.pbrtx_image_encode8.argtypes = [vp, u64, vp]
         L.pbrtx_renderer_set_chunk.argtypes = [vp, u32]

@@ -119,6 +119,20 @@ class Sphere:
         self.radius, self.z0, self.z1, self.phimax = float(radius), float(z0), float(z1), float(phimax)
 
 
+def sphere_descs(spheres):
+    """pbrtx_sphere records (a ctypes array) of Sphere shapes"""
+    sph = (_lib.SphereDesc * max(1, len(spheres)))()
+    for k, s in enumerate(spheres):
+        for i, v in enumerate(np.asarray(s.o2w.m, F32).reshape(16)):
+            sph[k].o2w[i] = float(v)
+        for i, v in enumerate(np.asarray(s.o2w.m_inv, F32).reshape(16)):
+            sph[k].o2w_inv[i] = float(v)
+        sph[k].radius, sph[k].z0, sph[k].z1 = s.radius, s.z0, s.z1
+        sph[k].phimax_deg = s.phimax
+        sph[k].reverse_orientation = int(s.reverse_orientation)
+    return sph
+
+
 class Primitive:
     """Primitive::simple (primitive/mod.rs:79-85): a shape in the aggregate."""
 
@@ -169,7 +183,11 @@ class BVHAccelerator:
     """GPU aggregate with the reference's BVHAccelerator::new signature."""
 
     def __init__(self, prims, max_prims=1, split_method="sah"):
-        split = {"sah": 0, "middle": 1, "equal": 2}.get(split_method)
+        """split_method "lbvh": the tree is built on the GPU (a linear BVH,
+        pbrtx_scene_build_gpu) straight into device memory; its closest hits
+        equal the reference tree's except among primitives at the same t,
+        its occlusion bits equal them exactly; no host tree is kept."""
+        split = {"sah": 0, "middle": 1, "equal": 2, "lbvh": 3}.get(split_method)
         if split is None:
             warnings.warn('BVH split method %s unknown. Using "SAH"' % split_method)  # bvh.rs:345-348
             split = 0
@@ -188,25 +206,28 @@ class BVHAccelerator:
             base += m.p.shape[0]
         self.verts = np.ascontiguousarray(np.concatenate(verts) if verts else np.zeros((0, 3), F32))
         self.idx = np.ascontiguousarray(np.concatenate(idx) if idx else np.zeros(0, np.uint32))
-        sph = (_lib.SphereDesc * max(1, len(spheres)))()
-        for k, s in enumerate(spheres):
-            for i, v in enumerate(s.o2w.m.reshape(16)):
-                sph[k].o2w[i] = float(v)
-            for i, v in enumerate(s.o2w.m_inv.reshape(16)):
-                sph[k].o2w_inv[i] = float(v)
-            sph[k].radius, sph[k].z0, sph[k].z1 = s.radius, s.z0, s.z1
-            sph[k].phimax_deg = s.phimax
-            sph[k].reverse_orientation = int(s.reverse_orientation)
+        sph = sphere_descs(spheres)
         L = lib()
         h = ctypes.c_void_p()
+        self.nfaces = self.idx.size // 3
+        self.nspheres = len(spheres)
+        self._scene = None
+        self._bvh = None
+        self.build_ms = None
+        if split == 3:
+            ms = ctypes.c_double()
+            check(L.pbrtx_scene_build_gpu(self.verts.ctypes.data if self.verts.size else None,
+                                          self.verts.shape[0], self.idx.ctypes.data if self.idx.size else None,
+                                          self.idx.size // 3, ctypes.cast(sph, ctypes.c_void_p) if spheres else None,
+                                          len(spheres), ctypes.byref(h), ctypes.byref(ms)))
+            self._scene = h.value
+            self.build_ms = ms.value
+            return
         check(L.pbrtx_bvh_build(self.verts.ctypes.data if self.verts.size else None,
                                 self.verts.shape[0], self.idx.ctypes.data if self.idx.size else None,
                                 self.idx.size // 3, ctypes.cast(sph, ctypes.c_void_p) if spheres else None,
                                 len(spheres), split, int(max_prims), ctypes.byref(h)))
         self._bvh = h.value
-        self.nfaces = self.idx.size // 3
-        self.nspheres = len(spheres)
-        self._scene = None
 
     # ---- host-side tree -------------------------------------------------------
     def info(self):
@@ -251,6 +272,9 @@ class BVHAccelerator:
         return r, d.value
 
     def world_bound(self):
+        if self._bvh is None:  # a device-built tree: the primitives' bounds
+            lo, hi = self.verts.min(axis=0), self.verts.max(axis=0)
+            return lo.astype(F32), hi.astype(F32)
         b = self.nodes()[0]
         return b[0, :3].copy(), b[0, 3:].copy()
 
@@ -538,6 +562,27 @@ class LDSampler:
         return self.xs
 
 
+class HaltonSampler:
+    """HaltonSampler::new(x0, x1, y0, y1, samples_per_pixel, ...) over the
+    whole image as one window (sampler/halton.rs:18-107): max(xres, yres)^2 *
+    samples_per_pixel points of the (base 3, base 2) Halton sequence scaled
+    to the square, those inside the image kept, so pixels receive varying
+    numbers of samples.  The renderer gives every pixel K slots (K the most
+    any pixel received); a pixel's samples fill its first slots in sequence
+    order and its other slots stay empty (zero radiance, no film weight).
+    seed keys the light and BSDF sample draws only."""
+    kind = 2
+
+    def __init__(self, samples_per_pixel, seed=0):
+        n = int(samples_per_pixel)
+        if n < 1:
+            raise ValueError("samples_per_pixel must be positive")
+        self.xs, self.ys, self.seed = n, 1, int(seed)
+
+    def maximum_sample_count(self):
+        return 1  # halton.rs:33
+
+
 class DirectLightingIntegrator:
     """Direct lighting: emitted light plus, per light, the mean of
     light_samples light samples (shadow rays) -- pbrt-v2 DirectLighting with
@@ -580,6 +625,15 @@ def _render_desc(scene, camera, sampler, integrator, split="sah"):
     d.area_le, d.n_area = (k["area_le"].ctypes.data if k["area_le"].size else None), k["area_le"].size // 3
     d.point_lights = k["point_lights"].ctypes.data if k["point_lights"].size else None
     d.n_point = k["point_lights"].size // 6
+    sph = scene.get("spheres")
+    if sph is not None and len(sph):
+        keep["spheres"] = sphere_descs(sph)
+        keep["sphere_mat"] = np.ascontiguousarray(scene["sphere_mat"], np.uint32)
+        if keep["sphere_mat"].size != len(sph):
+            raise ValueError("one material per sphere")
+        d.spheres = ctypes.cast(keep["spheres"], ctypes.c_void_p)
+        d.nspheres = len(sph)
+        d.sphere_mat = keep["sphere_mat"].ctypes.data
     if camera is not None:
         _set_camera(d, camera)
     if sampler is not None:
@@ -589,7 +643,7 @@ def _render_desc(scene, camera, sampler, integrator, split="sah"):
     d.light_samples = getattr(integrator, "light_samples", 1)
     d.max_depth = getattr(integrator, "max_depth", 5)
     d.rr_start = getattr(integrator, "rr_start", 3)
-    d.split = {"sah": 0, "middle": 1, "equal": 2}[split]
+    d.split = {"sah": 0, "middle": 1, "equal": 2, "lbvh": 3}[split]
     d.max_prims = 1
     return d, keep
 
@@ -673,10 +727,13 @@ class SamplerRenderer:
     def __init__(self, scene, camera, sampler, integrator, split="sah", chunk=None, filter=None):
         d, self._keep = _render_desc(scene, camera, sampler, integrator, split)
         self.desc = d
-        self.xres, self.yres, self.spp = d.xres, d.yres, d.nx * d.ny
+        self.xres, self.yres = d.xres, d.yres
         h = ctypes.c_void_p()
         check(lib().pbrtx_renderer_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h.value
+        k = ctypes.c_uint32()
+        check(lib().pbrtx_renderer_sample_slots(self._h, ctypes.byref(k)))
+        self.spp = k.value  # sample slots per pixel of the frame layout
         if chunk:
             check(lib().pbrtx_renderer_set_chunk(self._h, int(chunk)))
         if filter is not None:

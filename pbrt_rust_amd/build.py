@@ -18,7 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libpbrtx_gpu.so")
 
-SOURCES = ["bvh_build.cpp", "image_out.cpp", "trace.hip", "capi.hip", "render.hip", "serve.hip", "render_serve.hip"]
+SOURCES = ["bvh_build.cpp", "image_out.cpp", "trace.hip", "capi.hip", "render.hip", "serve.hip", "render_serve.hip",
+           "bvh_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PBRTX_ARCH", "gfx950")

@@ -12,7 +12,8 @@ struct CamParams {
     uint32_t xres, nx, spp, seed;
     float dx, dy;            // 1/nx, 1/ny
     uint64_t g0;             // global index of the launch's first sample
-    const float2* spos;      // low-discrepancy positions within the pixel, or null
+    const float2* spos;      // low-discrepancy or Halton positions within the pixel, or null
+    const uint32_t* scount;  // Halton: samples in each pixel's spp slots (the rest empty), or null
 };
 
 // ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
@@ -61,10 +62,16 @@ __device__ __forceinline__ void camera_ray(const P& p, float ix, float iy, Ray* 
     *r = Ray{o.x, o.y, o.z, 0.0f, dw.x, dw.y, dw.z, FLT_MAX};
 }
 
-// The camera ray of global sample g (pixel g / spp, sample g % spp).
+// The camera ray of global sample g (pixel g / spp, sample g % spp).  An
+// empty Halton slot gets an empty interval (tmin 1 > tmax 0): it meets
+// nothing, so its radiance is zero, and the film skips it.
 template <class P>
 __device__ __forceinline__ void camera_ray_of(const P& p, uint64_t g, Ray* r) {
     const uint32_t pixel = (uint32_t)(g / p.spp), s = (uint32_t)(g % p.spp);
+    if (p.scount && s >= p.scount[pixel]) {
+        *r = Ray{0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 1.0f, 0.0f};
+        return;
+    }
     float ix, iy;
     sample_pos(p, pixel % p.xres, pixel / p.xres, s, &ix, &iy);
     camera_ray(p, ix, iy, r);

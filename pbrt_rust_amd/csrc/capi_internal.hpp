@@ -18,6 +18,12 @@ struct DeviceScene {
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out);
+// the render description's split value for a tree built on the device
+constexpr int32_t SPLIT_GPU_LBVH = 3;
+// Builds a linear BVH over mesh faces then spheres on the device, straight
+// into S (bvh_gpu.hip); *build_ms = its wall time.
+int gpu_build_scene(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t nfaces, const SphereIn* sph,
+                    uint32_t nsph, DeviceScene* S, double* build_ms);
 int scene_from_host(const HostBvh& h, DeviceScene* s);
 void scene_free(DeviceScene* s);
 int run_closest(const SceneView& v, Ray* rays, uint32_t n, Hit* hits, int path, hipStream_t st);

@@ -18,7 +18,9 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "render.hpp"
@@ -105,6 +107,58 @@ __device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
 }
 
 __device__ __forceinline__ void set_dead(Ray* r) { *r = Ray{0, 0, 0, 1.0f, 0, 0, 1.0f, 0.0f}; }
+
+// The shading geometry of a hit: the point, dg.nn, the BSDF frame's tn =
+// normalize(dpdu) (bsdf/mod.rs:70-73), the material and the emitting area
+// light (-1 for none).  A face's come precomputed (FaceShade; the point is
+// ray.point_at(t), ray.rs:60).  A sphere's (Sphere::intersect,
+// sphere.rs:135-175, and compute_dg, helpers.rs:17-43): the object-space hit
+// of the ray at t, theta = acos(clamp(z / r)) and sin(theta) from the
+// device's acosf and sinf (the reference's come from the host C library, so
+// these may differ in the last place: sphere shading matches the reference
+// to a tolerance, not bit for bit), dp/du and dp/dv, the world point
+// o2w(p_hit) and dg.nn = normalize(o2w(dpdu) x o2w(dpdv)), flipped when
+// reverse_orientation ^ transform_swaps_handedness (diff_geom.rs:52-60).
+struct HitShade {
+    f3 p, nn, tn;
+    uint32_t mat;
+    int32_t light;
+};
+__device__ __forceinline__ void sphere_shade(const SphereShade& s, const Ray& r, float t, HitShade* out) {
+    const f3 o = xf_point(s.w2o, mk3(r.ox, r.oy, r.oz));
+    const f3 d = xf_vector(s.w2o, mk3(r.dx, r.dy, r.dz));
+    const f3 ph = add3(o, mul3(d, t));
+    float c = ph.z / s.radius;  // f32::clamp(-1, 1), a NaN kept
+    c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+    const float theta = acosf(c);
+    const float zradius = sqrtf(ph.x * ph.x + ph.y * ph.y);
+    const float inv_z = 1.0f / zradius;
+    const float cos_phi = ph.x * inv_z, sin_phi = ph.y * inv_z;
+    const f3 dpdu = mk3(-s.phimax * ph.y, s.phimax * ph.x, 0.0f);
+    const f3 dpdv = mul3(mk3(ph.z * cos_phi, ph.z * sin_phi, -s.radius * sinf(theta)), s.thetamax - s.thetamin);
+    out->p = xf_point(s.o2w, ph);
+    const f3 du = xf_vector(s.o2w, dpdu), dv = xf_vector(s.o2w, dpdv);
+    f3 n = normalize3(cross3(du, dv));
+    if (s.flip) n = mul3(n, -1.0f);
+    out->nn = n;
+    out->tn = normalize3(du);
+    out->mat = s.mat;
+    out->light = -1;
+}
+__device__ __forceinline__ HitShade hit_shade(const RenderParams& P, const Ray& r, const Hit& h) {
+    HitShade out;
+    if (h.prim >= P.nfaces) {
+        sphere_shade(P.sph[h.prim - P.nfaces], r, h.t, &out);
+        return out;
+    }
+    const FaceShade fs = P.faces[h.prim];
+    out.p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
+    out.nn = ld3(fs.nn);
+    out.tn = ld3(fs.tn);
+    out.mat = fs.mat;
+    out.light = fs.light;
+    return out;
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -263,14 +317,14 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
         Ray r;
         if (rays) r = rays[i];
         else camera_ray_of(P, g0 + i, &r);
-        p = add3(mk3(r.ox, r.oy, r.oz), mul3(mk3(r.dx, r.dy, r.dz), h.t));  // ray.rs:60
-        const FaceShade fs = P.faces[h.prim];
-        nn = ld3(fs.nn);
+        const HitShade hs = hit_shade(P, r, h);
+        p = hs.p;
+        nn = hs.nn;
         wo = neg3(mk3(r.dx, r.dy, r.dz));
-        eps = h.t * 5e-4f;  // mesh.rs:262
-        mat = P.mats[fs.mat];
+        eps = h.t * 5e-4f;  // mesh.rs:262, sphere.rs:174
+        mat = P.mats[hs.mat];
         f3 le = mk3(0, 0, 0);
-        if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+        if (hs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * hs.light);
         lbase[i] = make_float4(le.x, le.y, le.z, 0.0f);
         uint64_t g = g0 + i;
         pixel = (uint32_t)(g / P.spp); s = (uint32_t)(g % P.spp);
@@ -402,7 +456,7 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
 
 __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0, uint32_t n, const float4* lbase,
                                                      const float4* contrib, const uint32_t* qidx,
-                                                     const uint32_t* occ, float4* sample_l) {
+                                                     const uint8_t* occ, float4* sample_l) {
     for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
         const float4 lb = lbase[i];
         f3 L = mk3(lb.x, lb.y, lb.z);
@@ -413,7 +467,7 @@ __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0
             for (uint32_t j = 0; j < ns; ++j, ++k) {
                 const uint32_t q = qidx[(size_t)k * n + i];
                 if (q == NONE) continue;
-                if ((occ[q >> 5] >> (q & 31u)) & 1u) continue;
+                if (occ[q]) continue;  // occluded
                 const float4 c = contrib[q];
                 Ld = add3(Ld, mk3(c.x, c.y, c.z));
             }
@@ -514,19 +568,19 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
         if (fresh) camera_ray_of(P, g0 + i, &r);
         else r = rays_in[i];
         const f3 d = mk3(r.dx, r.dy, r.dz);
-        const f3 p = add3(mk3(r.ox, r.oy, r.oz), mul3(d, h.t));
-        const FaceShade fs = P.faces[h.prim];
-        const f3 nn = ld3(fs.nn);
+        const HitShade hs = hit_shade(P, r, h);
+        const f3 p = hs.p;
+        const f3 nn = hs.nn;
         const f3 wo = neg3(d);
         const float eps = h.t * 5e-4f;
         f3 beta = ld3(ps.beta), L = ld3(ps.L);
         const bool specular = (ps.flags & 1u) != 0u;
         if (bounce == 0 || specular) {
             f3 le = mk3(0, 0, 0);
-            if (fs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * fs.light);
+            if (hs.light >= 0 && dot3(nn, wo) > 0.0f) le = ld3(P.a_le + 3 * hs.light);
             L = add3(L, mulc(beta, le));
         }
-        const MatRec mat = P.mats[fs.mat];
+        const MatRec mat = P.mats[hs.mat];
         const SampleRef sr = sample_ref(P, g0, ps, sample_l);
         const uint32_t pixel = (uint32_t)(sr.g / sr.spp), s = (uint32_t)(sr.g % sr.spp);
         float ua[4], ub[4];
@@ -544,7 +598,7 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
         bool cont = bounce != P.max_depth;
         if (cont) {
             // BSDF frame (bsdf/mod.rs:69-84) and world_to_local (:103-105)
-            const f3 tn = ld3(fs.tn);
+            const f3 tn = hs.tn;
             const f3 sn = cross3(nn, tn);
             const f3 wol = mk3(dot3(wo, sn), dot3(wo, tn), dot3(wo, nn));
             f3 wil, f;
@@ -704,7 +758,7 @@ __global__ __launch_bounds__(VERTEX_B) void k_path_shade(RenderParams P, uint64_
 // Adds the unoccluded light samples of this bounce; finishes paths that ended.
 __global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0, uint32_t cap,
                                                     const uint32_t* sh_count, const uint32_t* sh_pid,
-                                                    const float4* shc, const uint32_t* occ, PathState* st,
+                                                    const float4* shc, const uint8_t* occ, PathState* st,
                                                     float4* sample_l) {
     const uint32_t n = sh_count ? *sh_count : cap;
     for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
@@ -715,7 +769,7 @@ __global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0,
             if (ps.flags & 4u) continue;   // plain form: finished earlier
             if (c.w == 0.0f) continue;     // no light sample this bounce
         }
-        if (!((occ[i >> 5] >> (i & 31u)) & 1u)) {
+        if (!occ[i]) {  // unoccluded
             ps.L[0] = ps.L[0] + c.x; ps.L[1] = ps.L[1] + c.y; ps.L[2] = ps.L[2] + c.z;
         }
         if (ps.flags & 2u) {
@@ -826,6 +880,10 @@ __device__ __forceinline__ bool covers(float ixf, int32_t x) {
 // order, as a gather over all nine neighbourhoods, without evaluating the
 // sample positions of neighbours that cannot contribute.
 constexpr int FT = 16;  // film tile side (FT * FT = RB threads)
+// the samples of a pixel the film adds: all spp, or a Halton pixel's count
+__device__ __forceinline__ uint32_t film_samples(const RenderParams& P, uint32_t pixel) {
+    return P.scount ? P.scount[pixel] : P.spp;
+}
 static_assert(FT * FT == RB, "one film tile per render block");
 __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sample_l, float* rgb) {
     __shared__ uint8_t s_edge[FT + 2][FT + 2];
@@ -834,7 +892,8 @@ __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sampl
         const int32_t qx = tx0 - 1 + (int32_t)(h % (FT + 2)), qy = ty0 - 1 + (int32_t)(h / (FT + 2));
         uint8_t e = 0;
         if (qx >= 0 && qx < (int32_t)P.xres && qy >= 0 && qy < (int32_t)P.yres) {
-            for (uint32_t s = 0; s < P.spp; ++s) {
+            const uint32_t ns = film_samples(P, (uint32_t)qy * P.xres + (uint32_t)qx);
+            for (uint32_t s = 0; s < ns; ++s) {
                 float ix, iy;
                 sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
                 if (ix == (float)qx) e |= 1u;
@@ -863,6 +922,7 @@ __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sampl
             if (!reach_x || !reach_y) continue;
             const uint32_t qpix = (uint32_t)qy * P.xres + (uint32_t)qx;
             const float4* sl = sample_l + (size_t)qpix * P.spp;
+            const uint32_t ns = film_samples(P, qpix);
             // rgb_to_xyz (spectrum.rs:37-41), weight 1
             auto add = [&](const float4 v) {
                 const float c0 = 0.412453f * v.x + 0.357580f * v.y + 0.180423f * v.z;
@@ -878,17 +938,17 @@ __global__ __launch_bounds__(RB) void k_film(RenderParams P, const float4* sampl
                 // every own sample covers the pixel: read them eight at a time
                 // (one 128-byte line a thread), add them in order
                 uint32_t s = 0;
-                for (; s + 8 <= P.spp; s += 8) {
+                for (; s + 8 <= ns; s += 8) {
                     float4 v[8];
 #pragma unroll
                     for (int k = 0; k < 8; ++k) v[k] = sl[s + k];
 #pragma unroll
                     for (int k = 0; k < 8; ++k) add(v[k]);
                 }
-                for (; s < P.spp; ++s) add(sl[s]);
+                for (; s < ns; ++s) add(sl[s]);
                 continue;
             }
-            for (uint32_t s = 0; s < P.spp; ++s) {
+            for (uint32_t s = 0; s < ns; ++s) {
                 float ix, iy;
                 sample_pos(P, (uint32_t)qx, (uint32_t)qy, s, &ix, &iy);
                 if (covers(ix, x) && covers(iy, y)) add(sl[s]);
@@ -947,6 +1007,64 @@ __global__ __launch_bounds__(RB) void k_ld_points(uint32_t seed, uint32_t npix, 
     }
 }
 
+// HaltonSampler (halton.rs:48-107) over the whole image as one window: sample
+// k of the sequence lies at (lerp(0, delta, rinv(k, 3)), lerp(0, delta,
+// rinv(k, 2))) and is kept when inside the image; a pixel's samples take its
+// slots in sequence order.  radical_inverse (montecarlo.rs:8-21) in f64,
+// Lerp for f32 (utils/mod.rs:15-18), as the reference computes them.
+__device__ double radical_inverse(uint64_t n, uint64_t b) {
+    double v = 0.0, aib = 1.0;
+    const double inv_base = 1.0 / (double)b;
+    while (n > 0) {
+        const double d = (double)(n % b);
+        n /= b;
+        aib *= inv_base;
+        v += d * aib;
+    }
+    return v;
+}
+__device__ __forceinline__ bool halton_point(uint64_t k, uint32_t xres, uint32_t yres, float delta, float* ix,
+                                             float* iy) {
+    const float u = (float)radical_inverse(k, 3), v = (float)radical_inverse(k, 2);
+    const float x0 = 0.0f, y0 = 0.0f;
+    *ix = x0 * (1.0f - u) + (x0 + delta) * u;
+    *iy = y0 * (1.0f - v) + (y0 + delta) * v;
+    return !(*ix >= (float)xres || *iy >= (float)yres);
+}
+// pass 0: count each pixel's samples; pass 1: give each its slot (any order)
+__global__ __launch_bounds__(RB) void k_halton_bin(uint64_t wanted, uint32_t xres, uint32_t yres, float delta,
+                                                   uint32_t* cnt, uint64_t* slots, uint32_t K) {
+    for (uint64_t k = (uint64_t)blockIdx.x * RB + threadIdx.x; k < wanted; k += (uint64_t)gridDim.x * RB) {
+        float ix, iy;
+        if (!halton_point(k, xres, yres, delta, &ix, &iy)) continue;
+        const uint32_t pixel = (uint32_t)iy * xres + (uint32_t)ix;
+        const uint32_t at = atomicAdd(cnt + pixel, 1u);
+        if (slots && at < K) slots[(size_t)pixel * K + at] = k;
+    }
+}
+// per pixel: its slots in sequence order, then each sample's position within
+// the pixel (exact: ix - px is exact for ix in [px, px + 1), and px + that
+// gives ix back); empty slots at 0
+__global__ __launch_bounds__(RB) void k_halton_pos(uint32_t npix, uint32_t xres, uint32_t yres, float delta,
+                                                   const uint32_t* cnt, uint64_t* slots, uint32_t K, float2* spos) {
+    for (uint32_t p = blockIdx.x * RB + threadIdx.x; p < npix; p += gridDim.x * RB) {
+        uint64_t* sl = slots + (size_t)p * K;
+        const uint32_t n = min(cnt[p], K);
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint64_t v = sl[i];
+            uint32_t j = i;
+            for (; j > 0 && sl[j - 1] > v; --j) sl[j] = sl[j - 1];
+            sl[j] = v;
+        }
+        const float px = (float)(p % xres), py = (float)(p / xres);
+        for (uint32_t s = 0; s < K; ++s) {
+            float ix = px, iy = py;
+            if (s < n) halton_point(sl[s], xres, yres, delta, &ix, &iy);
+            spos[(size_t)p * K + s] = make_float2(ix - px, iy - py);
+        }
+    }
+}
+
 __global__ __launch_bounds__(RB) void k_film_stage(RenderParams P, const float4* sample_l, uint64_t n) {
     const uint64_t npix = (uint64_t)P.xres * P.yres;
     for (uint64_t g = (uint64_t)blockIdx.x * RB + threadIdx.x; g < n; g += (uint64_t)gridDim.x * RB) {
@@ -982,7 +1100,8 @@ __global__ __launch_bounds__(RB) void k_film_filter(RenderParams P, float* rgb) 
     for (int32_t qy = max(0, y - P.fry); qy <= min(yres - 1, y + P.fry); ++qy) {
         for (int32_t qx = max(0, x - P.frx); qx <= min(xres - 1, x + P.frx); ++qx) {
             const size_t npix = (size_t)P.xres * P.yres, q = (size_t)qy * P.xres + (size_t)qx;
-            for (uint32_t s = 0; s < P.spp; ++s) {
+            const uint32_t ns = film_samples(P, (uint32_t)q);
+            for (uint32_t s = 0; s < ns; ++s) {
                 // sample_pos: the position within the pixel plus the pixel's coordinates
                 const float2 f = P.fpos[(size_t)s * npix + q];
                 const float ix = f.x + (float)qx, iy = f.y + (float)qy;
@@ -1199,7 +1318,7 @@ int grid_for(size_t n) {
 // ---- scene and camera set-up -------------------------------------------------
 void RenderScene::release() {
     scene_free(&scene);
-    faces.release(); mats.release(); atris.release(); a_first.release(); a_count.release();
+    faces.release(); sph.release(); mats.release(); atris.release(); a_first.release(); a_count.release();
     a_total.release(); a_le.release(); points.release();
 }
 
@@ -1216,13 +1335,49 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
         return fail(PBRTX_ERR_ARG, "light_samples must be >= 1");
     for (uint32_t f = 0; f < d->nfaces; ++f)
         if (d->face_mat[f] >= d->nmats) return fail(PBRTX_ERR_ARG, "face material out of range");
-    std::string err;
-    if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, nullptr, 0, d->split, d->max_prims, &S->bvh, &err))
-        return fail(PBRTX_ERR_BUILD, err);
-    S->build_ms = S->bvh.build_ms;
-    int rc = scene_from_host(S->bvh, &S->scene);
-    if (rc) return rc;
-    S->bvh = HostBvh{};  // the device copy is all that renders need
+    if (d->nspheres && (!d->spheres || !d->sphere_mat)) return fail(PBRTX_ERR_ARG, "null sphere arrays");
+    std::vector<SphereShade> ssh(d->nspheres);
+    for (uint32_t k = 0; k < d->nspheres; ++k) {
+        const pbrtx_sphere& q = d->spheres[k];
+        if (d->sphere_mat[k] >= d->nmats) return fail(PBRTX_ERR_ARG, "sphere material out of range");
+        if (!(q.radius > 0.0f)) return fail(PBRTX_ERR_ARG, "sphere radius must be positive");
+        // Sphere::new (sphere.rs:28-44) with Transform::inverse swapping m and
+        // m_inv (world2object), and swaps_handedness of object2world
+        // (transform.rs): the theta range from the host's acos, as the
+        // reference computes it at construction
+        SphereShade& h = ssh[k];
+        memcpy(h.w2o, q.o2w_inv, sizeof(h.w2o));
+        memcpy(h.o2w, q.o2w, sizeof(h.o2w));
+        const float r = q.radius;
+        const float zmin = std::min(std::max(fminf(q.z0, q.z1), -r), r);
+        const float zmax = std::min(std::max(fmaxf(q.z0, q.z1), -r), r);
+        h.radius = r;
+        h.thetamin = std::acos(zmin / r);
+        h.thetamax = std::acos(zmax / r);
+        h.phimax = std::min(std::max(q.phimax_deg, 0.0f), 360.0f) * PI_F / 180.0f;
+        const float* a = q.o2w;
+        const float det = a[0] * (a[5] * a[10] - a[6] * a[9]) - a[1] * (a[4] * a[10] - a[6] * a[8]) +
+                          a[2] * (a[4] * a[9] - a[5] * a[8]);
+        h.flip = ((q.reverse_orientation != 0) != (0.0f > det)) ? 1u : 0u;
+        h.mat = d->sphere_mat[k];
+        h.pad0 = h.pad1 = 0;
+    }
+    static_assert(sizeof(pbrtx_sphere) == sizeof(SphereIn), "sphere layout");
+    int rc;
+    if (d->split == SPLIT_GPU_LBVH) {
+        // the tree built on the device (a linear BVH, bvh_gpu.hip)
+        if ((rc = gpu_build_scene(d->verts, d->nverts, d->idx, d->nfaces, reinterpret_cast<const SphereIn*>(d->spheres),
+                                  d->nspheres, &S->scene, &S->build_ms)))
+            return rc;
+    } else {
+        std::string err;
+        if (!build_bvh(d->verts, d->nverts, d->idx, d->nfaces, reinterpret_cast<const SphereIn*>(d->spheres),
+                       d->nspheres, d->split, d->max_prims, &S->bvh, &err))
+            return fail(PBRTX_ERR_BUILD, err);
+        S->build_ms = S->bvh.build_ms;
+        if ((rc = scene_from_host(S->bvh, &S->scene))) return rc;
+        S->bvh = HostBvh{};  // the device copy is all that renders need
+    }
     S->nfaces = d->nfaces;
     auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
     std::vector<FaceShade> fsh(d->nfaces);
@@ -1271,7 +1426,8 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
         mats[m] = MatRec{(uint32_t)d->mats[4 * m], d->mats[4 * m + 1], d->mats[4 * m + 2], d->mats[4 * m + 3]};
     std::vector<float> pts(d->point_lights ? d->point_lights : (const float*)nullptr,
                            d->point_lights ? d->point_lights + 6 * d->n_point : (const float*)nullptr);
-    if ((rc = S->faces.upload(fsh)) || (rc = S->mats.upload(mats)) || (rc = S->atris.upload(at)) ||
+    if ((rc = S->faces.upload(fsh)) || (rc = S->sph.upload(ssh)) || (rc = S->mats.upload(mats)) ||
+        (rc = S->atris.upload(at)) ||
         (rc = S->a_first.upload(afirst)) || (rc = S->a_count.upload(acount)) || (rc = S->a_total.upload(atotal)) ||
         (rc = S->a_le.upload(ale)) || (rc = S->points.upload(pts)))
         return rc;
@@ -1281,6 +1437,8 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
     P.max_depth = d->max_depth; P.rr_start = d->rr_start;
     P.S = P.n_point + P.n_area * P.light_samples;
     P.faces = S->faces.p; P.mats = S->mats.p; P.atris = S->atris.p;
+    P.sph = d->nspheres ? S->sph.p : nullptr;
+    P.nfaces = d->nfaces;
     P.a_first = S->a_first.p; P.a_count = S->a_count.p; P.a_total = S->a_total.p; P.a_le = S->a_le.p;
     P.points = S->points.p;
     P.segs = nullptr; P.reqs = nullptr; P.nsegs = 0;
@@ -1306,6 +1464,7 @@ CamParams cam_params(const RenderParams& P, uint64_t g0) {
     cp.xres = P.xres; cp.nx = P.nx; cp.spp = P.spp; cp.seed = P.seed;
     cp.dx = P.dx; cp.dy = P.dy; cp.g0 = g0;
     cp.spos = P.spos;
+    cp.scount = P.scount;
     return cp;
 }
 
@@ -1339,7 +1498,7 @@ int reserve_path_work(WorkBufs& W, uint32_t cap, bool osort) {
     ICHK(W.rays2.ensure(cap));
     ICHK(W.sh.ensure(cap));
     ICHK(W.contrib.ensure(cap));
-    ICHK(W.occ.ensure((size_t)cap / 32 + 2));
+    ICHK(W.occ.ensure((size_t)cap + 64));
     ICHK(W.pid.ensure(cap));
     ICHK(W.pid2.ensure(cap));
     ICHK(W.shpid.ensure(cap));
@@ -1417,9 +1576,9 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         RCHK(sg.end(a, &Timing::other));
         a = sg.begin();
         if (batched)
-            RCHK(trace_any(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st, cnt + 5));
+            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st, cnt + 5));
         else
-            RCHK(trace_any(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st));
+            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st));
         RCHK(sg.end(a, &Timing::any));
         a = sg.begin();
         hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
@@ -1458,7 +1617,8 @@ struct Renderer {
     DevArr<uint32_t> counters;
     DevArr<float> rgb;
     DevArr<float> ftab;  // the film filter's weight table (none: box 0.5)
-    DevArr<float2> spos;  // LDSampler: the samples' positions within their pixels
+    DevArr<float2> spos;  // LDSampler, HaltonSampler: the samples' positions within their pixels
+    DevArr<uint32_t> scount;  // HaltonSampler: samples of each pixel
     DevArr<float2> fpos;  // and its staging of the frame's samples
     DevArr<float4> fxyz;
     hipStream_t stream = nullptr;
@@ -1483,6 +1643,49 @@ struct Renderer {
         }
     }
 };
+
+// HaltonSampler::new over the whole image (halton.rs:18-29): max(dx, dy)^2 *
+// spp sequence samples, those inside the image kept; K slots a pixel, K the
+// most any pixel received.
+static int halton_setup(Renderer* R, const pbrtx_render_desc* d) {
+    const uint32_t spp = d->nx * d->ny, xres = d->xres, yres = d->yres;
+    const uint64_t side = std::max(xres, yres), wanted = side * side * spp;
+    if (wanted >= (1ull << 36)) return fail(PBRTX_ERR_ARG, "Halton sampler: image too large");
+    const float dx = (float)xres, dy = (float)yres;
+    const float delta = dy > dx ? dy : dx;  // dy.max(dx)
+    const uint32_t npix = xres * yres;
+    ICHK(R->scount.ensure(npix));
+    hipStream_t st = R->stream;
+    RCHK(fill_u32(R->scount.p, 0u, npix, st));
+    const uint32_t gk = (uint32_t)std::min<uint64_t>((wanted + RB - 1) / RB, 65536u);
+    hipLaunchKernelGGL(k_halton_bin, dim3(gk), dim3(RB), 0, st, wanted, xres, yres, delta, R->scount.p,
+                       (uint64_t*)nullptr, 0u);
+    RCHK(hipGetLastError());
+    std::vector<uint32_t> cnt(npix);
+    RCHK(hipMemcpyAsync(cnt.data(), R->scount.p, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RCHK(hipStreamSynchronize(st));
+    uint32_t K = 0;
+    for (uint32_t c : cnt) K = std::max(K, c);
+    if (K == 0) return fail(PBRTX_ERR_ARG, "Halton sampler: no samples");
+    if ((uint64_t)npix * K >= (1ull << 40)) return fail(PBRTX_ERR_ARG, "image too large");
+    DevArr<uint64_t> slots;
+    ICHK(slots.ensure((size_t)npix * K));
+    ICHK(R->spos.ensure((size_t)npix * K));
+    RCHK(fill_u32(R->scount.p, 0u, npix, st));
+    hipLaunchKernelGGL(k_halton_bin, dim3(gk), dim3(RB), 0, st, wanted, xres, yres, delta, R->scount.p, slots.p, K);
+    RCHK(hipGetLastError());
+    const uint32_t gp = std::min<uint32_t>((npix + RB - 1) / RB, 65536u);
+    hipLaunchKernelGGL(k_halton_pos, dim3(gp), dim3(RB), 0, st, npix, xres, yres, delta, R->scount.p, slots.p, K,
+                       R->spos.p);
+    RCHK(hipGetLastError());
+    RCHK(hipStreamSynchronize(st));
+    slots.release();
+    R->P.nx = K; R->P.ny = 1; R->P.spp = K;
+    R->P.dx = 1.0f / (float)K; R->P.dy = 1.0f;
+    R->P.spos = R->spos.p;
+    R->P.scount = R->scount.p;
+    return PBRTX_OK;
+}
 
 static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
     if (!d->verts || !d->idx || !d->face_mat || !d->mats) return fail(PBRTX_ERR_ARG, "null scene arrays");
@@ -1511,8 +1714,10 @@ static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
         if (e == hipSuccess) e = hipStreamSynchronize(R->stream);
         if (e != hipSuccess) return hip_fail(e, "LDSampler positions");
         R->P.spos = R->spos.p;
+    } else if (d->sampler == 2) {
+        ICHK(halton_setup(R, d));
     } else if (d->sampler != 0) {
-        return fail(PBRTX_ERR_ARG, "unknown sampler (0 stratified, 1 low-discrepancy)");
+        return fail(PBRTX_ERR_ARG, "unknown sampler (0 stratified, 1 low-discrepancy, 2 Halton)");
     }
     R->n_samples = (uint64_t)d->xres * d->yres * R->P.spp;
     return PBRTX_OK;
@@ -1554,7 +1759,7 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         ICHK(W.contrib.ensure(slots ? slots : 1));
         ICHK(W.qidx.ensure(slots ? slots : 1));
         ICHK(W.lbase.ensure(n));
-        ICHK(W.occ.ensure((slots + 31) / 32 + 1));
+        ICHK(W.occ.ensure((size_t)slots + 64));
         hipEvent_t a;
         // batched: the packet kernel makes the camera rays itself and the
         // shade kernel makes them again (rays = null): no ray array
@@ -1590,9 +1795,9 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         if (slots) {
             a = sg.begin();
             if (batched)
-                RCHK(trace_any(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st, cnt + 5));
+                RCHK(trace_any_bytes(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st, cnt + 5));
             else
-                RCHK(trace_any(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st));
+                RCHK(trace_any_bytes(sv, W.sh.p, (uint32_t)slots, W.occ.p, path, tunA, scr, st));
             RCHK(sg.end(a, &Timing::any));
         }
         a = sg.begin();
@@ -1799,6 +2004,12 @@ int pbrtx_renderer_render(pbrtx_renderer* h, int32_t path, float* rgb_out, pbrtx
     return PBRTX_OK;
 }
 
+int pbrtx_renderer_sample_slots(pbrtx_renderer* h, uint32_t* slots_per_pixel) {
+    if (!h || !slots_per_pixel) return fail(PBRTX_ERR_ARG, "null argument");
+    *slots_per_pixel = h->r.P.spp;
+    return PBRTX_OK;
+}
+
 int pbrtx_renderer_write_image(pbrtx_renderer* h, const char* filename) {
     if (!h || !filename) return fail(PBRTX_ERR_ARG, "null argument");
     Renderer& R = h->r;
@@ -1828,7 +2039,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.drop_graphs();
     R.sc.release();
     R.W.release();
-    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release(); R.spos.release();
+    R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release(); R.spos.release(); R.scount.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
     delete h;
     return PBRTX_OK;

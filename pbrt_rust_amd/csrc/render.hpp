@@ -20,6 +20,14 @@ struct FaceShade {  // per face, precomputed with the hit-time formulas
     uint32_t mat;
     int32_t light;
 };
+// per sphere: what shading a hit needs (Sphere::intersect, sphere.rs:135-175)
+struct SphereShade {
+    float w2o[16], o2w[16];  // world->object and object->world m (row-major)
+    float radius, thetamin, thetamax, phimax;
+    uint32_t flip;           // reverse_orientation ^ transform_swaps_handedness
+    uint32_t mat;
+    uint32_t pad0, pad1;
+};
 struct AreaTriRec {
     float p1[3], p2[3], p3[3], nn[3];
     float area, pad0, pad1, pad2;
@@ -57,6 +65,8 @@ struct RenderParams {
     const float* a_total;
     const float* a_le;     // 3 per area light
     const float* points;   // 6 per point light
+    const SphereShade* sph;  // prims nfaces + k are spheres k
+    uint32_t nfaces;
     // served batches only (null for a frame): the batch's segments and the
     // requests they name; the camera fields above are then unused
     const RenderSeg* segs;
@@ -72,9 +82,12 @@ struct RenderParams {
     // position within its pixel and its xyz
     float2* fpos;
     float4* fxyz;
-    // LDSampler frames: each sample's position within its pixel (pixel-major);
-    // null for the stratified sampler
+    // LDSampler and HaltonSampler frames: each sample's position within its
+    // pixel (pixel-major); null for the stratified sampler
     const float2* spos;
+    // HaltonSampler frames: how many of each pixel's spp slots hold a sample
+    // (the rest are empty); null otherwise
+    const uint32_t* scount;
 };
 
 // Filter (filter.rs:4-124) of a film: kind 0 mean, 1 triangle, 2 gaussian
@@ -137,6 +150,7 @@ struct RenderScene {
     int integrator = 0;
     double build_ms = 0.0;
     DevArr<FaceShade> faces;
+    DevArr<SphereShade> sph;
     DevArr<MatRec> mats;
     DevArr<AreaTriRec> atris;
     DevArr<uint32_t> a_first, a_count;
@@ -156,7 +170,8 @@ struct WorkBufs {
     DevArr<Ray> rays, rays2, rays3, sh;
     DevArr<Hit> hits;
     DevArr<float4> contrib, lbase;
-    DevArr<uint32_t> qidx, occ, pid, pid2, pid3, shpid, bcounts;
+    DevArr<uint32_t> qidx, pid, pid2, pid3, shpid, bcounts;
+    DevArr<uint8_t> occ;  // occlusion of each queued shadow ray: a byte per ray
     DevArr<PathState> st;
     void release();
 };

@@ -21,6 +21,7 @@
 //              its children's boxes, and the rounded slab distances are
 //              monotone in the box bounds, so whenever a grandchild passes,
 //              its parent passes too (at the same or a larger tmax).
+
 #include "camera.hpp"
 #include "trace.hpp"
 
@@ -194,7 +195,7 @@ __device__ __forceinline__ void set_ray(const Ray& a, RayState& r) {
 // ---------------------------------------------------------------------------
 template <bool ANY>
 __global__ __launch_bounds__(MAX_BLOCK) void k_trace_plain(SceneView sv, Ray* rays, uint32_t n,
-                                                           Hit* hits, uint32_t* occ,
+                                                           Hit* hits, uint32_t* occ, uint8_t* occ8,
                                                            uint32_t* spill, uint32_t spill_levels,
                                                            uint32_t* err) {
     __shared__ uint32_t stk[STK_PLAIN][MAX_BLOCK];
@@ -256,7 +257,9 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_trace_plain(SceneView sv, Ray* ra
                 if (hit) rays[i].tmax = r.tmax;
             }
         }
-        if (ANY) {
+        if (ANY && occ8) {
+            if (i < n) occ8[i] = hit ? 1u : 0u;  // a byte per ray
+        } else if (ANY) {
             // 64 consecutive rays per wave: one ballot, two words
             const unsigned long long m = __ballot(hit);
             const uint32_t lane = __lane_id();
@@ -418,7 +421,7 @@ static_assert(WIDE_PARTS_AT + WIDE_PARTS * WIDE_PART_STRIDE <= TRACE_COUNTER_WOR
 template <bool ANY, bool SEG, bool CQ, int STK_WIDE>
 __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_trace_wide(SceneView sv, Ray* rays, uint32_t n_host,
                                                           const uint32_t* n_dev, Hit* hits,
-                                                          uint32_t* occ, uint32_t* counter,
+                                                          uint32_t* occ, uint8_t* occ8, uint32_t* counter,
                                                           uint2* spill, uint32_t spill_levels,
                                                           uint32_t refill, uint32_t* err,
                                                           const TraceSeg* segs, uint32_t nseg) {
@@ -559,6 +562,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                 }
                 if (!have_cur) {
                     // ray finished
+                    if (ANY && occ8) occ8[rid] = 0u;  // no hit
                     if (!ANY) {
                         Hit* hp = SEG ? segs[rseg].hits : hits;
                         Ray* rp = SEG ? segs[rseg].rays : rays;
@@ -733,8 +737,12 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
                 }
                 have_cur = false;
                 if (ANY && hitnow) {
-                    uint32_t* ow = SEG ? segs[rseg].occ : occ;
-                    atomicOr(ow + (rid >> 5), 1u << (rid & 31u));
+                    if (occ8) {
+                        occ8[rid] = 1u;  // a byte per ray: one plain store, no atomic
+                    } else {
+                        uint32_t* ow = SEG ? segs[rseg].occ : occ;
+                        atomicOr(ow + (rid >> 5), 1u << (rid & 31u));
+                    }
                     has = false;
                 }
             }
@@ -1151,13 +1159,13 @@ size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
 }
 
 template <bool ANY>
-static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, uint32_t* occ, int path,
-                         const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
+static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, uint32_t* occ, uint8_t* occ8,
+                         int path, const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
                          const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     const int block = tun.block;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
-    if (ANY) {
+    if (ANY && !occ8) {
         hipError_t e = fill_u32(occ, 0u, ((size_t)n + 31) / 32, st);
         if (e != hipSuccess) return e;
     }
@@ -1167,7 +1175,7 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
         const int grid = plain_grid(n, block);
         const uint32_t lv = plain_levels(s);
         if ((size_t)lv * grid * block * sizeof(uint32_t) > scr->spill_bytes) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_trace_plain<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, hits, occ,
+        hipLaunchKernelGGL(k_trace_plain<ANY>, dim3(grid), dim3(block), 0, st, s, rays, n, hits, occ, occ8,
                            (uint32_t*)scr->spill, lv, err);
         return hipGetLastError();
     }
@@ -1190,18 +1198,22 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ, parts,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, hits, occ, occ8, parts,
                        (uint2*)scr->spill, lv, refill, err, (const TraceSeg*)nullptr, 0u);
     return hipGetLastError();
 }
 
 hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, int path,
                          const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
-    return launch<false>(s, rays, n, hits, nullptr, path, tun, scr, st, n_dev);
+    return launch<false>(s, rays, n, hits, nullptr, nullptr, path, tun, scr, st, n_dev);
 }
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
-    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, path, tun, scr, st, n_dev);
+    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, occ, nullptr, path, tun, scr, st, n_dev);
+}
+hipError_t trace_any_bytes(const SceneView& s, const Ray* rays, uint32_t n, uint8_t* occ8, int path,
+                           const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
+    return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, nullptr, occ8, path, tun, scr, st, n_dev);
 }
 
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
@@ -1235,7 +1247,8 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
     uint32_t* err = scr->counter + 1;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total, (const uint32_t*)nullptr,
-                       (Hit*)nullptr, (uint32_t*)nullptr, parts, (uint2*)scr->spill, lv, refill, err, d_segs, nseg);
+                       (Hit*)nullptr, (uint32_t*)nullptr, (uint8_t*)nullptr, parts, (uint2*)scr->spill, lv, refill,
+                       err, d_segs, nseg);
     return hipGetLastError();
 }
 

@@ -64,6 +64,12 @@ hipError_t trace_closest(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, i
 hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* occ, int path,
                      const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
                      const uint32_t* n_dev = nullptr);
+// The same with a byte per ray (occ8[i] = 0 or 1, every byte of the
+// launch's rays written; no zeroing needed): plain stores instead of one
+// atomic per occluded ray.
+hipError_t trace_any_bytes(const SceneView& s, const Ray* rays, uint32_t n, uint8_t* occ8, int path,
+                           const TraceTuning& tun, TraceScratch* scr, hipStream_t st,
+                           const uint32_t* n_dev = nullptr);
 struct CamParams;
 // Closest hit of the camera rays of samples [cam.g0, cam.g0 + n), generated
 // inside the (packet) traversal kernel; only the hits are written.

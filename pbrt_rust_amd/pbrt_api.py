@@ -13,9 +13,9 @@ for pbrt's scene-file syntax, feeding the GPU renderer.
     renderer = p.world_end()          # a SamplerRenderer, ready to render
 
 or `load("scene.pbrt")` / `parse(text)`.  Supported: Camera "perspective";
-Film "image"; Sampler "stratified" (jittered) and "lowdiscrepancy";
+Film "image"; Sampler "stratified" (jittered), "lowdiscrepancy" and "halton";
 PixelFilter "box", "triangle", "gaussian", "mitchell", "sinc"; SurfaceIntegrator
-"directlighting" and "path"; Shape "trianglemesh"; Material "matte" and
+"directlighting" and "path"; Shape "trianglemesh" and "sphere"; Material "matte" and
 "mirror"; LightSource "point"; AreaLightSource "diffuse"; the transform
 directives and AttributeBegin/End, TransformBegin/End.  The reference's
 make_camera, make_renderer and parse_file are unimplemented (main.rs:200-202,
@@ -24,8 +24,8 @@ composed in float32 as transform.rs does (Matrix4x4::mul, Transform::rotate,
 look_at and the LU inverse of matrix4x4.rs:66-160), each product summed in
 index order, sin and cos from the C library as f32::sin/cos.  Beyond the
 reference's make_shape / make_material (main.rs:288-330: "sphere" and "matte"
-only): "trianglemesh" (the GPU renderer's shape) and "mirror" (its second
-material); spheres are refused, as the renderer takes triangles.
+only): "trianglemesh" and "mirror" (the renderer's second material);
+ReverseOrientation; spheres are not area lights.
 """
 import ctypes
 import ctypes.util
@@ -213,6 +213,8 @@ class Pbrt:
     def _reset_world(self):
         self.mat_state = ("matte", {})
         self.area_light = None
+        self.reverse_orientation = False
+        self.spheres, self.sphere_mat = [], []
         self.verts, self.idx, self.face_mat, self.face_light = [], [], [], []
         self.mats, self.area_le, self.points = [], [], []
         self.nverts = 0
@@ -262,8 +264,8 @@ class Pbrt:
         self.film_params = dict(params or {})
 
     def sampler(self, name, params=None):
-        if name not in ("stratified", "lowdiscrepancy"):
-            raise ValueError("sampler %r: \"stratified\" or \"lowdiscrepancy\"" % name)
+        if name not in ("stratified", "lowdiscrepancy", "halton"):
+            raise ValueError("sampler %r: \"stratified\", \"lowdiscrepancy\" or \"halton\"" % name)
         self.sampler_name, self.sampler_params = name, dict(params or {})
 
     def pixel_filter(self, name, params=None):
@@ -284,15 +286,20 @@ class Pbrt:
         self._reset_world()
 
     def attribute_begin(self):
-        self.stack.append((self.ctm, self.mat_state, self.area_light))
+        self.stack.append((self.ctm, self.mat_state, self.area_light, self.reverse_orientation))
 
     def attribute_end(self):
         if not self.stack:
             raise ValueError("unmatched AttributeEnd")
-        self.ctm, self.mat_state, self.area_light = self.stack.pop()
+        self.ctm, self.mat_state, self.area_light, self.reverse_orientation = self.stack.pop()
+
+    def reverse_orientation_toggle(self):
+        """ReverseOrientation: the graphics state's flag flips (pbrt-v2
+        GraphicsState::reverseOrientation, kept by attribute blocks)."""
+        self.reverse_orientation = not self.reverse_orientation
 
     def transform_begin(self):
-        self.stack.append((self.ctm, None, None))
+        self.stack.append((self.ctm, None, None, None))
 
     def transform_end(self):
         if not self.stack:
@@ -331,9 +338,29 @@ class Pbrt:
         kr = np.broadcast_to(np.asarray(p.get("Kr", [1, 1, 1]), F32), (3,))
         return [1.0] + list(kr)
 
+    def _material_index(self):
+        row = self._material_row()
+        if row not in self.mats:
+            self.mats.append(row)
+        return self.mats.index(row)
+
     def shape(self, name, params=None):
+        if name == "sphere":
+            # make_shape "sphere" (main.rs:314-323): radius 1, z range
+            # [-radius, radius], phimax 360 by default; object_to_world = CTM
+            if self.area_light is not None:
+                raise ValueError("sphere: area light spheres are not rendered (spheres are not emitters here)")
+            p = dict(params or {})
+            radius = float(_param(p, "radius", 1.0))
+            zmin = float(_param(p, "zmin", -radius))
+            zmax = float(_param(p, "zmax", radius))
+            phimax = float(_param(p, "phimax", 360.0))
+            o2w = api.Transform(self.ctm.m, self.ctm.m_inv)
+            self.spheres.append(api.Sphere(o2w, o2w.inverse(), self.reverse_orientation, radius, zmin, zmax, phimax))
+            self.sphere_mat.append(self._material_index())
+            return
         if name != "trianglemesh":
-            raise ValueError("shape %r: only \"trianglemesh\" is rendered on the GPU" % name)
+            raise ValueError("shape %r: \"trianglemesh\" or \"sphere\"" % name)
         p = dict(params or {})
         P = np.asarray(p["P"], F32).reshape(-1, 3)
         ind = np.asarray(p["indices"], np.int64).reshape(-1, 3)
@@ -341,10 +368,7 @@ class Pbrt:
             raise ValueError("trianglemesh: vertex index out of range")
         # Mesh::new pre-transforms the vertices to world space (mesh.rs:308)
         Pw = api.Transform(self.ctm.m).xf_points(P).astype(F32)
-        row = self._material_row()
-        if row not in self.mats:
-            self.mats.append(row)
-        mat = self.mats.index(row)
+        mat = self._material_index()
         light = -1
         if self.area_light is not None:
             self.area_le.append(list(self.area_light))
@@ -357,13 +381,15 @@ class Pbrt:
 
     def scene(self):
         """The scene dict of what the world block declared (see scenes.py)."""
-        if not self.verts:
+        if not self.verts and not self.spheres:
             raise ValueError("the world holds no shapes")
-        return dict(verts=np.concatenate(self.verts).astype(F32), idx=np.concatenate(self.idx).astype(np.uint32),
-                    face_mat=np.concatenate(self.face_mat), mats=np.asarray(self.mats, F32).reshape(-1, 4),
-                    face_light=np.concatenate(self.face_light),
+        cat = lambda xs, shape, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(shape, dt)  # noqa: E731
+        return dict(verts=cat(self.verts, (0, 3), F32), idx=cat(self.idx, (0, 3), np.uint32),
+                    face_mat=cat(self.face_mat, 0, np.uint32), mats=np.asarray(self.mats, F32).reshape(-1, 4),
+                    face_light=cat(self.face_light, 0, np.int32),
                     area_le=np.asarray(self.area_le, F32).reshape(-1, 3),
-                    point_lights=np.asarray(self.points, F32).reshape(-1, 6))
+                    point_lights=np.asarray(self.points, F32).reshape(-1, 6),
+                    spheres=list(self.spheres), sphere_mat=np.asarray(self.sphere_mat, np.uint32))
 
     def camera_dict(self):
         xres = int(_param(self.film_params, "xresolution", 640))
@@ -378,6 +404,8 @@ class Pbrt:
         p, seed = self.sampler_params, int(_param(self.sampler_params, "seed", 0))
         if self.sampler_name == "lowdiscrepancy":
             return api.LDSampler(int(_param(p, "pixelsamples", 4)), seed)
+        if self.sampler_name == "halton":
+            return api.HaltonSampler(int(_param(p, "pixelsamples", 4)), seed)
         if not bool(_param(p, "jitter", True)):
             raise ValueError("stratified sampler: only jittered strata are rendered")
         return api.StratifiedSampler(int(_param(p, "xsamples", 2)), int(_param(p, "ysamples", 2)), seed)
@@ -497,6 +525,7 @@ def parse(text, pbrt=None):
              "SurfaceIntegrator": p.surface_integrator, "Material": p.material, "LightSource": p.light_source,
              "AreaLightSource": p.area_light_source, "Shape": p.shape}
     blocks = {"AttributeBegin": p.attribute_begin, "AttributeEnd": p.attribute_end,
+              "ReverseOrientation": p.reverse_orientation_toggle,
               "TransformBegin": p.transform_begin, "TransformEnd": p.transform_end, "WorldBegin": p.world_begin}
     while i < len(toks):
         d = toks[i]

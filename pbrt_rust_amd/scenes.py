@@ -194,6 +194,39 @@ def spheres_scene(n_spheres=64, level=6, seed=2, mirror_frac=0.3, light_radiance
                 area_le=np.array([[light_radiance] * 3], F32), point_lights=np.zeros((0, 6), F32))
 
 
+def analytic_spheres_scene(n_spheres=12, seed=4, mirror_frac=0.3, light_radiance=10.0, partial=True):
+    """Sphere shapes (Shape::sphere) of radius U(0.3, 1) at U[-4,4] x
+    U[-1,3] x U[-4,4] above a ground quad (albedo 0.8), Lambertian (albedo
+    0.7) or mirror (0.9), under a downward-facing quad area light; with
+    partial, some spheres are clipped in z and phi, some scaled non-uniformly
+    or mirrored (reverse_orientation / a handedness-swapping transform)."""
+    from .api import Sphere, Transform
+    rng = np.random.default_rng(seed)
+    g = 1.2
+    ground = quad([-20, -g, -20], [-20, -g, 20], [20, -g, 20], [20, -g, -20])
+    light = quad([-1.5, 6.0, -1.5], [-1.5, 6.0, 1.5], [1.5, 6.0, 1.5], [1.5, 6.0, -1.5])
+    v, idx = _merge([ground, light])
+    mats = np.array([[0, 0.8, 0.8, 0.8], [0, 0.0, 0.0, 0.0], [0, 0.7, 0.7, 0.7], [1, 0.9, 0.9, 0.9]], F32)
+    spheres, smat = [], []
+    for k in range(n_spheres):
+        c = [rng.uniform(-4, 4), rng.uniform(-1, 3), rng.uniform(-4, 4)]
+        r = float(rng.uniform(0.3, 1.0))
+        xf = Transform.translate(c)
+        z0, z1, phimax, ro = -r, r, 360.0, False
+        if partial and k % 4 == 1:
+            z0, z1, phimax = -0.6 * r, 0.8 * r, 270.0
+        if partial and k % 4 == 2:
+            xf = xf * Transform.scale(1.0, 1.4, 0.8)
+        if partial and k % 4 == 3:
+            xf = xf * Transform.scale(-1.0, 1.0, 1.0)  # swaps handedness
+            ro = True
+        spheres.append(Sphere(xf, xf.inverse(), ro, r, z0, z1, phimax))
+        smat.append(3 if rng.uniform() < mirror_frac else 2)
+    return dict(verts=v, idx=idx, face_mat=np.array([0, 0, 1, 1], np.uint32), mats=mats,
+                face_light=np.array([-1, -1, 0, 0], np.int32), area_le=np.array([[light_radiance] * 3], F32),
+                point_lights=np.zeros((0, 6), F32), spheres=spheres, sphere_mat=np.array(smat, np.uint32))
+
+
 _DISPLACED_POOL = {}
 
 

@@ -40,6 +40,8 @@ class RenderDesc(ctypes.Structure):
         ("filter_kind", ctypes.c_int32), ("filter_xw", ctypes.c_float), ("filter_yw", ctypes.c_float),
         ("filter_p0", ctypes.c_float), ("filter_p1", ctypes.c_float),
         ("sampler_kind", ctypes.c_int32),
+        ("spheres", ctypes.c_void_p), ("nspheres", ctypes.c_uint32),
+        ("sphere_mat", ctypes.c_void_p),
     ]
 
 # film filters as (kind, x width, y width, p0, p1), filter.rs:44-86
@@ -78,6 +80,11 @@ def lib():
                                   ctypes.c_float, ctypes.c_float, fp]
     L.orc_filter_table.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    F32P]
+    L.orc_hit_box_misses.argtypes = [ctypes.c_void_p, ctypes.c_uint32, F32P, F32P, F32P, F32P, U32P]
+    L.orc_radical_inverse.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.orc_radical_inverse.restype = ctypes.c_double
+    L.orc_halton_table.argtypes = [ctypes.c_uint32] * 3 + [ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p,
+                                                          ctypes.c_void_p]
     L.orc_mirror_sample_f.argtypes = [F32P, F32P, F32P, F32P, F32P]
     L.orc_mirror_sample_f.restype = None
     L.orc_ld_draw.argtypes = [ctypes.c_uint32] * 3
@@ -203,6 +210,17 @@ class OracleScene:
         self.L.orc_intersect_p(self.h, n, o, d, mint, maxt, occ, st, nthreads)
         return occ, st
 
+    def box_misses(self, o, d, mint, maxt):
+        """per ray: prims it hits whose own box the slab test rejects at
+        maxt = the hit's t (the order-sensitive cases of the traversal)"""
+        o = f32(o).reshape(-1); d = f32(d).reshape(-1)
+        n = o.size // 3
+        mint = f32(np.broadcast_to(mint, (n,)))
+        maxt = f32(np.broadcast_to(maxt, (n,)))
+        out = np.zeros(n, np.uint32)
+        self.L.orc_hit_box_misses(self.h, n, o, d, mint, maxt, out)
+        return out
+
     def brute(self, o, d, mint, maxt):
         o = f32(o).reshape(-1); d = f32(d).reshape(-1)
         n = o.size // 3
@@ -245,6 +263,15 @@ def make_render_desc(scene, cam, integ, keep):
     d.rr_start = integ.get("rr_start", 3)
     d.split = 0
     d.max_prims = 1
+    sph = scene.get("spheres")
+    if sph is not None and len(sph):
+        from pbrt_rust_amd.api import sphere_descs
+        arr = sphere_descs(sph)
+        mats = u32(scene["sphere_mat"])
+        keep.update(sph_arr=arr, sph_mat=mats)
+        d.spheres = ctypes.cast(arr, ctypes.c_void_p)
+        d.nspheres = len(sph)
+        d.sphere_mat = mats.ctypes.data
     return d
 
 
@@ -264,6 +291,8 @@ def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False, fil
     spp = integ["nx"] * integ["ny"]
     if integ.get("sampler", 0) == 1:
         spp = 1 << (spp - 1).bit_length()  # LDSampler rounds up to a power of two
+    elif integ.get("sampler", 0) == 2:
+        spp = halton_table(cam["xres"], cam["yres"], spp)[0]  # K slots a pixel
     samples = np.zeros((y1 - y0) * cam["xres"] * spp * 3, np.float32) if want_samples else None
     r = L.orc_render(ctypes.byref(d), y0, y1, nthreads, out,
                      samples.ctypes.data if want_samples else None, st)
@@ -272,3 +301,15 @@ def render(scene, cam, integ, y0=0, y1=None, nthreads=1, want_samples=False, fil
     if want_samples:
         res["samples"] = samples.reshape(y1 - y0, cam["xres"], spp, 3)
     return res
+
+
+def halton_table(xres, yres, spp):
+    """(K, counts[yres, xres], positions[yres, xres, K, 2]) of the
+    HaltonSampler over the whole image"""
+    L = lib()
+    K = ctypes.c_uint32()
+    assert L.orc_halton_table(xres, yres, spp, ctypes.byref(K), None, None) == 0
+    cnt = np.zeros(xres * yres, np.uint32)
+    pos = np.zeros(xres * yres * K.value * 2, np.float32)
+    assert L.orc_halton_table(xres, yres, spp, ctypes.byref(K), cnt.ctypes.data, pos.ctypes.data) == 0
+    return K.value, cnt.reshape(yres, xres), pos.reshape(yres, xres, K.value, 2)

@@ -119,7 +119,7 @@ def test_defaults_and_low_discrepancy():
 
 
 @pytest.mark.parametrize("text", [
-    'WorldBegin Shape "sphere" "float radius" 1 WorldEnd',
+    'WorldBegin AreaLightSource "diffuse" Shape "sphere" "float radius" 1 WorldEnd',
     'WorldBegin AttributeEnd',
     'Bogus 1 2 3',
     'WorldBegin Shape "trianglemesh" "point P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 3] WorldEnd',
@@ -138,3 +138,28 @@ def test_transform_file_convention():
     q = Recorder()
     q.concat_transform([1, 0, 0, 5, 0, 1, 0, 6, 0, 0, 1, 7, 0, 0, 0, 1])
     assert np.array_equal(p.ctm.m, q.ctm.m)
+
+
+def test_sphere_shapes_and_reverse_orientation():
+    # make_shape "sphere" (main.rs:314-323): defaults radius 1, z range
+    # [-r, r], phimax 360; object_to_world is the CTM; ReverseOrientation is
+    # kept by attribute blocks
+    text = """WorldBegin
+    AttributeBegin
+      Translate 1 2 3
+      ReverseOrientation
+      Material "mirror" "rgb Kr" [ 0.5 0.5 0.5 ]
+      Shape "sphere" "float radius" 2 "float zmin" -1 "float phimax" 180
+    AttributeEnd
+    Shape "sphere"
+    WorldEnd"""
+    p = pa.parse(text, Recorder())
+    sc = p.scene()
+    assert sc["idx"].shape == (0, 3) and len(sc["spheres"]) == 2
+    a, b = sc["spheres"]
+    assert (a.radius, a.z0, a.z1, a.phimax, a.reverse_orientation) == (2.0, -1.0, 2.0, 180.0, True)
+    assert (b.radius, b.z0, b.z1, b.phimax, b.reverse_orientation) == (1.0, -1.0, 1.0, 360.0, False)
+    assert np.array_equal(a.o2w.m[:3, 3], [1, 2, 3]) and np.array_equal(a.o2w.m_inv[:3, 3], [-1, -2, -3])
+    assert np.array_equal(b.o2w.m, np.eye(4, dtype=F32))
+    assert sc["mats"][sc["sphere_mat"][0]].tolist() == [1, 0.5, 0.5, 0.5]
+    assert sc["mats"][sc["sphere_mat"][1]].tolist() == [0, 0.5, 0.5, 0.5]
