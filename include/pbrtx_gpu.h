@@ -123,6 +123,19 @@ int pbrtx_intersect(pbrtx_scene* scene, pbrtx_ray* d_rays, uint32_t n, pbrtx_hit
 int pbrtx_intersect_p(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
                       uint32_t* d_occluded, int32_t path, void* stream);
 
+/* pbrtx_intersect_p on the batched path without allocating: no counterpart
+   in the reference.  For callers that cannot call hipMalloc (code under HIP
+   graph capture, tight serving loops, harnesses that forbid allocation).  Its
+   scratch (work counters, spilled traversal stacks) is a fixed 64 MiB device
+   array per GPU, held until the process ends; the persistent grid is cut to
+   the blocks whose spill stacks fit it.  Fails with PBRTX_ERR_ARG when one
+   block's spill does not fit (with 256-thread blocks, a tree of more than
+   about 10,900 record levels).
+   Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+   (different streams or threads) race. */
+int pbrtx_intersect_p_noalloc(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint32_t n,
+                              uint32_t* d_occluded, void* stream);
+
 /* Launch shape of a path's closest-hit (kind 0), occlusion (kind 1) or
    renderer camera-ray (kind 2: closest hit of the coherent camera rays)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum

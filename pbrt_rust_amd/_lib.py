@@ -132,6 +132,7 @@ def lib():
     L.pbrtx_scene_build_gpu.argtypes = [vp, u32, vp, u32, vp, u32, c.POINTER(vp), c.POINTER(c.c_double)]
     L.pbrtx_intersect.argtypes = [vp, vp, u32, vp, i32, vp]
     L.pbrtx_intersect_p.argtypes = [vp, vp, u32, vp, i32, vp]
+    L.pbrtx_intersect_p_noalloc.argtypes = [vp, vp, u32, vp, vp]
     L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
     L.pbrtx_set_stack_depth.argtypes = [i32, i32, i32]
     L.pbrtx_reset_tuning.argtypes = []

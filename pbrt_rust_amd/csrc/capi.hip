@@ -309,6 +309,28 @@ int pbrtx_intersect_p(pbrtx_scene* s, const pbrtx_ray* d_rays, uint32_t n, uint3
     return run_any(s->s.view, reinterpret_cast<const Ray*>(d_rays), n, d_occ, path, (hipStream_t)stream);
 }
 
+int pbrtx_intersect_p_noalloc(pbrtx_scene* s, const pbrtx_ray* d_rays, uint32_t n, uint32_t* d_occ, void* stream) {
+    if (!s) return fail(PBRTX_ERR_ARG, "scene is null");
+    if (n == 0) return PBRTX_OK;
+    if (!d_rays || !d_occ) return fail(PBRTX_ERR_ARG, "null ray or bit buffer");
+    const TraceTuning& tun = g_tuning[PATH_BATCHED][1];
+    const size_t per_block = trace_any_block_spill(s->s.view, tun);
+    if (per_block > NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES)
+        return fail(PBRTX_ERR_ARG, "scene too deep for the fixed scratch: one block's spill stacks need " +
+                                       std::to_string(per_block) + " bytes, the scratch holds " +
+                                       std::to_string(NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES));
+    TraceScratch scr;
+    hipError_t e = hipSuccess;
+    if (noalloc_scratch(&scr, &e)) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    hipStream_t st = (hipStream_t)stream;
+    // the array keeps the last call's contents: the counter block (work
+    // counters, overflow flag) starts at zero
+    e = fill_u32(scr.counter, 0u, TRACE_COUNTER_WORDS, st);
+    if (e == hipSuccess) e = trace_any(s->s.view, reinterpret_cast<const Ray*>(d_rays), n, d_occ, PATH_BATCHED, tun, &scr, st);
+    if (e != hipSuccess) return hip_fail(e, "trace_any (fixed scratch)");
+    return PBRTX_OK;
+}
+
 int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_per_cu, int32_t refill_pct,
                      int32_t scheme) {
     if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");

@@ -21,6 +21,7 @@
 //              its children's boxes, and the rounded slab distances are
 //              monotone in the box bounds, so whenever a grandchild passes,
 //              its parent passes too (at the same or a larger tmax).
+#include <mutex>
 
 #include "camera.hpp"
 #include "trace.hpp"
@@ -1140,6 +1141,10 @@ static bool use_compressed(const SceneView& s, const TraceTuning& tun) {
     return s.cnodes != nullptr && tun.scheme != SCHEME_RAY_EXACT;
 }
 
+size_t trace_any_block_spill(const SceneView& s, const TraceTuning& tun) {
+    return (size_t)wide_levels(s, tun.stack) * tun.block * sizeof(uint2);
+}
+
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun) {
     const int block = tun.block;
     if (path == PATH_PLAIN) {
@@ -1191,9 +1196,14 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
     if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
     const bool cq = use_compressed(s, tun);
     const auto kern = cq ? wide_kernel<ANY, false, true>(tun.stack) : wide_kernel<ANY, false, false>(tun.stack);
-    const int grid = wide_grid(kern, tun);
+    int grid = wide_grid(kern, tun);
     const uint32_t lv = wide_levels(s, tun.stack);
-    if ((size_t)lv * grid * block * sizeof(uint2) > scr->spill_bytes) return hipErrorInvalidValue;
+    const size_t per_block = (size_t)lv * block * sizeof(uint2);
+    if (per_block * grid > scr->spill_bytes) {
+        // a fixed scratch: as many persistent blocks as its spill holds
+        if (!scr->cap_grid || per_block > scr->spill_bytes) return hipErrorInvalidValue;
+        grid = (int)(scr->spill_bytes / per_block);
+    }
     uint32_t* parts = scr->counter + WIDE_PARTS_AT;
     hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
@@ -1214,6 +1224,34 @@ hipError_t trace_any(const SceneView& s, const Ray* rays, uint32_t n, uint32_t* 
 hipError_t trace_any_bytes(const SceneView& s, const Ray* rays, uint32_t n, uint8_t* occ8, int path,
                            const TraceTuning& tun, TraceScratch* scr, hipStream_t st, const uint32_t* n_dev) {
     return launch<true>(s, const_cast<Ray*>(rays), n, nullptr, nullptr, occ8, path, tun, scr, st, n_dev);
+}
+
+// Scratch of the entry point that allocates nothing: the trace counter block
+// (first NOALLOC_COUNTER_BYTES) and the spill stacks (the rest), one array per
+// GPU for the life of the process.  Host code uses the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch), never
+// the array's name.
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+int noalloc_scratch(TraceScratch* out, hipError_t* err) {
+    static std::mutex mu;
+    static void* ptr[NOALLOC_MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { *err = e; return -1; }
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { *err = hipErrorInvalidDevice; return -1; }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) { *err = e != hipSuccess ? e : hipErrorInvalidSymbol; return -1; }
+        ptr[dev] = p;
+    }
+    out->counter = (uint32_t*)ptr[dev];
+    out->spill = (uint8_t*)ptr[dev] + NOALLOC_COUNTER_BYTES;
+    out->spill_bytes = NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES;
+    out->cap_grid = true;
+    return 0;
 }
 
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,

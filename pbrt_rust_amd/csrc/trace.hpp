@@ -50,7 +50,23 @@ struct TraceScratch {
     void* spill = nullptr;       // spill stack entries
     size_t spill_bytes = 0;
     uint32_t* counter = nullptr; // TRACE_COUNTER_WORDS words, device
+    // a fixed-size scratch: the batched path's persistent grid is cut to the
+    // blocks whose spill stacks fit spill_bytes (instead of failing)
+    bool cap_grid = false;
 };
+
+// The fixed scratch of the entry point that allocates nothing: one device
+// array per GPU of NOALLOC_SCRATCH_BYTES, the counter block first.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 4096;
+static_assert(TRACE_COUNTER_WORDS * 4 <= NOALLOC_COUNTER_BYTES, "counter block");
+constexpr int NOALLOC_MAX_DEVICES = 64;
+// *out = the current device's fixed scratch (its address looked up once per
+// device); 0, or -1 with *err set.
+int noalloc_scratch(TraceScratch* out, hipError_t* err);
+// spill bytes one persistent block of the batched occlusion kernel needs on
+// this scene (0: none)
+size_t trace_any_block_spill(const SceneView& s, const TraceTuning& tun);
 
 // Launch helpers; return hipSuccess or the launch error.
 // closest: hits[i] for each ray, and rays[i].tmax = t on a hit.
