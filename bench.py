@@ -22,6 +22,20 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# HIP maps every stream of a process onto GPU_MAX_HW_QUEUES hardware queues
+# (4 by default).  The request servers overlap uploads, traversal and
+# downloads on three streams, and the renderer holds streams of its own: with
+# 4 queues the server's compute stream can share a queue with its copy
+# streams, whose waits then hold its kernels back (at 70% of the saturated
+# rate the ray server fell behind: 0.91 of the offered rate, p50 161 ms;
+# with 8 queues 1.00, p50 2.9 ms).  Read by the HIP runtime when it starts,
+# so raised to 8 before the library loads (a larger setting is kept).
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _hwq = 4
+if _hwq < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32
@@ -195,6 +209,8 @@ def main():
     gpu_img, _ = r.render("batched")
     band = cpu_band(scene, cam, dict(kind="direct", nx=nx, ny=ny, seed=seed, light_samples=4), args.cpu_seconds)
     gpu_samples = r.sample_radiance(band[1] * cam["xres"] * nx * ny)[band[0] * cam["xres"] * nx * ny:]
+    # the headline renderer's streams and buffers are not needed past here
+    r.close()
     st = sts_b[-1]
     rays = st["camera_rays"] + st["shadow_rays"]
     ms_b = float(np.mean([s["ms_total"] for s in sts_b]))
@@ -348,7 +364,6 @@ def main():
         "device": px.device_name(),
     }
     print(json.dumps(out), flush=True)
-    r.close()
 
 
 def frame_rate(px, r, path, steps, warmup):
@@ -487,6 +502,7 @@ def serving(args, px, scenes, log_fn):
                 ready.release()
 
         th = threading.Thread(target=producer)
+        c0 = srv.counters()
         t = time.perf_counter()
         th.start()
         lat = []
@@ -496,11 +512,16 @@ def serving(args, px, scenes, log_fn):
             lat.append(st["latency_ms"])
         th.join()
         dt = time.perf_counter() - t
+        c1 = srv.counters()
         rays = sum(int(sizes[i % len(pool)]) for i in range(nreq))
         srv.close()
         lat = np.array(lat)
+        k = max(1, nreq // 10)
         return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "mean_ms": round(float(lat.mean()), 3), "achieved_mrays_per_s": round(rays / dt / 1e6, 2)}
+                "mean_ms": round(float(lat.mean()), 3), "achieved_mrays_per_s": round(rays / dt / 1e6, 2),
+                "gpu_busy_share": round((c1["busy_ms"] - c0["busy_ms"]) / (dt * 1e3), 3),
+                "steps": int(c1["steps"] - c0["steps"]), "launches": int(c1["launches"] - c0["launches"]),
+                "p50_ms_by_arrival_decile": [round(float(np.median(lat[i:i + k])), 2) for i in range(0, nreq, k)][:10]}
 
     sat_b, passes_b = saturated("batched")
     sat_p, passes_p = saturated("plain")
@@ -520,6 +541,7 @@ def serving(args, px, scenes, log_fn):
         "saturated_mrays_per_s": {"batched": round(sat_b, 2), "plain": round(sat_p, 2)},
         "saturated_passes_mrays_per_s": {"batched": passes_b, "plain": passes_p},
         "served_over_plain": round(sat_b / sat_p, 3),
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
         "rate_mrays_per_s": round(rate, 2),
         "rate_requests_per_s": round(req_per_s, 2),
         "requests": nreq,
@@ -529,8 +551,9 @@ def serving(args, px, scenes, log_fn):
 
 def render_serving(args, px, scenes, log_fn):
     """Render-request serving: 16 resident scenes of 15 level-6 displaced
-    icospheres each (1.23M triangles, 19.7M in all) plus one 64M-triangle
-    scene of 782 level-6 displaced icospheres (left out with --rs-small);
+    icospheres each (1.23M triangles, 19.7M in all; host SAH trees) plus one
+    64M-triangle scene of 782 level-6 displaced icospheres whose tree is built
+    on the GPU at admission (left out with --rs-small);
     requests of square images with side log-uniform in 128..2048, spp from
     {1, 4, 16, 64}, path depth 5 with roulette after bounce 3, a look-at
     camera on a sphere around the scene.  Saturated throughput with every
@@ -549,12 +572,19 @@ def render_serving(args, px, scenes, log_fn):
         defs.append((782, 99, 40.0))
     tris = [0] * len(defs)
 
+    admit_s = [0.0] * len(defs)
+
     def add(k):
+        # the 64M-triangle scene's tree is built on the GPU (linear BVH), the
+        # others' on the host (the reference's SAH build)
         n, seed, ext = defs[k]
         sc = scenes.sphere_field(n, seed=seed, extent=ext)
         tris[k] = int(sc["idx"].shape[0])
-        sid = srv.add_scene(sc, integ)
-        log_fn("render serving: scene %d (%d triangles) resident after %.1fs" % (sid, tris[k], time.time() - t0))
+        ta = time.perf_counter()
+        sid = srv.add_scene(sc, integ, split="lbvh" if k == 16 else "sah")
+        admit_s[k] = time.perf_counter() - ta
+        log_fn("render serving: scene %d (%d triangles) resident after %.1fs (add_scene %.2fs)"
+               % (sid, tris[k], time.time() - t0, admit_s[k]))
         return sid
 
     scenes.displaced_pool()
@@ -658,6 +688,11 @@ def render_serving(args, px, scenes, log_fn):
         "latency": {"batched": lat_b, "plain": lat_p},
         "batch_paths": args.rs_batch,
         "scene_bytes": int(c["scene_bytes"]),
+        "add_scene_s": {"small_scenes_max": round(max(admit_s[:16]), 3),
+                        "large_scene_gpu_tree": None if args.rs_small else round(admit_s[16], 3),
+                        "note": "wall time of add_scene: tree build, uploads and the shading tables; the large "
+                                "scene's tree is built on the GPU (linear BVH), the small ones' on the host (SAH, "
+                                "four host builds at a time)"},
     }
 
 

@@ -1328,6 +1328,65 @@ void WorkBufs::release() {
     shpid.release(); bcounts.release(); st.release();
 }
 
+// Triangle::intersect's dg with the default uvs (mesh.rs:212-254) for the
+// face (p1, p2, p3) = the vertices of idx[2], idx[1], idx[0]: dg.nn =
+// normalize(dpdu x dpdv) and normalize(dpdu) (bsdf/mod.rs:70-73).  The
+// host's area-light records and the device's face records use it alike.
+PX_HD void face_frame(f3 p1, f3 p2, f3 p3, f3* nn, f3* tn) {
+    const float du1 = 0.0f - 1.0f, du2 = 1.0f - 1.0f, dv1 = 0.0f - 1.0f, dv2 = 0.0f - 1.0f;
+    const f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
+    const float det = du1 * dv2 - dv1 * du2;
+    const float inv_det = 1.0f / det;
+    const f3 dpdu = mul3(sub3(mul3(dp1, dv2), mul3(dp2, dv1)), inv_det);
+    const f3 dpdv = mul3(add3(mul3(dp1, -du2), mul3(dp2, du1)), inv_det);
+    *nn = normalize3(cross3(dpdu, dpdv));
+    *tn = normalize3(dpdu);
+}
+
+__global__ __launch_bounds__(RB) void k_face_shade(const float* verts, const uint32_t* idx, uint32_t nfaces,
+                                                   const uint32_t* face_mat, const int32_t* face_light,
+                                                   FaceShade* out) {
+    for (uint32_t f = blockIdx.x * RB + threadIdx.x; f < nfaces; f += gridDim.x * RB) {
+        auto V = [&](uint32_t i) { return mk3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]); };
+        const f3 p1 = V(idx[3 * f + 2]), p2 = V(idx[3 * f + 1]), p3 = V(idx[3 * f]);
+        f3 nn, tn;
+        face_frame(p1, p2, p3, &nn, &tn);
+        FaceShade sh;
+        sh.nn[0] = nn.x; sh.nn[1] = nn.y; sh.nn[2] = nn.z;
+        sh.tn[0] = tn.x; sh.tn[1] = tn.y; sh.tn[2] = tn.z;
+        sh.mat = face_mat[f];
+        sh.light = face_light ? face_light[f] : -1;
+        out[f] = sh;
+    }
+}
+
+// The per-face shading records, computed on the device from uploaded copies
+// of the mesh (the host loop over every face took seconds on 64M faces).
+static int face_shade_device(const pbrtx_render_desc* d, DevArr<FaceShade>* faces) {
+    int rc = faces->ensure(d->nfaces ? d->nfaces : 1);
+    if (rc || d->nfaces == 0) return rc;
+    DevArr<float> v;
+    DevArr<uint32_t> ix, fm;
+    DevArr<int32_t> fl;
+    auto put = [](auto* arr, const auto* src, size_t count) -> int {
+        int r = arr->ensure(count);
+        if (r) return r;
+        hipError_t e = hipMemcpy(arr->p, src, count * sizeof(*src), hipMemcpyHostToDevice);
+        return e == hipSuccess ? PBRTX_OK : hip_fail(e, "hipMemcpy(mesh for face records)");
+    };
+    if (!(rc = put(&v, d->verts, 3 * (size_t)d->nverts)) && !(rc = put(&ix, d->idx, 3 * (size_t)d->nfaces)) &&
+        !(rc = put(&fm, d->face_mat, d->nfaces)) && !(d->face_light && (rc = put(&fl, d->face_light, d->nfaces)))) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)d->nfaces + RB - 1) / RB, 65536);
+        hipLaunchKernelGGL(k_face_shade, dim3(grid), dim3(RB), 0, nullptr, v.p, ix.p, d->nfaces, fm.p,
+                           d->face_light ? fl.p : nullptr, faces->p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);  // before the temporaries are freed
+        if (e != hipSuccess) rc = hip_fail(e, "k_face_shade");
+    }
+    v.release(); ix.release(); fm.release(); fl.release();
+    return rc;
+}
+
 int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
     if (!d->verts || !d->idx || !d->face_mat || !d->mats) return fail(PBRTX_ERR_ARG, "null scene arrays");
     if (d->integrator != 0 && d->integrator != 1) return fail(PBRTX_ERR_ARG, "unknown integrator");
@@ -1379,35 +1438,25 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
         S->bvh = HostBvh{};  // the device copy is all that renders need
     }
     S->nfaces = d->nfaces;
+    if ((rc = face_shade_device(d, &S->faces))) return rc;
+    // the emitting faces' records on the host (face_frame is the device's
+    // formula: the same bits)
     auto V = [&](uint32_t i) { return mk3(d->verts[3 * i], d->verts[3 * i + 1], d->verts[3 * i + 2]); };
-    std::vector<FaceShade> fsh(d->nfaces);
     std::vector<std::vector<AreaTriRec>> per_light(d->n_area);
-    for (uint32_t f = 0; f < d->nfaces; ++f) {
-        // Triangle::intersect dg with the default uvs (mesh.rs:212-254)
+    for (uint32_t f = 0; d->face_light && f < d->nfaces; ++f) {
+        const int32_t light = d->face_light[f];
+        if (light < 0) continue;
+        if ((uint32_t)light >= d->n_area) return fail(PBRTX_ERR_ARG, "face light out of range");
         const f3 p1 = V(d->idx[3 * f + 2]), p2 = V(d->idx[3 * f + 1]), p3 = V(d->idx[3 * f]);
-        const float du1 = 0.0f - 1.0f, du2 = 1.0f - 1.0f, dv1 = 0.0f - 1.0f, dv2 = 0.0f - 1.0f;
-        const f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
-        const float det = du1 * dv2 - dv1 * du2;
-        const float inv_det = 1.0f / det;
-        const f3 dpdu = mul3(sub3(mul3(dp1, dv2), mul3(dp2, dv1)), inv_det);
-        const f3 dpdv = mul3(add3(mul3(dp1, -du2), mul3(dp2, du1)), inv_det);
-        const f3 nn = normalize3(cross3(dpdu, dpdv));
-        const f3 tn = normalize3(dpdu);
-        FaceShade& sh = fsh[f];
-        sh.nn[0] = nn.x; sh.nn[1] = nn.y; sh.nn[2] = nn.z;
-        sh.tn[0] = tn.x; sh.tn[1] = tn.y; sh.tn[2] = tn.z;
-        sh.mat = d->face_mat[f];
-        sh.light = d->face_light ? d->face_light[f] : -1;
-        if (sh.light >= 0) {
-            if ((uint32_t)sh.light >= d->n_area) return fail(PBRTX_ERR_ARG, "face light out of range");
-            AreaTriRec t{};
-            t.p1[0] = p1.x; t.p1[1] = p1.y; t.p1[2] = p1.z;
-            t.p2[0] = p2.x; t.p2[1] = p2.y; t.p2[2] = p2.z;
-            t.p3[0] = p3.x; t.p3[1] = p3.y; t.p3[2] = p3.z;
-            t.nn[0] = nn.x; t.nn[1] = nn.y; t.nn[2] = nn.z;
-            t.area = 0.5f * sqrtf(lensq3(cross3(sub3(p2, p1), sub3(p3, p1))));  // mesh.rs:100-103
-            per_light[(size_t)sh.light].push_back(t);
-        }
+        f3 nn, tn;
+        face_frame(p1, p2, p3, &nn, &tn);
+        AreaTriRec t{};
+        t.p1[0] = p1.x; t.p1[1] = p1.y; t.p1[2] = p1.z;
+        t.p2[0] = p2.x; t.p2[1] = p2.y; t.p2[2] = p2.z;
+        t.p3[0] = p3.x; t.p3[1] = p3.y; t.p3[2] = p3.z;
+        t.nn[0] = nn.x; t.nn[1] = nn.y; t.nn[2] = nn.z;
+        t.area = 0.5f * sqrtf(lensq3(cross3(sub3(p2, p1), sub3(p3, p1))));  // mesh.rs:100-103
+        per_light[(size_t)light].push_back(t);
     }
     std::vector<AreaTriRec> at;
     std::vector<uint32_t> afirst, acount;
@@ -1426,7 +1475,7 @@ int render_scene_setup(RenderScene* S, const pbrtx_render_desc* d) {
         mats[m] = MatRec{(uint32_t)d->mats[4 * m], d->mats[4 * m + 1], d->mats[4 * m + 2], d->mats[4 * m + 3]};
     std::vector<float> pts(d->point_lights ? d->point_lights : (const float*)nullptr,
                            d->point_lights ? d->point_lights + 6 * d->n_point : (const float*)nullptr);
-    if ((rc = S->faces.upload(fsh)) || (rc = S->sph.upload(ssh)) || (rc = S->mats.upload(mats)) ||
+    if ((rc = S->sph.upload(ssh)) || (rc = S->mats.upload(mats)) ||
         (rc = S->atris.upload(at)) ||
         (rc = S->a_first.upload(afirst)) || (rc = S->a_count.upload(acount)) || (rc = S->a_total.upload(atotal)) ||
         (rc = S->a_le.upload(ale)) || (rc = S->points.upload(pts)))
