@@ -280,8 +280,18 @@ int RenderServer::launch_batches() {
         int rc = stream_scratch(S.st, spill, &scr);
         if (rc) return rc;
         Stager none;
-        rc = enqueue_path_batch(P, scn->scene.view, S.W, S.cnt.p, 0, (uint32_t)total, PATH_BATCHED, false, nullptr,
-                                scr, S.st, none);
+        if (S.parts.size() == 1) {
+            // one request: the frame path over its samples [g0, g0 + total)
+            // with the request's own parameters, where the traversal makes
+            // the camera rays itself (no segment table, no init pass); a
+            // sample's radiance is the same either way
+            RRequest* r = S.parts[0].first;
+            rc = enqueue_path_batch(r->P, scn->scene.view, S.W, S.cnt.p, S.h_segs[0].g0, (uint32_t)total,
+                                    PATH_BATCHED, false, r->d_sl, scr, S.st, none);
+        } else {
+            rc = enqueue_path_batch(P, scn->scene.view, S.W, S.cnt.p, 0, (uint32_t)total, PATH_BATCHED, false,
+                                    nullptr, scr, S.st, none);
+        }
         if (rc) return rc;
         e = hipMemcpyAsync(S.h_cnt, S.cnt.p, N_COUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, S.st);
         if (e == hipSuccess) e = hipEventRecord(S.end, S.st);
