@@ -14,6 +14,9 @@ struct CamParams {
     uint64_t g0;             // global index of the launch's first sample
     const float2* spos;      // low-discrepancy or Halton positions within the pixel, or null
     const uint32_t* scount;  // Halton: samples in each pixel's spp slots (the rest empty), or null
+    // camera-ray packets over pixel tiles (set by the packet kernel's
+    // launcher): log2 of the tile width and of spp, tile_lw = 0xFF for none
+    uint32_t tile_lw = 0xFFu, spp_l = 0;
 };
 
 // ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)

@@ -818,7 +818,19 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
             base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
             if (base >= n) break;
         }
-        const uint32_t i = base + lane;
+        uint32_t i = base + lane;
+        if (GEN && cam.tile_lw != 0xFFu) {
+            // the packet is a tile of 64 / spp pixels, tile_w wide, all their
+            // samples (whole rows of tiles: the launch covers whole rows)
+            const uint32_t tw = 1u << cam.tile_lw;
+            const uint32_t th_l = 6u - cam.spp_l - cam.tile_lw;
+            const uint32_t ntx = cam.xres >> cam.tile_lw;
+            const uint32_t tile = base >> 6;
+            const uint32_t tx = tile % ntx, ty = tile / ntx;
+            const uint32_t q = lane >> cam.spp_l, sidx = lane & ((1u << cam.spp_l) - 1u);
+            const uint32_t x = (tx << cam.tile_lw) + (q & (tw - 1u)), y = (ty << th_l) + (q >> cam.tile_lw);
+            i = ((y * cam.xres + x) << cam.spp_l) + sidx;
+        }
         const bool valid = i < n;
         RayState r{};
         bool alive = false;
@@ -1262,8 +1274,28 @@ hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hi
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
+    // Packets over pixel tiles (64 / spp pixels each, as square as a power
+    // of two allows) when the launch is whole rows of the image from its
+    // first sample, spp is a power of two up to 64 and the tiles divide the
+    // image; else packets of 64 consecutive samples (64 / spp pixels of one
+    // row).  Which lane makes which sample's ray is all this changes.
+    CamParams c = cam;
+    c.tile_lw = 0xFFu;
+    const uint32_t spp = cam.spp;
+    if (cam.g0 == 0 && spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0 && cam.xres > 0) {
+        uint32_t sl = 0;
+        while ((1u << sl) < spp) ++sl;
+        const uint32_t pl = 6 - sl;           // log2 pixels per tile
+        const uint32_t wl = (pl + 1) / 2;     // log2 tile width
+        const uint32_t hl = pl - wl;
+        const uint64_t row = (uint64_t)cam.xres * spp;
+        if (n % row == 0 && (cam.xres & ((1u << wl) - 1u)) == 0 && ((n / row) & ((1u << hl) - 1u)) == 0) {
+            c.tile_lw = wl;
+            c.spp_l = sl;
+        }
+    }
     hipLaunchKernelGGL(k_closest_packet<true>, dim3(wide_grid(k_closest_packet<true>, tun)), dim3(block), 0, st, s,
-                       (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter, scr->counter + 1, cam);
+                       (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter, scr->counter + 1, c);
     return hipGetLastError();
 }
 
