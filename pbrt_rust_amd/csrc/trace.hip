@@ -1275,21 +1275,22 @@ hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hi
     hipError_t e = fill_u32(scr->counter, 0u, 1, st);
     if (e != hipSuccess) return e;
     // Packets over pixel tiles (64 / spp pixels each, as square as a power
-    // of two allows) when the launch is whole rows of the image from its
-    // first sample, spp is a power of two up to 64 and the tiles divide the
-    // image; else packets of 64 consecutive samples (64 / spp pixels of one
-    // row).  Which lane makes which sample's ray is all this changes.
+    // of two allows) when the launch is whole rows of the image starting at
+    // a row, spp is a power of two up to 64 and the tiles divide its rows and
+    // the row width; else packets of 64 consecutive samples (64 / spp pixels
+    // of one row).  Which lane makes which sample's ray is all this changes.
     CamParams c = cam;
     c.tile_lw = 0xFFu;
     const uint32_t spp = cam.spp;
-    if (cam.g0 == 0 && spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0 && cam.xres > 0) {
+    if (spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0 && cam.xres > 0) {
         uint32_t sl = 0;
         while ((1u << sl) < spp) ++sl;
         const uint32_t pl = 6 - sl;           // log2 pixels per tile
         const uint32_t wl = (pl + 1) / 2;     // log2 tile width
         const uint32_t hl = pl - wl;
         const uint64_t row = (uint64_t)cam.xres * spp;
-        if (n % row == 0 && (cam.xres & ((1u << wl) - 1u)) == 0 && ((n / row) & ((1u << hl) - 1u)) == 0) {
+        if (cam.g0 % row == 0 && n % row == 0 && (cam.xres & ((1u << wl) - 1u)) == 0 &&
+            ((n / row) & ((1u << hl) - 1u)) == 0) {
             c.tile_lw = wl;
             c.spp_l = sl;
         }
