@@ -1561,9 +1561,12 @@ int reserve_path_work(WorkBufs& W, uint32_t cap, bool osort) {
 
 int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, uint32_t* cnt, uint64_t g0,
                        uint32_t n, int path, bool osort_req, float4* sample_l, TraceScratch* scr, hipStream_t st,
-                       Stager& sg) {
+                       Stager& sg, const ShadowStream* ss) {
     if (n == 0) return PBRTX_OK;
     const bool batched = path == PATH_BATCHED;
+    // shadow rays on their own stream (batched queues only)
+    const bool split = batched && ss && ss->st && ss->scr && ss->ev;
+    hipEvent_t joined = nullptr;  // the last shadow work enqueued on ss->st
     const TraceTuning& tunC = g_tuning[path][0];
     const TraceTuning& tunA = g_tuning[path][1];
     const TraceTuning tunCam = camera_tuning(path, P.spp);  // camera rays (coherent)
@@ -1601,6 +1604,9 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st));
         }
         RCHK(sg.end(a, &Timing::closest));
+        // the previous bounce's shadow work has read the shadow queue and
+        // added its results to the path states the vertex kernel reads
+        if (joined) RCHK(hipStreamWaitEvent(st, joined, 0));
         a = sg.begin();
         RCHK(fill_u32(cout, 0u, 1, st));
         RCHK(fill_u32(cnt + 5, 0u, 1, st));
@@ -1623,18 +1629,36 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             RCHK(hipGetLastError());
         }
         RCHK(sg.end(a, &Timing::other));
-        a = sg.begin();
+        // shadow rays: occlusion, then the unoccluded samples added
+        hipStream_t sst = split ? ss->st : st;
+        TraceScratch* sscr = split ? ss->scr : scr;
+        Stager& ssg = split ? *ss->sg : sg;
+        if (split) {
+            hipEvent_t fork = ss->ev->next();
+            if (!fork) return fail(PBRTX_ERR_OOM, "events");
+            RCHK(hipEventRecord(fork, st));
+            RCHK(hipStreamWaitEvent(sst, fork, 0));
+        }
+        a = ssg.begin();
         if (batched)
-            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st, cnt + 5));
+            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, sscr, sst, cnt + 5));
         else
-            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, scr, st));
-        RCHK(sg.end(a, &Timing::any));
-        a = sg.begin();
-        hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n,
+            RCHK(trace_any_bytes(sv, W.sh.p, n, W.occ.p, path, tunA, sscr, sst));
+        RCHK(ssg.end(a, &Timing::any));
+        a = ssg.begin();
+        hipLaunchKernelGGL(k_path_shadow, dim3(grid_for(n)), dim3(RB), 0, sst, P, g0, n,
                            batched ? cnt + 5 : (const uint32_t*)nullptr,
                            batched ? W.shpid.p : (const uint32_t*)nullptr, W.contrib.p, W.occ.p, W.st.p, sample_l);
         RCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, sst, cnt + 6, cnt + 5, cnt + 7);
+        RCHK(hipGetLastError());
+        if (split) {
+            RCHK(ssg.end(a, &Timing::other));
+            joined = ss->ev->next();
+            if (!joined) return fail(PBRTX_ERR_OOM, "events");
+            RCHK(hipEventRecord(joined, sst));
+            a = sg.begin();
+        }
         if (bounce < P.max_depth)
             hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
         RCHK(hipGetLastError());
@@ -1652,6 +1676,8 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         std::swap(pin, pout);
         std::swap(cin, cout);
     }
+    // the batch ends when its last shadow work has
+    if (joined) RCHK(hipStreamWaitEvent(st, joined, 0));
     return PBRTX_OK;
 }
 
@@ -1671,6 +1697,7 @@ struct Renderer {
     DevArr<float2> fpos;  // and its staging of the frame's samples
     DevArr<float4> fxyz;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // the path tracer's shadow rays
     EventPool events;
     bool octant_sort = false;  // path tracer, batched: octant-sort the extension queue
     bool have_frame = false;   // rgb holds a finished frame
@@ -1744,6 +1771,7 @@ static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
     // for its last event, so host reads after a frame see its results)
     {
         hipError_t e = hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking);
+        if (e == hipSuccess && d->integrator == 1) e = hipStreamCreateWithFlags(&R->stream2, hipStreamNonBlocking);
         if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     }
     R->P = R->sc.base;
@@ -1775,7 +1803,8 @@ static int renderer_setup(Renderer* R, const pbrtx_render_desc* d) {
 // Enqueues one frame's device work on the renderer's stream: every kernel and
 // fill from the counter reset to the film, nothing that waits on the host, so
 // it can be captured as a graph.  sg: per-stage events (or none).
-static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, uint64_t* camera_rays) {
+static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, uint64_t* camera_rays,
+                         const ShadowStream* ss) {
     RenderParams& P = R->P;
     hipStream_t st = R->stream;
     WorkBufs& W = R->W;
@@ -1800,7 +1829,7 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         const uint32_t n = (uint32_t)((R->n_samples - g0) < per ? (R->n_samples - g0) : per);
         total_camera += n;
         if (R->sc.integrator != 0) {
-            ICHK(enqueue_path_batch(P, sv, W, cnt, g0, n, path, R->octant_sort, R->sample_l.p, scr, st, sg));
+            ICHK(enqueue_path_batch(P, sv, W, cnt, g0, n, path, R->octant_sort, R->sample_l.p, scr, st, sg, ss));
             continue;
         }
         const size_t slots = (size_t)n * P.S;
@@ -1875,6 +1904,8 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
     hipStream_t st = R->stream;
     TraceScratch* scr;
     ICHK(stream_scratch(st, render_spill_bytes(R->sc.scene.view, path), &scr));
+    TraceScratch* scr2 = nullptr;
+    if (R->stream2) ICHK(stream_scratch(R->stream2, render_spill_bytes(R->sc.scene.view, path), &scr2));
     R->events.used = 0;
     Timing T;
     uint64_t total_camera = 0;
@@ -1885,8 +1916,9 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         if (R->gexec[path]) { (void)hipGraphExecDestroy(R->gexec[path]); R->gexec[path] = nullptr; }
         if (R->graph[path]) { (void)hipGraphDestroy(R->graph[path]); R->graph[path] = nullptr; }
         RCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        Stager none;
-        const int rc = enqueue_frame(R, path, scr, none, &R->gcamera[path]);
+        Stager none, none2;
+        const ShadowStream ss{R->stream2, scr2, &R->events, &none2};
+        const int rc = enqueue_frame(R, path, scr, none, &R->gcamera[path], &ss);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &g);
         if (rc) {
@@ -1903,8 +1935,9 @@ static int render_frame(Renderer* R, int path, pbrtx_render_stats* stats) {
         RCHK(hipGraphLaunch(R->gexec[path], st));
         total_camera = R->gcamera[path];
     } else {
-        Stager sg{&R->events, &T, st};
-        ICHK(enqueue_frame(R, path, scr, sg, &total_camera));
+        Stager sg{&R->events, &T, st}, sg2{&R->events, &T, R->stream2};
+        const ShadowStream ss{R->stream2, scr2, &R->events, &sg2};
+        ICHK(enqueue_frame(R, path, scr, sg, &total_camera, &ss));
         R->warm[path] = true;
     }
     RCHK(hipEventRecord(e_end, st));
@@ -2090,6 +2123,7 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.W.release();
     R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release(); R.spos.release(); R.scount.release();
     if (R.stream) (void)hipStreamDestroy(R.stream);
+    if (R.stream2) (void)hipStreamDestroy(R.stream2);
     delete h;
     return PBRTX_OK;
 }

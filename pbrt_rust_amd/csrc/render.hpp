@@ -216,6 +216,17 @@ struct Stager {
     }
 };
 
+// A second stream for the path tracer's shadow rays: bounce b's occlusion
+// launch and shadow accumulation run on it beside bounce b + 1's extension
+// traversal (a path's next vertex waits for its shadow result, its next ray
+// does not).  ev: fork / join events; sg: per-stage timing on st.
+struct ShadowStream {
+    hipStream_t st = nullptr;
+    TraceScratch* scr = nullptr;
+    EventPool* ev = nullptr;
+    Stager* sg = nullptr;
+};
+
 // Sizes W for batches of up to cap paths (no allocation afterwards).
 int reserve_path_work(WorkBufs& W, uint32_t cap, bool osort);
 
@@ -232,7 +243,7 @@ constexpr uint32_t N_COUNTERS = 16;
 // Adds the batch's rays to cnt[6..9]; enqueues device work only.
 int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, uint32_t* cnt, uint64_t g0,
                        uint32_t n, int path, bool osort, float4* sample_l, TraceScratch* scr, hipStream_t st,
-                       Stager& sg);
+                       Stager& sg, const ShadowStream* ss = nullptr);
 // Film::add_sample + write_image over a frame's sample radiance.
 hipError_t enqueue_film(const RenderParams& P, const float4* sample_l, float* rgb, hipStream_t st);
 // Spill bytes of the traversal launches of a path (largest over the kinds).
