@@ -551,12 +551,14 @@ struct VertexOut {
 __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, uint32_t bounce, uint32_t i,
                                             bool queued, bool fresh, const uint32_t* pid_q, const Ray* rays_in,
                                             const Hit* hits, PathState* st, float4* sample_l, VertexOut& o) {
-    const uint32_t pid = (queued && !fresh) ? pid_q[i] : i;
+    // queue form: the path of queue entry i; fresh paths come from the list
+    // of camera rays that hit (pid_q), their hit records at the path index
+    const uint32_t pid = queued ? pid_q[i] : i;
     PathState ps;
-    if (fresh) ps = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, 0u};
+    if (fresh) ps = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, pid, 0u};
     else ps = st[pid];
     const bool active = queued || !(ps.flags & 4u);  // plain form: bit 4 marks a finished path
-    Hit h = active ? hits[i] : Hit{NO_HIT, 0, 0, 0};
+    Hit h = active ? hits[fresh ? pid : i] : Hit{NO_HIT, 0, 0, 0};
     bool want_shadow = false, want_next = false, finish = false;
     Ray& sray = o.sray;
     Ray& nray = o.nray;
@@ -565,7 +567,7 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
         finish = true;  // a miss ends the path (no infinite lights)
     } else if (active) {
         Ray r;
-        if (fresh) camera_ray_of(P, g0 + i, &r);
+        if (fresh) camera_ray_of(P, g0 + pid, &r);
         else r = rays_in[i];
         const f3 d = mk3(r.dx, r.dy, r.dz);
         const HitShade hs = hit_shade(P, r, h);
@@ -659,6 +661,67 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
     o.pid = pid;
     o.shadow = want_shadow;
     o.next = want_next;
+}
+
+// Bounce 0 of fused frames: a camera ray that missed ends its path with zero
+// radiance (no infinite lights), written here; the paths whose rays hit are
+// listed densely in path order, and only they go through the vertex kernel.
+// A block takes a tile of MISS_B * MISS_PER paths in ticket order (the
+// look-back's progress guarantee); round k of it reads paths tile + k*MISS_B
+// + thread (coalesced), and list positions follow path order: round-major,
+// then wave, then lane.
+constexpr uint32_t MISS_B = 1024, MISS_PER = 8;
+__global__ __launch_bounds__(MISS_B) void k_path_miss0(uint64_t g0, uint32_t n, const Hit* hits, float4* sample_l,
+                                                       uint32_t* list, uint32_t* count,
+                                                       unsigned long long* tiles, uint32_t* ticket) {
+    constexpr uint32_t W = MISS_B / 64;
+    __shared__ uint32_t s_bid, s_excl;
+    __shared__ uint32_t s_pre[MISS_PER * W];  // hits before (round, wave) in path order
+    if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t bid = s_bid;
+    const uint32_t base = bid * (MISS_B * MISS_PER);
+    const uint32_t wid = threadIdx.x / 64, lane = __lane_id();
+    const unsigned long long ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long m[MISS_PER];
+#pragma unroll
+    for (uint32_t k = 0; k < MISS_PER; ++k) {
+        const uint32_t i = base + k * MISS_B + threadIdx.x;
+        bool hit = false;
+        if (i < n) {
+            hit = hits[i].prim != NO_HIT;
+            if (!hit) sample_l[g0 + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        m[k] = __ballot(hit);
+        if (lane == 0) s_pre[k * W + wid] = (uint32_t)__popcll(m[k]);
+    }
+    __syncthreads();
+    // exclusive prefix over the (round, wave) counts, by the first wave: two
+    // entries a lane
+    if (threadIdx.x < 64) {
+        static_assert(MISS_PER * W == 128, "two entries a lane");
+        const uint32_t a = s_pre[2 * lane], b = s_pre[2 * lane + 1];
+        uint32_t x = a + b;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        const uint32_t tot = __shfl(x, 63);
+        const uint32_t ex = x - (a + b);
+        s_pre[2 * lane] = ex;
+        s_pre[2 * lane + 1] = ex + a;
+        const uint32_t e = lookback(tiles, bid, tot);
+        if (lane == 0) {
+            s_excl = e;
+            if ((uint64_t)base + MISS_B * MISS_PER >= n) *count = e + tot;  // the last tile
+        }
+    }
+    __syncthreads();
+    const uint32_t e = s_excl;
+#pragma unroll
+    for (uint32_t k = 0; k < MISS_PER; ++k)
+        if ((m[k] >> lane) & 1ull)
+            list[e + s_pre[k * W + wid] + (uint32_t)__popcll(m[k] & ltm)] = base + k * MISS_B + threadIdx.x;
 }
 
 // The path vertex kernel.  PSHADE_QUEUE: persistent blocks take tiles of
@@ -1604,6 +1667,17 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st));
         }
         RCHK(sg.end(a, &Timing::closest));
+        if (fused && bounce == 0) {
+            // the misses end here; the hits' path ids become the queue
+            a = sg.begin();
+            const uint32_t mb = (uint32_t)(((uint64_t)n + MISS_B * MISS_PER - 1) / (MISS_B * MISS_PER));
+            ICHK(W.bcounts.ensure(2 * (size_t)mb + 2));
+            RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)mb + 2, st));
+            hipLaunchKernelGGL(k_path_miss0, dim3(mb), dim3(MISS_B), 0, st, g0, n, W.hits.p, sample_l, pin, cin,
+                               (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb);
+            RCHK(hipGetLastError());
+            RCHK(sg.end(a, &Timing::other));
+        }
         // the previous bounce's shadow work has read the shadow queue and
         // added its results to the path states the vertex kernel reads
         if (joined) RCHK(hipStreamWaitEvent(st, joined, 0));
