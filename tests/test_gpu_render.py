@@ -145,6 +145,22 @@ def test_chunking_and_repeat_frames_are_identical():
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 1), (2, 2), (3, 3), (4, 4), (8, 8)])
+def test_camera_tile_packets(nx, ny):
+    # camera-ray packets take pixel tiles when a launch covers whole rows
+    # that the tiles divide (spp a power of two), else runs of 64 samples:
+    # the whole frame, chunks of whole tile rows (launches starting at row
+    # 8, 16, ...) and ragged chunks all give the oracle's samples
+    scene = scenes.direct_lighting_scene(level=3)
+    cam = scenes.pinhole(64, 48)
+    integ = dict(kind="direct", nx=nx, ny=ny, seed=21, light_samples=1)
+    ref = oracle_render(scene, cam, integ)
+    spp = nx * ny
+    for chunk in (None, 64 * spp * 8, 64 * spp * 8 + spp, 777):
+        img, samples, _ = gpu_render(scene, cam, integ, "batched", chunk=chunk)
+        assert_same(ref, img, samples)
+
+
 def test_film_boundary_samples():
     # 1x1 spp with strata of width 1: with jitter the sample can sit exactly
     # on a pixel edge only when u = 0; a large image makes the gather path
