@@ -46,7 +46,7 @@ SIMDS = 1024  # 256 CUs x 4 SIMDs
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_direct1024.json")
 PROFILE_VALU = os.path.join(ROOT, "profiles", "valu_direct1024.json")
 KERNEL_ANY = "k_trace_wide<true, false, true, 16>"
-KERNEL_CLOSEST = "k_trace_wide<false, false, true, 8>"
+KERNEL_CLOSEST = "k_closest_packet<true, false>"  # c1: camera rays made in the packet kernel
 
 
 def parse():

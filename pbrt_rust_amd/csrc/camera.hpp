@@ -19,6 +19,29 @@ struct CamParams {
     uint32_t tile_lw = 0xFFu, spp_l = 0;
 };
 
+// A served batch of several requests: batch index b belongs to segment s,
+// the last whose start <= b; the segment's camera is the CamParams at
+// cams + cam * stride and its first sample is g0, so b makes the ray of
+// sample g0 + (b - start) of that camera.
+struct CamSeg {
+    uint32_t start, cam;
+    uint64_t g0;
+};
+struct CamTable {
+    const CamSeg* segs = nullptr;
+    const uint8_t* cams = nullptr;
+    uint32_t nsegs = 0, stride = 0;
+};
+__host__ __device__ __forceinline__ uint32_t cam_seg_of(const CamSeg* segs, uint32_t nsegs, uint32_t b) {
+    uint32_t lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (segs[mid].start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ApplyTransform<Point> (transform.rs:195-207) and <Vector> (:210-218)
 __device__ __forceinline__ f3 xf_point(const float* m, f3 p) {
     float xt = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];

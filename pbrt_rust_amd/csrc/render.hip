@@ -663,17 +663,23 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
     o.next = want_next;
 }
 
-// Bounce 0 of fused frames: a camera ray that missed ends its path with zero
-// radiance (no infinite lights), written here; the paths whose rays hit are
-// listed densely in path order, and only they go through the vertex kernel.
-// A block takes a tile of MISS_B * MISS_PER paths in ticket order (the
-// look-back's progress guarantee); round k of it reads paths tile + k*MISS_B
-// + thread (coalesced), and list positions follow path order: round-major,
-// then wave, then lane.
+// Bounce 0 of fused batches: a camera ray that missed ends its path with
+// zero radiance (no infinite lights), written here; the paths whose rays hit
+// are listed densely in path order, and only they go through the vertex
+// kernel.  A block takes a tile of MISS_B * MISS_PER paths in ticket order
+// (the look-back's progress guarantee); round k of it reads paths tile +
+// k*MISS_B + thread (coalesced), and list positions follow path order:
+// round-major, then wave, then lane.  SEG (a served batch of several
+// requests): a path's radiance goes to its segment's request, and a hit's
+// path starts here -- its state (segment in the flags), its camera ray and
+// its hit record stored at its queue position, as the vertex kernel reads a
+// queue.
 constexpr uint32_t MISS_B = 1024, MISS_PER = 8;
-__global__ __launch_bounds__(MISS_B) void k_path_miss0(uint64_t g0, uint32_t n, const Hit* hits, float4* sample_l,
-                                                       uint32_t* list, uint32_t* count,
-                                                       unsigned long long* tiles, uint32_t* ticket) {
+template <bool SEG>
+__global__ __launch_bounds__(MISS_B) void k_path_miss0(RenderParams P, uint64_t g0, uint32_t n, const Hit* hits,
+                                                       float4* sample_l, uint32_t* list, uint32_t* count,
+                                                       unsigned long long* tiles, uint32_t* ticket, Ray* rays_q,
+                                                       Hit* hits_q, PathState* st) {
     constexpr uint32_t W = MISS_B / 64;
     __shared__ uint32_t s_bid, s_excl;
     __shared__ uint32_t s_pre[MISS_PER * W];  // hits before (round, wave) in path order
@@ -683,6 +689,7 @@ __global__ __launch_bounds__(MISS_B) void k_path_miss0(uint64_t g0, uint32_t n, 
     const uint32_t base = bid * (MISS_B * MISS_PER);
     const uint32_t wid = threadIdx.x / 64, lane = __lane_id();
     const unsigned long long ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const CamSeg* segs = reinterpret_cast<const CamSeg*>(P.segs);
     unsigned long long m[MISS_PER];
 #pragma unroll
     for (uint32_t k = 0; k < MISS_PER; ++k) {
@@ -690,7 +697,16 @@ __global__ __launch_bounds__(MISS_B) void k_path_miss0(uint64_t g0, uint32_t n, 
         bool hit = false;
         if (i < n) {
             hit = hits[i].prim != NO_HIT;
-            if (!hit) sample_l[g0 + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (!hit) {
+                float4* out;
+                if (SEG) {
+                    const CamSeg sg = segs[cam_seg_of(segs, P.nsegs, i)];
+                    out = P.reqs[sg.cam].sample_l + sg.g0 + (i - sg.start);
+                } else {
+                    out = sample_l + g0 + i;
+                }
+                *out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         }
         m[k] = __ballot(hit);
         if (lane == 0) s_pre[k * W + wid] = (uint32_t)__popcll(m[k]);
@@ -720,8 +736,20 @@ __global__ __launch_bounds__(MISS_B) void k_path_miss0(uint64_t g0, uint32_t n, 
     const uint32_t e = s_excl;
 #pragma unroll
     for (uint32_t k = 0; k < MISS_PER; ++k)
-        if ((m[k] >> lane) & 1ull)
-            list[e + s_pre[k * W + wid] + (uint32_t)__popcll(m[k] & ltm)] = base + k * MISS_B + threadIdx.x;
+        if ((m[k] >> lane) & 1ull) {
+            const uint32_t i = base + k * MISS_B + threadIdx.x;
+            const uint32_t q = e + s_pre[k * W + wid] + (uint32_t)__popcll(m[k] & ltm);
+            list[q] = i;
+            if (SEG) {
+                const uint32_t s = cam_seg_of(segs, P.nsegs, i);
+                const CamSeg sg = segs[s];
+                Ray r;
+                camera_ray_of(P.reqs[sg.cam].cam, sg.g0 + (i - sg.start), &r);
+                rays_q[q] = r;
+                hits_q[q] = hits[i];
+                st[i] = PathState{{1.0f, 1.0f, 1.0f}, {0.0f, 0.0f, 0.0f}, i, s << 8};
+            }
+        }
 }
 
 // The path vertex kernel.  PSHADE_QUEUE: persistent blocks take tiles of
@@ -1386,7 +1414,7 @@ void RenderScene::release() {
 }
 
 void WorkBufs::release() {
-    rays.release(); rays2.release(); rays3.release(); sh.release(); hits.release(); contrib.release();
+    rays.release(); rays2.release(); rays3.release(); sh.release(); hits.release(); hits2.release(); contrib.release();
     lbase.release(); qidx.release(); occ.release(); pid.release(); pid2.release(); pid3.release();
     shpid.release(); bcounts.release(); st.release();
 }
@@ -1639,8 +1667,23 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
     // vertex makes them again and starts the paths (no init pass); served
     // batches (several cameras) start from an init pass
     const bool fused = batched && !P.segs && camera_fusable(sv, tunCam);
+    // served batches of several requests: camera rays made in the packet
+    // kernel from the segment table too, and the misses ended the same way
+    const bool fused_seg = batched && P.segs && camera_fusable(sv, tunCam);
+    CamTable ct;
+    if (fused_seg) {
+        static_assert(sizeof(RenderSeg) == sizeof(CamSeg) && offsetof(RenderSeg, req) == offsetof(CamSeg, cam) &&
+                          offsetof(RenderSeg, g0) == offsetof(CamSeg, g0),
+                      "segment layouts");
+        static_assert(offsetof(ReqView, cam) == 0, "camera first in a request view");
+        ct.segs = reinterpret_cast<const CamSeg*>(P.segs);
+        ct.cams = reinterpret_cast<const uint8_t*>(P.reqs);
+        ct.nsegs = P.nsegs;
+        ct.stride = (uint32_t)sizeof(ReqView);
+        ICHK(W.hits2.ensure(n));
+    }
     hipEvent_t a = sg.begin();
-    if (fused) {
+    if (fused || fused_seg) {
         RCHK(fill_u32(cnt + 1, n, 1, st));
     } else {
         hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.rays.p, W.st.p,
@@ -1661,20 +1704,28 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         const TraceTuning& tb = (bounce == 0 && !P.segs) ? tunCam : tunC;  // frames' bounce 0: camera rays
         if (fused && bounce == 0) {
             RCHK(trace_camera(sv, cam_params(P, g0), n, W.hits.p, tb, scr, st));
+        } else if (fused_seg && bounce == 0) {
+            RCHK(trace_camera(sv, CamParams{}, n, W.hits.p, tunCam, scr, st, &ct));
         } else if (batched) {
             RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st, cin));
         } else {
             RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st));
         }
         RCHK(sg.end(a, &Timing::closest));
-        if (fused && bounce == 0) {
+        if ((fused || fused_seg) && bounce == 0) {
             // the misses end here; the hits' path ids become the queue
             a = sg.begin();
             const uint32_t mb = (uint32_t)(((uint64_t)n + MISS_B * MISS_PER - 1) / (MISS_B * MISS_PER));
             ICHK(W.bcounts.ensure(2 * (size_t)mb + 2));
             RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)mb + 2, st));
-            hipLaunchKernelGGL(k_path_miss0, dim3(mb), dim3(MISS_B), 0, st, g0, n, W.hits.p, sample_l, pin, cin,
-                               (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb);
+            if (fused)
+                hipLaunchKernelGGL(k_path_miss0<false>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
+                                   pin, cin, (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb,
+                                   (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
+            else
+                hipLaunchKernelGGL(k_path_miss0<true>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
+                                   pin, cin, (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb, rin,
+                                   W.hits2.p, W.st.p);
             RCHK(hipGetLastError());
             RCHK(sg.end(a, &Timing::other));
         }
@@ -1696,7 +1747,8 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(VERTEX_B), 0, st, P, g0, bounce, n,
                                batched ? (const uint32_t*)cin : (const uint32_t*)nullptr,
                                batched ? (const uint32_t*)pin : (const uint32_t*)nullptr,
-                               (fused && bounce == 0) ? (const Ray*)nullptr : rin, W.hits.p, W.st.p, rout,
+                               (fused && bounce == 0) ? (const Ray*)nullptr : rin,
+                               (fused_seg && bounce == 0) ? W.hits2.p : W.hits.p, W.st.p, rout,
                                batched ? pout : nullptr, cout, W.sh.p, batched ? W.shpid.p : nullptr, W.contrib.p,
                                cnt + 5, sample_l, tiles, tiles + nb, W.bcounts.p + 4 * (size_t)nb,
                                (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));

@@ -169,6 +169,7 @@ CamParams cam_params(const RenderParams& P, uint64_t g0);
 struct WorkBufs {
     DevArr<Ray> rays, rays2, rays3, sh;
     DevArr<Hit> hits;
+    DevArr<Hit> hits2;  // served batches: bounce 0's hits at their queue positions
     DevArr<float4> contrib, lbase;
     DevArr<uint32_t> qidx, pid, pid2, pid3, shpid, bcounts;
     DevArr<uint8_t> occ;  // occlusion of each queued shadow ray: a byte per ray

@@ -784,10 +784,10 @@ __device__ __forceinline__ float4 cload4(const void* p, uint32_t k) {
 // its current tmax before the leaf's prims.  Interior records are the exact
 // 128-byte ones.  GEN: ray i is the camera ray of sample cam.g0 + i, made
 // here instead of read (no ray array; only the hits are written).
-template <bool GEN>
+template <bool GEN, bool SEGS = false>
 __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_closest_packet(SceneView sv, Ray* rays, uint32_t n_host,
                                                               const uint32_t* n_dev, Hit* hits, uint32_t* counter,
-                                                              uint32_t* err, CamParams cam) {
+                                                              uint32_t* err, CamParams cam, CamTable ct) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[PWAVES][PSTK];
     __shared__ unsigned long long s_msk[PWAVES][PSTK];
@@ -835,11 +835,31 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
         RayState r{};
         bool alive = false;
         uint32_t negm = 0;
+        if (GEN && SEGS) {
+            // several cameras: each distinct segment of the packet in turn,
+            // its camera read with scalar loads
+            const uint32_t sidx = valid ? cam_seg_of(ct.segs, ct.nsegs, i) : 0u;
+            unsigned long long pend = __ballot(valid);
+            while (pend) {
+                const uint32_t s0 = __builtin_amdgcn_readfirstlane(__shfl(sidx, (int)__builtin_ctzll(pend)));
+                const unsigned long long mine = __ballot(valid && sidx == s0);
+                pend &= ~mine;
+                const CamSeg sg = ct.segs[s0];
+                const CamParams* cp = reinterpret_cast<const CamParams*>(ct.cams + (size_t)sg.cam * ct.stride);
+                if ((mine >> lane) & 1ull) {
+                    Ray g;
+                    camera_ray_of(*cp, sg.g0 + (i - sg.start), &g);
+                    set_ray(g, r);
+                }
+            }
+        }
         if (valid) {
             if constexpr (GEN) {
-                Ray g;
-                camera_ray_of(cam, cam.g0 + i, &g);
-                set_ray(g, r);
+                if (!SEGS) {
+                    Ray g;
+                    camera_ray_of(cam, cam.g0 + i, &g);
+                    set_ray(g, r);
+                }
             } else {
                 load_ray(rays, i, r);
             }
@@ -1202,7 +1222,7 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
         hipError_t e = fill_u32(scr->counter, 0u, 1, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_closest_packet<false>, dim3(wide_grid(k_closest_packet<false>, tun)), dim3(block), 0,
-                           st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{});
+                           st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{}, CamTable{});
         return hipGetLastError();
     }
     if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
@@ -1267,7 +1287,7 @@ int noalloc_scratch(TraceScratch* out, hipError_t* err) {
 }
 
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
-                        TraceScratch* scr, hipStream_t st) {
+                        TraceScratch* scr, hipStream_t st, const CamTable* ct) {
     if (n == 0) return hipSuccess;
     if (!camera_fusable(s, tun)) return hipErrorNotSupported;
     const int block = tun.block;
@@ -1282,7 +1302,7 @@ hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hi
     CamParams c = cam;
     c.tile_lw = 0xFFu;
     const uint32_t spp = cam.spp;
-    if (spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0 && cam.xres > 0) {
+    if (!ct && spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0 && cam.xres > 0) {
         uint32_t sl = 0;
         while ((1u << sl) < spp) ++sl;
         const uint32_t pl = 6 - sl;           // log2 pixels per tile
@@ -1295,8 +1315,16 @@ hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hi
             c.spp_l = sl;
         }
     }
-    hipLaunchKernelGGL(k_closest_packet<true>, dim3(wide_grid(k_closest_packet<true>, tun)), dim3(block), 0, st, s,
-                       (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter, scr->counter + 1, c);
+    // several cameras: their own instance, so that the single-camera kernel
+    // keeps its registers
+    if (ct)
+        hipLaunchKernelGGL((k_closest_packet<true, true>), dim3(wide_grid(k_closest_packet<true, true>, tun)),
+                           dim3(block), 0, st, s, (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter,
+                           scr->counter + 1, c, *ct);
+    else
+        hipLaunchKernelGGL((k_closest_packet<true, false>), dim3(wide_grid(k_closest_packet<true, false>, tun)),
+                           dim3(block), 0, st, s, (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter,
+                           scr->counter + 1, c, CamTable{});
     return hipGetLastError();
 }
 

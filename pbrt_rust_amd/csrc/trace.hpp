@@ -90,8 +90,11 @@ struct CamParams;
 // Closest hit of the camera rays of samples [cam.g0, cam.g0 + n), generated
 // inside the (packet) traversal kernel; only the hits are written.
 // hipErrorNotSupported unless camera_fusable(s, tun).
+// ct (served batches of several requests): the batch's segments and their
+// cameras instead of cam, whose samples [g0, g0 + n) are then unused.
+struct CamTable;
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
-                        TraceScratch* scr, hipStream_t st);
+                        TraceScratch* scr, hipStream_t st, const CamTable* ct = nullptr);
 bool camera_fusable(const SceneView& s, const TraceTuning& tun);
 
 // One request chunk of a served launch (see trace_segments).
