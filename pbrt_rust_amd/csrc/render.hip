@@ -1746,14 +1746,12 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         // added its results to the path states the vertex kernel reads
         if (joined) RCHK(hipStreamWaitEvent(st, joined, 0));
         a = sg.begin();
-        RCHK(fill_u32(cout, 0u, 1, st));
-        RCHK(fill_u32(cnt + 5, 0u, 1, st));
         {
             const uint32_t nb = (n + VERTEX_B - 1) / VERTEX_B;
-            // look-back tiles of the two queues (two words per block each)
-            // and the block ticket
+            // the two queues' counts, and the look-back tiles of the two
+            // queues (two words per block each) with the block ticket
             ICHK(W.bcounts.ensure(4 * (size_t)nb + 2));
-            if (batched) RCHK(fill_u32(W.bcounts.p, 0u, 4 * (size_t)nb + 2, st));
+            RCHK(zero3_u32(cout, 1, cnt + 5, 1, W.bcounts.p, batched ? 4 * (size_t)nb + 2 : 0, st));
             unsigned long long* tiles = (unsigned long long*)W.bcounts.p;
             // queue form: persistent blocks over the live paths' tiles
             const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
@@ -2015,14 +2013,12 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
             RCHK(sg.end(a, &Timing::other));
         }
         a = sg.begin();
-        RCHK(fill_u32(cnt + 5, 0u, 1, st));
         {
             const uint32_t nb = (n + SHADE_B - 1) / SHADE_B;
-            if (batched) {
-                // look-back tiles (two words per block) and the block ticket
-                ICHK(W.bcounts.ensure(2 * (size_t)nb + 2));
-                RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)nb + 2, st));
-            }
+            // the shadow-ray count, and (batched) the look-back tiles (two
+            // words per block) with the block ticket
+            if (batched) ICHK(W.bcounts.ensure(2 * (size_t)nb + 2));
+            RCHK(zero3_u32(cnt + 5, 1, W.bcounts.p, batched ? 2 * (size_t)nb + 2 : 0, nullptr, 0, st));
             hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(SHADE_B), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
                                W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (unsigned long long*)W.bcounts.p,
                                W.bcounts.p + 2 * (size_t)nb, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN), list, nlist);

@@ -112,6 +112,8 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
 // Number of spill bytes a launch needs for this scene and tuning.
 size_t trace_spill_bytes(const SceneView& s, int path, const TraceTuning& tun);
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
+// zeroes a[0..na), b[0..nb) and c[0..nc) with one launch (null ranges of length 0)
+hipError_t zero3_u32(uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc, hipStream_t st);
 // streaming 16-byte-vector copy (bytes a multiple of 16)
 hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipStream_t st);
 // 64-byte gather probe over a buffer of `records` (a power of two, at most
