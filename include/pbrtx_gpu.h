@@ -151,6 +151,13 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
    entries spill to global memory) of the batched path's per-ray kernel for
    that kind.  Process-wide setting, kept by pbrtx_set_tuning. */
 int pbrtx_set_stack_depth(int32_t path, int32_t kind, int32_t entries);
+/* Lanes per ray of the batched path's occlusion queries: 0 (the default)
+   chooses by launch size, several lanes per ray for launches too small to
+   fill the GPU; 1 always one; 4, 8 or 16 always that many (when the
+   scene has the compressed records and its record depth fits the group
+   stack, else one).  No reference counterpart: a launch setting of this
+   implementation.  Process-wide setting, kept by pbrtx_set_tuning. */
+int pbrtx_set_ray_lanes(int32_t lanes);
 /* Back to the built-in launch settings (the autotuner's choices on the
    direct-lighting and path-tracing frames).  Unless set explicitly, the
    batched path's camera rays use one stack per wave of 64 rays. */

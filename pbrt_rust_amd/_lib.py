@@ -135,6 +135,7 @@ def lib():
     L.pbrtx_intersect_p_noalloc.argtypes = [vp, vp, u32, vp, vp]
     L.pbrtx_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
     L.pbrtx_set_stack_depth.argtypes = [i32, i32, i32]
+    L.pbrtx_set_ray_lanes.argtypes = [i32]
     L.pbrtx_reset_tuning.argtypes = []
     L.pbrtx_device_copy.argtypes = [vp, vp, u64, vp]
     f32 = c.c_float

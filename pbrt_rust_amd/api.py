@@ -506,19 +506,25 @@ KINDS = {"closest": 0, "any": 1, "camera": 2}
 SCHEMES = {"ray": 0, "packet": 1, "ray_exact": 2}
 
 
-def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray", stack=None):
+def set_tuning(path, kind, block=256, blocks_per_cu=0, refill_pct=0, scheme="ray", stack=None, lanes=None):
     """Launch shape of the closest-hit ("closest"), occlusion ("any") or
     renderer camera-ray ("camera") kernel of a path, and its traversal scheme: "ray" (a stack per ray, the
     compressed 64-byte records when the scene has them), "ray_exact" (a stack
     per ray on the exact 128-byte records) or "packet" (a stack per wave of 64
     consecutive rays in equal-sign groups; batched closest hit and camera rays
     only, refused for occlusion).  stack: LDS stack entries per lane
-    of the per-ray kernel (8, 16 or 24; None keeps the current depth)."""
+    of the per-ray kernel (8, 16 or 24; None keeps the current depth).
+    lanes: batched occlusion only, lanes per ray (0 chooses by launch size,
+    1, 4, 8 or 16; None keeps the current choice)."""
     k = KINDS[kind] if isinstance(kind, str) else int(kind)
     sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
     check(lib().pbrtx_set_tuning(_path_code(path), k, block, blocks_per_cu, refill_pct, sc))
     if stack is not None:
         check(lib().pbrtx_set_stack_depth(_path_code(path), k, int(stack)))
+    if lanes is not None:
+        if _path_code(path) != 1 or k != 1:
+            raise ValueError("lanes per ray is a setting of batched occlusion only")
+        check(lib().pbrtx_set_ray_lanes(int(lanes)))
 
 
 def reset_tuning():

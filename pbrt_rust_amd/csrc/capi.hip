@@ -346,6 +346,7 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
                                    "cause not found");
     TraceTuning t{block, blocks_per_cu, refill_pct, scheme};
     t.stack = g_tuning[path][kind].stack;
+    t.lanes = g_tuning[path][kind].lanes;
     g_tuning[path][kind] = t;
     if (kind == 2 && path == PATH_BATCHED) g_camera_auto = false;  // an explicit choice
     ++g_tuning_gen;  // captured frames are captured again
@@ -366,6 +367,14 @@ int pbrtx_set_stack_depth(int32_t path, int32_t kind, int32_t entries) {
     if (entries != 8 && entries != 16 && entries != 24) return fail(PBRTX_ERR_ARG, "stack depth must be 8, 16 or 24");
     g_tuning[path][kind].stack = entries;
     if (kind == 2 && path == PATH_BATCHED) g_camera_auto = false;
+    ++g_tuning_gen;
+    return PBRTX_OK;
+}
+
+int pbrtx_set_ray_lanes(int32_t lanes) {
+    if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(PBRTX_ERR_ARG, "lanes per ray must be 0 (by launch size), 1, 4, 8 or 16");
+    g_tuning[PATH_BATCHED][1].lanes = lanes;
     ++g_tuning_gen;
     return PBRTX_OK;
 }

@@ -41,6 +41,7 @@ struct TraceTuning {
     int refill_pct = 50;      // refill when fewer than this % of a wave's lanes trace
     int scheme = SCHEME_RAY;
     int stack = 16;           // LDS stack entries per lane of the per-ray kernel: 8, 16 or 24
+    int lanes = 0;            // occlusion, batched: lanes per ray (0 = by launch size, 1, 4, 8 or 16)
 };
 
 // Scratch owned by the caller (per stream): spill stacks and a block of

@@ -61,6 +61,9 @@ def test_errors_are_codes_not_aborts():
     assert L.pbrtx_intersect(None, None, 1, None, 0, None) < 0
     assert L.pbrtx_set_stack_depth(1, 1, 12) < 0 and b"stack depth" in L.pbrtx_last_error()
     assert L.pbrtx_set_stack_depth(1, 3, 16) < 0
+    assert L.pbrtx_set_ray_lanes(3) < 0 and b"lanes per ray" in L.pbrtx_last_error()
+    assert L.pbrtx_set_ray_lanes(32) < 0 and L.pbrtx_set_ray_lanes(2) < 0
+    assert L.pbrtx_set_ray_lanes(8) == 0 and L.pbrtx_set_ray_lanes(0) == 0
     from pbrt_rust_amd._lib import RenderServerConfig
     assert L.pbrtx_render_server_create(ctypes.byref(RenderServerConfig(1 << 30, 1 << 20, 7)), ctypes.byref(h)) < 0
     assert L.pbrtx_render_server_create(ctypes.byref(RenderServerConfig(1 << 30, 1000, 0)), ctypes.byref(h)) < 0

@@ -353,3 +353,28 @@ def test_packet_closest_scheme_matches_oracle(packet_closest):
     test_eight_spheres_prim_ids("batched", "equal")
     test_tetrahedron_maxt_side_effect("batched")
     test_deep_tree_uses_spill_stack("batched")
+
+
+@pytest.fixture(params=[1, 4, 16])
+def occlusion_lanes(request):
+    px.set_tuning("batched", "any", 256, 0, 60, "ray", lanes=request.param)
+    yield request.param
+    px.reset_tuning()
+
+
+def test_occlusion_lanes_match_oracle(occlusion_lanes):
+    # Batched occlusion launches of up to 2^20 rays take 8 lanes per ray by
+    # default; here 1 (the per-ray kernel), 4 and 16: the same fixtures and
+    # edge cases, the 29-level chain, spheres, and the full-size random soup
+    # whose rays pass many boxes (the group stacks run up to their bound);
+    # the overflow flag stays clear
+    test_random_scene_parity("batched", "sah")
+    test_random_scene_parity("batched", "middle")
+    test_edge_cases("batched")
+    test_grazing_rays_on_box_planes("batched")
+    test_mixed_finite_and_infinite_inverse_directions("batched")
+    test_deep_tree_uses_spill_stack("batched")
+    test_mixed_triangles_and_spheres("batched")
+    test_eight_spheres_prim_ids("batched", "sah")
+    test_paths_agree_at_full_size()
+    px.synchronize()
