@@ -770,11 +770,12 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
 // odepth = GSTK (three pending slots per record level below the entry it
 // started from, the fourth child taken), and every run starts from a top of
 // at most lim (the first step, or one after a step of k >= 2).
+// SEG: the rays come from a segment table (served launch) instead of one array.
 constexpr int GSTK = 128;
-template <int G>
+template <int G, bool SEG>
 __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_occlude_group(
     SceneView sv, const Ray* rays, uint32_t n_host, const uint32_t* n_dev, uint32_t* occ, uint8_t* occ8,
-    uint32_t lim, uint32_t* err) {
+    uint32_t lim, uint32_t* err, const TraceSeg* segs, uint32_t nseg) {
     static_assert(G >= 2 && G <= 64 && (64 % G) == 0, "a group lies within one wave");
     // (the grid covers n_host rays: a count on the device never exceeds it)
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
@@ -783,16 +784,23 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
     __shared__ int32_t s_stk[MAX_BLOCK / G][GSTK];
     const uint32_t tid = threadIdx.x, lane = __lane_id();
     const uint32_t g = tid / G, gl = tid % G;
-    const uint32_t rid = blockIdx.x * gpb + g;
+    const uint32_t id = blockIdx.x * gpb + g;
+    uint32_t rid = id, rseg = 0;  // SEG: rid is the ray's index in segment rseg
     const uint32_t gbase = lane & ~(uint32_t)(G - 1);
     const unsigned long long gm = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << gbase;  // the group's lanes
     const unsigned long long lowg = gm & (lane == 0 ? 0ull : (~0ull >> (64 - lane)));  // ... below this one
     int32_t* stk = s_stk[g];
     RayState r{};
-    bool done = rid >= n;
+    bool done = id >= n;
     uint32_t top = 0;
     if (!done) {
-        load_ray(rays, rid, r);
+        if (SEG) {
+            rseg = find_seg(segs, nseg, id);
+            rid = id - segs[rseg].start;
+            load_ray(segs[rseg].rays, rid, r);
+        } else {
+            load_ray(rays, rid, r);
+        }
         float tn;
         // the root's own box (popped first by the reference)
         if (slab(sv.root_lo[0], sv.root_hi[0], sv.root_lo[1], sv.root_hi[1], sv.root_lo[2], sv.root_hi[2], r.ox,
@@ -804,7 +812,7 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
     }
     const bool neg0 = r.ix < 0.0f, neg1 = r.iy < 0.0f, neg2 = r.iz < 0.0f;
     // every ray of the wave may take the ray-space record test (cq_fast_ray)
-    const bool wfast = cq_fast_scene(sv) && __ballot(rid < n && !cq_fast_ray(r)) == 0ull;
+    const bool wfast = cq_fast_scene(sv) && __ballot(id < n && !cq_fast_ray(r)) == 0ull;
     bool hit = false;
     while (__any(!done)) {
         // entries for this step: up to G, fewer near the stack's bound
@@ -903,9 +911,13 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
         hit = hit || ghit;
         done = done || ghit || top == 0u;
     }
-    if (rid < n && gl == 0) {
-        if (occ8) occ8[rid] = hit ? 1u : 0u;
-        else if (hit) atomicOr(occ + (rid >> 5), 1u << (rid & 31u));
+    if (id < n && gl == 0) {
+        if (occ8) {
+            occ8[rid] = hit ? 1u : 0u;
+        } else if (hit) {
+            uint32_t* ow = SEG ? segs[rseg].occ : occ;
+            atomicOr(ow + (rid >> 5), 1u << (rid & 31u));
+        }
     }
 }
 
@@ -1369,6 +1381,11 @@ static int group_lanes(const SceneView& s, uint32_t n, const TraceTuning& tun) {
     return n <= GROUP8_MAX ? 8 : (n <= GROUP4_MAX ? 4 : 1);
 }
 
+template <bool SEG>
+static auto group_kernel(int G) -> decltype(&k_occlude_group<8, SEG>) {
+    return G == 4 ? &k_occlude_group<4, SEG> : G == 8 ? &k_occlude_group<8, SEG> : &k_occlude_group<16, SEG>;
+}
+
 size_t trace_any_block_spill(const SceneView& s, const TraceTuning& tun) {
     return (size_t)wide_levels(s, tun.stack) * tun.block * sizeof(uint2);
 }
@@ -1429,8 +1446,9 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
             const uint32_t lim = (uint32_t)GSTK - 3u * s.odepth;
             const uint32_t gpb = (uint32_t)block / (uint32_t)G;
             const unsigned grid = (unsigned)(((size_t)n + gpb - 1) / gpb);
-            const auto kern = G == 4 ? &k_occlude_group<4> : G == 8 ? &k_occlude_group<8> : &k_occlude_group<16>;
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, occ, occ8, lim, err);
+            const auto kern = group_kernel<false>(G);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, rays, n, n_dev, occ, occ8, lim, err,
+                               (const TraceSeg*)nullptr, 0u);
             return hipGetLastError();
         }
     }
@@ -1542,6 +1560,19 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
     const uint32_t lv = wide_levels(s, tun.stack);
     const bool cq = use_compressed(s, tun);
+    uint32_t* err = scr->counter + 1;
+    if (any && cq && s.prims64) {
+        const int G = group_lanes(s, total, tun);
+        if (G > 1) {
+            const uint32_t lim = (uint32_t)GSTK - 3u * s.odepth;
+            const uint32_t gpb = (uint32_t)block / (uint32_t)G;
+            const unsigned grid = (unsigned)(((size_t)total + gpb - 1) / gpb);
+            hipLaunchKernelGGL(group_kernel<true>(G), dim3(grid), dim3(block), 0, st, s, (const Ray*)nullptr, total,
+                               (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint8_t*)nullptr, lim, err, d_segs,
+                               nseg);
+            return hipGetLastError();
+        }
+    }
     const auto kern = cq ? (any ? wide_kernel<true, true, true>(tun.stack) : wide_kernel<false, true, true>(tun.stack))
                          : (any ? wide_kernel<true, true, false>(tun.stack) : wide_kernel<false, true, false>(tun.stack));
     const int grid = wide_grid(kern, tun);
@@ -1550,7 +1581,6 @@ hipError_t trace_segments(const SceneView& s, bool any, const TraceSeg* d_segs, 
     hipError_t e = fill_u32(parts, 0u, (size_t)WIDE_PARTS * WIDE_PART_STRIDE, st);
     if (e != hipSuccess) return e;
     const uint32_t refill = (uint32_t)(tun.refill_pct < 0 ? 0 : (tun.refill_pct > 100 ? 100 : tun.refill_pct));
-    uint32_t* err = scr->counter + 1;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s, (Ray*)nullptr, total, (const uint32_t*)nullptr,
                        (Hit*)nullptr, (uint32_t*)nullptr, (uint8_t*)nullptr, parts, (uint2*)scr->spill, lv, refill,
                        err, d_segs, nseg);
