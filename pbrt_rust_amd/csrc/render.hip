@@ -271,10 +271,16 @@ __device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t t
 
 // Shade modes: SHADE_PLAIN writes every slot in place (dead slots carry an
 // empty interval) and counts live rays with one atomic per block;
-// SHADE_QUEUE writes the live rays densely, in sample order, at offsets from
-// a single-pass scan over the blocks (decoupled look-back), and the last
-// block writes their number.  With at most SH_CACHE slots per sample the
-// light samples are kept in registers between the count and the write.
+// SHADE_QUEUE writes the live rays densely: each block takes a run of the
+// queue for its live rays with one atomic on the count, in whatever order
+// the blocks get there, and orders its rays within the run.  A ray's result
+// does not depend on where it sits in the queue (the accumulation finds it
+// through qidx), so no block waits on another.  (A single-pass scan over the
+// blocks in block order, with decoupled look-back, had taken the
+// 1024x1024 direct-lighting frame's shading stage to 0.85 ms: the block,
+// alone on its CU, waited for its predecessors' prefixes.)  With at most
+// SH_CACHE slots per sample the light samples are kept in registers between
+// the count and the write.
 enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_QUEUE = 1 };
 // The path tracer's vertex kernel: PSHADE_PLAIN writes every path's slot,
 // PSHADE_QUEUE fills two dense queues (shadow and next rays) in queue order
@@ -295,23 +301,15 @@ constexpr int VERTEX_B = 1024;
 // sample list[q], and lbase, qidx and the accumulation are by entry.
 __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64_t g0, uint32_t n_host, const Ray* rays,
                                                      const Hit* hits, Ray* sh, float4* contrib,
-                                                     uint32_t* qidx, float4* lbase, uint32_t* sh_count,
-                                                     unsigned long long* tiles, uint32_t* ticket, int mode,
+                                                     uint32_t* qidx, float4* lbase, uint32_t* sh_count, int mode,
                                                      const uint32_t* list, const uint32_t* n_dev) {
     const uint32_t S = P.S;
     const uint32_t n = n_dev ? *n_dev : n_host;
-    __shared__ uint32_t s_bid, s_excl;
-    uint32_t bid = blockIdx.x;
-    if (mode == SHADE_QUEUE) {
-        // blocks take their sample ranges in start order (the look-back's
-        // progress guarantee)
-        if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
-        __syncthreads();
-        bid = s_bid;
-        // a launch sized for every sample: the blocks past the list's count
-        // (all after the last busy one, in ticket order) have nothing to do
-        if ((uint64_t)bid * SHADE_B >= n) return;
-    }
+    __shared__ uint32_t s_excl;
+    const uint32_t bid = blockIdx.x;
+    // a launch sized for every sample: the blocks past the list's count have
+    // nothing to do
+    if ((uint64_t)bid * SHADE_B >= n) return;
     const uint32_t ent = bid * SHADE_B + threadIdx.x;  // entry
     const bool live = ent < n;
     const uint32_t i = live ? (list ? list[ent] : ent) : 0u;  // its sample
@@ -424,13 +422,7 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
             run += bt;
         }
     }
-    if (threadIdx.x < 64) {
-        const uint32_t e = lookback(tiles, bid, tot);
-        if (threadIdx.x == 0) {
-            s_excl = e;
-            if ((bid + 1) * SHADE_B >= n) *sh_count = e + tot;  // the last block
-        }
-    }
+    if (threadIdx.x == 0) s_excl = tot ? atomicAdd(sh_count, tot) : 0u;  // the block's run of the queue
     __syncthreads();
     uint32_t pos = s_excl + local;  // sample-major order when the slots are not kept
     for (uint32_t k = 0; k < S; ++k) {
@@ -2015,13 +2007,10 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         a = sg.begin();
         {
             const uint32_t nb = (n + SHADE_B - 1) / SHADE_B;
-            // the shadow-ray count, and (batched) the look-back tiles (two
-            // words per block) with the block ticket
-            if (batched) ICHK(W.bcounts.ensure(2 * (size_t)nb + 2));
-            RCHK(zero3_u32(cnt + 5, 1, W.bcounts.p, batched ? 2 * (size_t)nb + 2 : 0, nullptr, 0, st));
+            RCHK(fill_u32(cnt + 5, 0u, 1, st));  // the shadow-ray count
             hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(SHADE_B), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
-                               W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (unsigned long long*)W.bcounts.p,
-                               W.bcounts.p + 2 * (size_t)nb, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN), list, nlist);
+                               W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN),
+                               list, nlist);
             RCHK(hipGetLastError());
         }
         RCHK(sg.end(a, &Timing::other));
