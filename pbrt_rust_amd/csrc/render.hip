@@ -199,76 +199,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 }
 
 
-// Decoupled look-back (D. Merrill and M. Garland, "Single-pass Parallel
-// Prefix Scan with Decoupled Look-back", NVIDIA technical report
-// NVR-2016-002), run by one wave of a block: publishes the block's total,
-// returns the sum of the totals of the blocks before it, then publishes its
-// inclusive prefix.  tiles[b] = (flag << 32) | value, flag 0 not yet
-// published, 1 block total, 2 inclusive prefix.  Each poll reads LB_PER
-// predecessors per lane (64 * LB_PER per poll); windows of 256 and 512
-// tiles per poll measured no faster than 64 on the path tracer's queues, so
-// one per lane.
-// Block ids are taken in start order, so every block looked at has started,
-// and it publishes its total without waiting on any other block: each wait
-// ends.
-constexpr int LB_PER = 1;
-__device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t tot) {
-    const uint32_t lane = __lane_id();
-    if (lane == 0)
-        __hip_atomic_store(tiles + bid, ((unsigned long long)(bid == 0 ? 2u : 1u) << 32) | tot, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t excl = 0;
-    if (bid > 0) {
-        long long top = (long long)bid - 1;
-        while (true) {
-            // lane l looks at tiles top - l - 64q, q = 0..LB_PER-1: distance
-            // order is (q, lane), and the nearest 64 -- the ones most likely
-            // still unpublished -- are waited for by 64 lanes at once.
-            // Before tile 0: an inclusive 0.
-            unsigned long long v[LB_PER];
-#pragma unroll
-            for (int q = 0; q < LB_PER; ++q) {
-                const long long j = top - (long long)lane - 64ll * q;
-                v[q] = j >= 0 ? __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ull << 32);
-            }
-            while (true) {
-                bool pending = false;
-#pragma unroll
-                for (int q = 0; q < LB_PER; ++q) pending |= (v[q] >> 32) == 0;
-                if (!pending) break;
-                __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-                for (int q = 0; q < LB_PER; ++q) {
-                    const long long j = top - (long long)lane - 64ll * q;
-                    if ((v[q] >> 32) == 0)  // not published yet (so j >= 0)
-                        v[q] = __hip_atomic_load(tiles + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-            // the nearest inclusive prefix: the first q with one, its first lane
-            int qs = LB_PER;
-            unsigned long long incl = 0;
-#pragma unroll
-            for (int q = LB_PER - 1; q >= 0; --q) {
-                const unsigned long long m = __ballot((v[q] >> 32) == 2);
-                if (m) { qs = q; incl = m; }
-            }
-            const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-            // the tiles from the nearest one back to the nearest inclusive prefix
-            uint32_t val = 0;
-#pragma unroll
-            for (int q = 0; q < LB_PER; ++q)
-                if (q < qs || (q == qs && lane <= stop)) val += (uint32_t)v[q];
-            for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off);
-            excl += val;
-            if (incl) break;
-            top -= 64 * LB_PER;
-        }
-        if (lane == 0)
-            __hip_atomic_store(tiles + bid, (2ull << 32) | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return excl;
-}
-
 // Shade modes: SHADE_PLAIN writes every slot in place (dead slots carry an
 // empty interval) and counts live rays with one atomic per block;
 // SHADE_QUEUE writes the live rays densely: each block takes a run of the
@@ -283,16 +213,15 @@ __device__ uint32_t lookback(unsigned long long* tiles, uint32_t bid, uint32_t t
 // the count and the write.
 enum ShadeMode : int { SHADE_PLAIN = 0, SHADE_QUEUE = 1 };
 // The path tracer's vertex kernel: PSHADE_PLAIN writes every path's slot,
-// PSHADE_QUEUE fills two dense queues (shadow and next rays) in queue order
-// with a decoupled look-back per queue.
+// PSHADE_QUEUE fills two dense queues (shadow and next rays), a run of each
+// per tile.
 enum PathShadeMode : int { PSHADE_PLAIN = 0, PSHADE_QUEUE = 1 };
-static_assert(RB >= 128, "the path vertex kernel runs its two look-backs on two waves");
 constexpr uint32_t SH_CACHE = 4;
 
-// Threads per block of the shade kernels: a block is one look-back tile, and
-// every tile costs a ticket and published words on shared lines (256 / 512 /
-// 1024 threads: direct-lighting frame 8.63 / 8.41 / 8.32 ms, path-tracing
-// frame 24.3 / 23.0 / 22.6 ms).
+// Threads per block of the shade kernels: a block is one tile of queue
+// output, and every tile costs atomics on shared counters (with block-order
+// scans, 256 / 512 / 1024 threads: direct-lighting frame 8.63 / 8.41 / 8.32
+// ms, path-tracing frame 24.3 / 23.0 / 22.6 ms).
 constexpr int SHADE_B = 1024;
 constexpr int VERTEX_B = 1024;
 
@@ -670,11 +599,11 @@ __device__ __forceinline__ void path_vertex(const RenderParams& P, uint64_t g0, 
 
 // Bounce 0 of fused batches: a camera ray that missed ends its path with
 // zero radiance (no infinite lights), written here; the paths whose rays hit
-// are listed densely in path order, and only they go through the vertex
-// kernel.  A block takes a tile of MISS_B * MISS_PER paths in ticket order
-// (the look-back's progress guarantee); round k of it reads paths tile +
-// k*MISS_B + thread (coalesced), and list positions follow path order:
-// round-major, then wave, then lane.  SEG (a served batch of several
+// are listed densely, and only they go through the vertex kernel.  A block
+// takes a tile of MISS_B * MISS_PER paths; round k of it reads paths tile +
+// k*MISS_B + thread (coalesced).  The tile's hits take a run of the list
+// with one atomic on the count (tiles in whatever order they get there), in
+// path order within the run: round-major, then wave, then lane.  SEG (a served batch of several
 // requests): a path's radiance goes to its segment's request, and a hit's
 // path starts here -- its state (segment in the flags), its camera ray and
 // its hit record stored at its queue position, as the vertex kernel reads a
@@ -683,15 +612,11 @@ constexpr uint32_t MISS_B = 1024, MISS_PER = 8;
 template <bool SEG>
 __global__ __launch_bounds__(MISS_B) void k_path_miss0(RenderParams P, uint64_t g0, uint32_t n, const Hit* hits,
                                                        float4* sample_l, uint32_t* list, uint32_t* count,
-                                                       unsigned long long* tiles, uint32_t* ticket, Ray* rays_q,
-                                                       Hit* hits_q, PathState* st) {
+                                                       Ray* rays_q, Hit* hits_q, PathState* st) {
     constexpr uint32_t W = MISS_B / 64;
-    __shared__ uint32_t s_bid, s_excl;
+    __shared__ uint32_t s_excl;
     __shared__ uint32_t s_pre[MISS_PER * W];  // hits before (round, wave) in path order
-    if (threadIdx.x == 0) s_bid = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t bid = s_bid;
-    const uint32_t base = bid * (MISS_B * MISS_PER);
+    const uint32_t base = blockIdx.x * (MISS_B * MISS_PER);
     const uint32_t wid = threadIdx.x / 64, lane = __lane_id();
     const unsigned long long ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const CamSeg* segs = reinterpret_cast<const CamSeg*>(P.segs);
@@ -731,11 +656,7 @@ __global__ __launch_bounds__(MISS_B) void k_path_miss0(RenderParams P, uint64_t 
         const uint32_t ex = x - (a + b);
         s_pre[2 * lane] = ex;
         s_pre[2 * lane + 1] = ex + a;
-        const uint32_t e = lookback(tiles, bid, tot);
-        if (lane == 0) {
-            s_excl = e;
-            if ((uint64_t)base + MISS_B * MISS_PER >= n) *count = e + tot;  // the last tile
-        }
+        if (lane == 0) s_excl = tot ? atomicAdd(count, tot) : 0u;  // the tile's run of the list
     }
     __syncthreads();
     const uint32_t e = s_excl;
@@ -758,20 +679,21 @@ __global__ __launch_bounds__(MISS_B) void k_path_miss0(RenderParams P, uint64_t 
 }
 
 // The path vertex kernel.  PSHADE_QUEUE: persistent blocks take tiles of
-// RB * PATH_PER paths (PATH_PER consecutive paths a thread) in ticket order
-// (the look-back's progress guarantee), and write the tile's shadow and
-// next rays densely in path order at offsets found by one look-back per
-// queue; a ticket is a device-scope atomic on one address, so fewer, larger
-// tiles (VERTEX_B * PATH_PER paths).  PSHADE_PLAIN: one path per thread, every path's slot written.
+// VERTEX_B * PATH_PER paths (PATH_PER consecutive paths a thread) by ticket,
+// and write the tile's shadow and next rays densely, each queue's rays in
+// path order within a run that the tile takes with one atomic on that
+// queue's count (tiles in whatever order they get there: a ray's result does
+// not depend on its queue position, and the path id travels with it).  A
+// ticket is a device-scope atomic on one address, so fewer, larger tiles.
+// PSHADE_PLAIN: one path per thread, every path's slot written.
 constexpr uint32_t PATH_PER = 2;
 __global__ __launch_bounds__(VERTEX_B) void k_path_shade(RenderParams P, uint64_t g0, uint32_t bounce, uint32_t cap,
                                                    const uint32_t* q_count, const uint32_t* pid_q,
                                                    const Ray* rays_in, const Hit* hits, PathState* st,
                                                    Ray* rays_out, uint32_t* pid_out, uint32_t* out_count,
                                                    Ray* sh, uint32_t* sh_pid, float4* shc,
-                                                   uint32_t* sh_count, float4* sample_l,
-                                                   unsigned long long* tiles_s, unsigned long long* tiles_n,
-                                                   uint32_t* ticket, int mode) {
+                                                   uint32_t* sh_count, float4* sample_l, uint32_t* ticket,
+                                                   int mode) {
     const uint32_t n = q_count ? *q_count : cap;
     const bool fresh = rays_in == nullptr;
     if (mode == PSHADE_PLAIN) {
@@ -820,18 +742,9 @@ __global__ __launch_bounds__(VERTEX_B) void k_path_shade(RenderParams P, uint64_
         uint32_t tot;
         const uint32_t ex = block_exclusive_scan<VERTEX_B>(cs | (cn << 16), &tot);
         const uint32_t ts = tot & 0xFFFFu, tn = tot >> 16;
-        // the two look-backs side by side: wave 0 the shadow queue, wave 1
-        // the next-ray queue
-        if (threadIdx.x < 128) {
-            const bool shq = threadIdx.x < 64;
-            const uint32_t t = shq ? ts : tn;
-            const uint32_t e = lookback(shq ? tiles_s : tiles_n, bid, t);
-            if (__lane_id() == 0) {
-                if (shq) s_es = e;
-                else s_en = e;
-                if (bid == ntiles - 1) *(shq ? sh_count : out_count) = e + t;  // the last tile
-            }
-        }
+        // the tile's runs of the two queues
+        if (threadIdx.x == 0) s_es = ts ? atomicAdd(sh_count, ts) : 0u;
+        if (threadIdx.x == 64) s_en = tn ? atomicAdd(out_count, tn) : 0u;
         __syncthreads();
         uint32_t qs = s_es + (ex & 0xFFFFu), qn = s_en + (ex >> 16);
 #pragma unroll
@@ -1647,7 +1560,7 @@ int reserve_path_work(WorkBufs& W, uint32_t cap, bool osort) {
     ICHK(W.pid.ensure(cap));
     ICHK(W.pid2.ensure(cap));
     ICHK(W.shpid.ensure(cap));
-    ICHK(W.bcounts.ensure(4 * (size_t)((cap + RB - 1) / RB) + 2));
+    ICHK(W.bcounts.ensure(2));
     if (osort) {
         ICHK(W.rays3.ensure(cap));
         ICHK(W.pid3.ensure(cap));
@@ -1721,16 +1634,13 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             // the misses end here; the hits' path ids become the queue
             a = sg.begin();
             const uint32_t mb = (uint32_t)(((uint64_t)n + MISS_B * MISS_PER - 1) / (MISS_B * MISS_PER));
-            ICHK(W.bcounts.ensure(2 * (size_t)mb + 2));
-            RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)mb + 2, st));
+            RCHK(fill_u32(cin, 0u, 1, st));  // the list's count
             if (fused)
                 hipLaunchKernelGGL(k_path_miss0<false>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
-                                   pin, cin, (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb,
-                                   (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
+                                   pin, cin, (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
             else
                 hipLaunchKernelGGL(k_path_miss0<true>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
-                                   pin, cin, (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb, rin,
-                                   W.hits2.p, W.st.p);
+                                   pin, cin, rin, W.hits2.p, W.st.p);
             RCHK(hipGetLastError());
             RCHK(sg.end(a, &Timing::other));
         }
@@ -1740,11 +1650,9 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         a = sg.begin();
         {
             const uint32_t nb = (n + VERTEX_B - 1) / VERTEX_B;
-            // the two queues' counts, and the look-back tiles of the two
-            // queues (two words per block each) with the block ticket
-            ICHK(W.bcounts.ensure(4 * (size_t)nb + 2));
-            RCHK(zero3_u32(cout, 1, cnt + 5, 1, W.bcounts.p, batched ? 4 * (size_t)nb + 2 : 0, st));
-            unsigned long long* tiles = (unsigned long long*)W.bcounts.p;
+            // the two queues' counts and (queue form) the tile ticket
+            ICHK(W.bcounts.ensure(2));
+            RCHK(zero3_u32(cout, 1, cnt + 5, 1, W.bcounts.p, batched ? 1 : 0, st));
             // queue form: persistent blocks over the live paths' tiles
             const uint32_t grid = batched ? std::min<uint32_t>(nb, (uint32_t)device_cu_count() * 8u) : nb;
             hipLaunchKernelGGL(k_path_shade, dim3(grid), dim3(VERTEX_B), 0, st, P, g0, bounce, n,
@@ -1753,8 +1661,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
                                (fused && bounce == 0) ? (const Ray*)nullptr : rin,
                                (fused_seg && bounce == 0) ? W.hits2.p : W.hits.p, W.st.p, rout,
                                batched ? pout : nullptr, cout, W.sh.p, batched ? W.shpid.p : nullptr, W.contrib.p,
-                               cnt + 5, sample_l, tiles, tiles + nb, W.bcounts.p + 4 * (size_t)nb,
-                               (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
+                               cnt + 5, sample_l, W.bcounts.p, (int)(batched ? PSHADE_QUEUE : PSHADE_PLAIN));
             RCHK(hipGetLastError());
         }
         RCHK(sg.end(a, &Timing::other));
@@ -1990,15 +1897,15 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         // it saves (512x512 at 1 spp: 0.348 -> 0.354 ms a frame).
         const uint32_t* list = nullptr;
         const uint32_t* nlist = nullptr;
-        if (batched && n >= (1u << 20)) {
+        const bool listed = batched && n >= (1u << 20);
+        // the count of the list of hits (listed) and of the shadow rays
+        RCHK(zero3_u32(cnt + 5, 1, listed ? cnt + 1 : nullptr, listed ? 1 : 0, nullptr, 0, st));
+        if (listed) {
             a = sg.begin();
             ICHK(W.pid.ensure(n));
             const uint32_t mb = (uint32_t)(((uint64_t)n + MISS_B * MISS_PER - 1) / (MISS_B * MISS_PER));
-            ICHK(W.bcounts.ensure(2 * (size_t)mb + 2));
-            RCHK(fill_u32(W.bcounts.p, 0u, 2 * (size_t)mb + 2, st));
             hipLaunchKernelGGL(k_path_miss0<false>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, R->sample_l.p,
-                               W.pid.p, cnt + 1, (unsigned long long*)W.bcounts.p, W.bcounts.p + 2 * (size_t)mb,
-                               (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
+                               W.pid.p, cnt + 1, (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
             RCHK(hipGetLastError());
             list = W.pid.p;
             nlist = cnt + 1;
@@ -2007,7 +1914,6 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         a = sg.begin();
         {
             const uint32_t nb = (n + SHADE_B - 1) / SHADE_B;
-            RCHK(fill_u32(cnt + 5, 0u, 1, st));  // the shadow-ray count
             hipLaunchKernelGGL(k_shade_direct, dim3(nb), dim3(SHADE_B), 0, st, P, g0, n, crays, W.hits.p, W.sh.p,
                                W.contrib.p, W.qidx.p, W.lbase.p, cnt + 5, (int)(batched ? SHADE_QUEUE : SHADE_PLAIN),
                                list, nlist);

@@ -1178,8 +1178,8 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
-// Three word ranges zeroed by one launch (the counters and look-back tiles
-// a stage needs cleared, one launch instead of one each).
+// Three word ranges zeroed by one launch (the counters a stage needs
+// cleared, one launch instead of one each).
 __global__ void k_zero3_u32(uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc) {
     const size_t n = na + nb + nc;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
