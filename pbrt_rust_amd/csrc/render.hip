@@ -222,7 +222,7 @@ constexpr uint32_t SH_CACHE = 4;
 // output, and every tile costs atomics on shared counters (with block-order
 // scans, 256 / 512 / 1024 threads: direct-lighting frame 8.63 / 8.41 / 8.32
 // ms, path-tracing frame 24.3 / 23.0 / 22.6 ms).
-constexpr int SHADE_B = 1024;
+constexpr int SHADE_B = 512;
 constexpr int VERTEX_B = 1024;
 
 // list (queue form, with its count at n_dev): the samples to shade, those
@@ -303,12 +303,14 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
     // queue index of slot k of sample i is stored at k*n + i (contiguous per
     // wave as well); the contribution of a traced ray sits at its queue
     // position.
+    // A shadow ray starts at the shading point with tmin = eps for every
+    // slot: only its direction and tmax are kept, with the contribution.
     const bool cached = S <= SH_CACHE;
-    Ray cr[SH_CACHE];
-    f3 cc[SH_CACHE];
+    f3 cd[SH_CACHE], cc[SH_CACHE];
+    float ct[SH_CACHE];
     uint32_t vm = 0, cnt = 0;
     for (uint32_t k = 0; k < S; ++k) {
-        Ray sr;
+        Ray sr{};
         f3 c = mk3(0, 0, 0);
         const bool valid = slot_sample(k, &sr, &c);
         if (valid) {
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
         if (cached) {
 #pragma unroll
             for (uint32_t kk = 0; kk < SH_CACHE; ++kk)
-                if (kk == k) { cr[kk] = sr; cc[kk] = c; }
+                if (kk == k) { cd[kk] = mk3(sr.dx, sr.dy, sr.dz); ct[kk] = sr.tmax; cc[kk] = c; }
         }
     }
     // rank of each lane's slot k among the block's valid slots k, and the
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(SHADE_B) void k_shade_direct(RenderParams P, uint64
             valid = (vm >> k) & 1u;
 #pragma unroll
             for (uint32_t kk = 0; kk < SH_CACHE; ++kk)
-                if (kk == k) { sr = cr[kk]; c = cc[kk]; }
+                if (kk == k) { sr = Ray{p.x, p.y, p.z, eps, cd[kk].x, cd[kk].y, cd[kk].z, ct[kk]}; c = cc[kk]; }
         } else {
             valid = slot_sample(k, &sr, &c);
         }
