@@ -803,7 +803,8 @@ __global__ __launch_bounds__(RB) void k_path_shadow(RenderParams P, uint64_t g0,
 // holds 64 or more of them, while rays stay near their queue neighbours
 // (nearby pixels).  Octant bit a is the sign bit of d[a] (the sign of 1/d
 // that picks the traversal's near child, -0 included).  Positions depend
-// only on the input order, so the queue order is deterministic.
+// only on the input order (which itself depends on the order in which the
+// vertex kernel's tiles took their runs; no result depends on either).
 constexpr uint32_t OS_BLOCK = 1024, OS_PER = 4, OS_TILE = OS_BLOCK * OS_PER;
 __device__ __forceinline__ uint32_t octant_of(const Ray& r) {
     return (__float_as_uint(r.dx) >> 31) | ((__float_as_uint(r.dy) >> 31) << 1) | ((__float_as_uint(r.dz) >> 31) << 2);
