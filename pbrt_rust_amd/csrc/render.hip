@@ -414,11 +414,15 @@ __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0
 }
 
 // 64-bit running total (lo, hi words) += *add
-__global__ void k_add_counter(uint32_t* lo, const uint32_t* add, uint32_t* hi) {
+// hi:lo += *add; then (when given) *add and *clr are cleared for the next
+// chunk, which saves that chunk a clearing launch.
+__global__ void k_add_counter(uint32_t* lo, uint32_t* add, uint32_t* hi, bool clear_add, uint32_t* clr) {
     uint64_t v = ((uint64_t)*hi << 32) | (uint64_t)*lo;
     v += *add;
     *lo = (uint32_t)v;
     *hi = (uint32_t)(v >> 32);
+    if (clear_add) *add = 0u;
+    if (clr) *clr = 0u;
 }
 
 // ---- path tracing ------------------------------------------------------------
@@ -1689,7 +1693,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
                            batched ? cnt + 5 : (const uint32_t*)nullptr,
                            batched ? W.shpid.p : (const uint32_t*)nullptr, W.contrib.p, W.occ.p, W.st.p, sample_l);
         RCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, sst, cnt + 6, cnt + 5, cnt + 7);
+        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, sst, cnt + 6, cnt + 5, cnt + 7, false, (uint32_t*)nullptr);
         RCHK(hipGetLastError());
         if (split) {
             RCHK(ssg.end(a, &Timing::other));
@@ -1699,7 +1703,7 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             a = sg.begin();
         }
         if (bounce < P.max_depth)
-            hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9);
+            hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 8, cout, cnt + 9, false, (uint32_t*)nullptr);
         RCHK(hipGetLastError());
         if (osort && bounce < P.max_depth) {
             const uint32_t tiles = (n + OS_TILE - 1) / OS_TILE;
@@ -1900,9 +1904,10 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         // it saves (512x512 at 1 spp: 0.348 -> 0.354 ms a frame).
         const uint32_t* list = nullptr;
         const uint32_t* nlist = nullptr;
+        // (the count of the list of hits and that of the shadow rays start
+        // at zero: cleared with the frame's counters, and again at the end
+        // of each chunk)
         const bool listed = batched && n >= (1u << 20);
-        // the count of the list of hits (listed) and of the shadow rays
-        RCHK(zero3_u32(cnt + 5, 1, listed ? cnt + 1 : nullptr, listed ? 1 : 0, nullptr, 0, st));
         if (listed) {
             a = sg.begin();
             ICHK(W.pid.ensure(n));
@@ -1935,8 +1940,9 @@ static int enqueue_frame(Renderer* R, int path, TraceScratch* scr, Stager& sg, u
         hipLaunchKernelGGL(k_accum_direct, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.lbase.p, W.contrib.p,
                            W.qidx.p, W.occ.p, R->sample_l.p, list, nlist);
         RCHK(hipGetLastError());
-        // running total of shadow rays
-        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7);
+        // running total of shadow rays; the shadow-ray and list counts
+        // cleared for the next chunk
+        hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(1), 0, st, cnt + 6, cnt + 5, cnt + 7, true, cnt + 1);
         RCHK(hipGetLastError());
         RCHK(sg.end(a, &Timing::other));
     }
