@@ -378,3 +378,31 @@ def test_occlusion_lanes_match_oracle(occlusion_lanes):
     test_eight_spheres_prim_ids("batched", "sah")
     test_paths_agree_at_full_size()
     px.synchronize()
+
+
+@pytest.mark.parametrize("n", [100, 110])
+def test_deep_chain_occlusion_group_stack(occlusion_lanes, n):
+    # triangles in the planes x = 2^k, split at the middle: a chain of
+    # occlusion records 36 levels deep (n = 100: the group stack's bound
+    # leaves room for multi-entry steps only near its bottom, so most steps
+    # take one entry) or 41 (n = 110: too deep for the group stack, so the
+    # per-ray kernel runs).  Rays along +x inside every box: ones that hit
+    # the k-th triangle past tmin, and ones outside the triangles (y + z > 1)
+    # that visit the whole chain
+    xs = 2.0 ** np.arange(n)
+    v = np.concatenate([np.array([[x, 0, 0], [x, 1, 0], [x, 0, 1]]) for x in xs]).astype(F32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    bvh, orc = mesh_pair(v, idx, "middle")
+    assert bvh.occlusion_records()[1] == {100: 36, 110: 41}[n]
+    rng = np.random.default_rng(n)
+    m = 3000
+    k = rng.integers(0, n, m)
+    tmin = np.where(k == 0, 0.0, xs[np.maximum(k - 1, 0)] * 1.5).astype(F32)
+    yz = rng.uniform(0.05, 0.45, (m, 2))
+    out = rng.random(m) < 0.5
+    yz[out] = rng.uniform(0.55, 0.95, (int(out.sum()), 2))
+    o = np.concatenate([np.full((m, 1), -1.0), yz], 1).astype(F32)
+    d = np.tile(np.array([[1, 0, 0]], F32), (m, 1))
+    occ = check_any(bvh, orc, o, d, mint=tmin, path="batched")
+    assert occ[~out].mean() > 0.9 and not occ[out].any()
+    px.synchronize()  # raises if a traversal stack overflowed
