@@ -92,6 +92,15 @@ int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out) {
     return PBRTX_OK;
 }
 
+void release_stream_scratch(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    auto it = g_scratch.find(st);
+    if (it == g_scratch.end()) return;
+    if (it->second.counter) (void)hipFree(it->second.counter);
+    if (it->second.spill) (void)hipFree(it->second.spill);
+    g_scratch.erase(it);
+}
+
 int upload(void** d, const void* h, size_t bytes) {
     *d = nullptr;
     if (bytes == 0) return PBRTX_OK;

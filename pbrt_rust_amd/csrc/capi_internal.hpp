@@ -18,6 +18,8 @@ struct DeviceScene {
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int stream_scratch(hipStream_t st, size_t spill, TraceScratch** out);
+// Frees the scratch of a stream about to be destroyed (its work finished).
+void release_stream_scratch(hipStream_t st);
 // the render description's split value for a tree built on the device
 constexpr int32_t SPLIT_GPU_LBVH = 3;
 // Builds a linear BVH over mesh faces then spheres on the device, straight

@@ -2181,8 +2181,12 @@ int pbrtx_renderer_destroy(pbrtx_renderer* h) {
     R.sc.release();
     R.W.release();
     R.sample_l.release(); R.counters.release(); R.rgb.release(); R.ftab.release(); R.fpos.release(); R.fxyz.release(); R.spos.release(); R.scount.release();
-    if (R.stream) (void)hipStreamDestroy(R.stream);
-    if (R.stream2) (void)hipStreamDestroy(R.stream2);
+    for (hipStream_t st : {R.stream, R.stream2})
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            release_stream_scratch(st);
+            (void)hipStreamDestroy(st);
+        }
     delete h;
     return PBRTX_OK;
 }

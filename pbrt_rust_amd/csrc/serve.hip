@@ -560,7 +560,10 @@ int pbrtx_server_destroy(pbrtx_server* p) {
     if (s.h_segs) (void)hipHostFree(s.h_segs);
     if (s.d_segs) (void)hipFree(s.d_segs);
     for (hipStream_t st : {s.s_comp, s.s_up, s.s_down})
-        if (st) (void)hipStreamDestroy(st);
+        if (st) {
+            release_stream_scratch(st);
+            (void)hipStreamDestroy(st);
+        }
     delete p;
     return PBRTX_OK;
 }
