@@ -380,20 +380,43 @@ def test_occlusion_lanes_match_oracle(occlusion_lanes):
     px.synchronize()
 
 
-@pytest.mark.parametrize("n", [100, 110])
-def test_deep_chain_occlusion_group_stack(occlusion_lanes, n):
-    # triangles in the planes x = 2^k, split at the middle: a chain of
-    # occlusion records 36 levels deep (n = 100: the group stack's bound
-    # leaves room for multi-entry steps only near its bottom, so most steps
-    # take one entry) or 41 (n = 110: too deep for the group stack, so the
-    # per-ray kernel runs).  Rays along +x inside every box: ones that hit
-    # the k-th triangle past tmin, and ones outside the triangles (y + z > 1)
-    # that visit the whole chain
+def test_occlusion_group_stack_fills(occlusion_lanes):
+    # 2000 thin triangles along the diagonal of the square [-1,1]^2, stacked
+    # in z: every box spans the square, so a ray along z away from the
+    # diagonal passes every box and misses every triangle.  Its frontier is
+    # the whole tree: the group stacks fill to their bound and continue a
+    # depth-first walk from there.  Rays near the diagonal's wide end hit.
+    n = 2000
+    rng = np.random.default_rng(9)
+    z = np.sort(rng.uniform(-1, 1, n))
+    w = 0.05
+    v = np.concatenate([np.array([[-1, -1, zz], [1, 1, zz], [-1 + w, -1, zz]]) for zz in z]).astype(F32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    bvh, orc = mesh_pair(v, idx)
+    m = 4000
+    xy = rng.uniform(-1, 1, (m, 2))
+    near = rng.random(m) < 0.3
+    xy[near, 0] = rng.uniform(-0.99, -0.97, int(near.sum()))
+    xy[near, 1] = xy[near, 0] - rng.uniform(0.0, 0.01, int(near.sum()))
+    o = np.concatenate([xy, np.full((m, 1), -2.0)], 1).astype(F32)
+    d = np.concatenate([rng.normal(0, 1e-3, (m, 2)), np.ones((m, 1))], 1).astype(F32)
+    occ = check_any(bvh, orc, o, d, path="batched")
+    assert 0.1 < occ.mean() < 0.5
+    px.synchronize()  # raises if a traversal stack overflowed
+
+
+def test_deep_chain_occlusion(occlusion_lanes):
+    # triangles in the planes x = 2^k, split at the middle: a reference tree
+    # 61 levels deep (its own records for closest hit; the occlusion records
+    # come from a tree built over the same leaves).  Rays along +x inside
+    # every box: ones that hit the k-th triangle past tmin, and ones outside
+    # the triangles (y + z > 1) that visit the whole chain
+    n = 100
     xs = 2.0 ** np.arange(n)
     v = np.concatenate([np.array([[x, 0, 0], [x, 1, 0], [x, 0, 1]]) for x in xs]).astype(F32)
     idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
     bvh, orc = mesh_pair(v, idx, "middle")
-    assert bvh.occlusion_records()[1] == {100: 36, 110: 41}[n]
+    assert bvh.info()["depth"] == 61
     rng = np.random.default_rng(n)
     m = 3000
     k = rng.integers(0, n, m)
