@@ -1371,14 +1371,24 @@ static bool use_compressed(const SceneView& s, const TraceTuning& tun) {
 //   512x512 1 spp, 82K-triangle sphere, 413,511 of 1,048,576: 0.311 / 0.173 / 0.161
 //   512x512 4 spp, that sphere, 1,654,016 of 4,194,304:       0.573 / 0.535 / 0.559
 //   1024x1024 16 spp, 1.3M-triangle sphere, 26.5M of 67.1M:   3.47 / 10.7 / 10.7
+// and from launches of random shadow segments (origins in the box, uniform
+// directions, lengths up to half its diagonal) in random-triangle soups,
+// four triangles a leaf (ms at 1 / 4 / 8 lanes per ray):
+//   200K triangles, 131,072 rays:     0.345 / 0.147 / 0.104
+//   200K triangles, 1,048,576 rays:   0.718 / 0.794 / 0.567
+//   200K triangles, 4,194,304 rays:   1.53 / 2.97 / 2.09
+//   2M triangles, 2,097,152 rays:     1.31 / 1.63 / 1.30
+//   2M triangles, 4,194,304 rays:     2.20 / 3.23 / 2.55
 // With few rays per SIMD the per-ray walk is a chain of dependent record
 // loads; with many, the group's idle lanes (a narrow frontier) cost more.
-constexpr uint32_t GROUP8_MAX = 1u << 20, GROUP4_MAX = 1u << 22;
+// Four lanes a ray hold more registers and LDS per lane and lose to eight
+// wherever either wins.
+constexpr uint32_t GROUP_MAX = 1u << 20;
 static int group_lanes(const SceneView& s, uint32_t n, const TraceTuning& tun) {
     if (tun.lanes == 1 || !s.cnodes || !s.prims64) return 1;
     if (3u * (uint64_t)s.odepth + 8u > (uint64_t)GSTK) return 1;
     if (tun.lanes >= 2) return tun.lanes;
-    return n <= GROUP8_MAX ? 8 : (n <= GROUP4_MAX ? 4 : 1);
+    return n <= GROUP_MAX ? 8 : 1;
 }
 
 template <bool SEG>
