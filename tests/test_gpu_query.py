@@ -429,3 +429,22 @@ def test_deep_chain_occlusion(occlusion_lanes):
     occ = check_any(bvh, orc, o, d, mint=tmin, path="batched")
     assert occ[~out].mean() > 0.9 and not occ[out].any()
     px.synchronize()  # raises if a traversal stack overflowed
+
+
+@pytest.mark.parametrize("max_prims,split", [(4, "sah"), (8, "equal"), (3, "middle")])
+def test_occlusion_leaves_of_several_prims(max_prims, split):
+    # leaves of several prims: the occlusion records' own tree is built over
+    # the reference's leaves, each leaf kept whole; occlusion at one and at
+    # eight lanes per ray, and closest hit, against the oracle
+    v, idx = scenes.random_triangles(20000, 40 + max_prims)
+    bvh, orc = mesh_pair(v, idx, split, max_prims)
+    o, d = random_rays(30000, 41)
+    try:
+        for lanes in (1, 8):
+            px.set_tuning("batched", "any", 256, 0, 60, "ray", lanes=lanes)
+            check_any(bvh, orc, o, d, path="batched")
+            check_any(bvh, orc, o, d, mint=1e-3, maxt=0.4, path="batched")
+    finally:
+        px.reset_tuning()
+    check_closest(bvh, orc, o, d, path="batched")
+    px.synchronize()
