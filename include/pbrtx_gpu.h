@@ -86,8 +86,9 @@ int pbrtx_bvh_export_layouts(const pbrtx_bvh* bvh, void* wide, void* compressed,
    their count (0 when the scene has none and any-hit queries use the
    compressed layout) and *depth their record levels; records may be NULL.
    No counterpart in the reference, whose any-hit query is the closest-hit
-   walk of the same binary tree (intersection.rs:62-64, bvh.rs:377-421); for
-   inspection and tests. */
+   walk of the same binary tree (intersection.rs:62-64, bvh.rs:377-421).
+   Their tree is the builder's own over that tree's leaves, which reaches
+   the same leaves.  For inspection and tests. */
 int pbrtx_bvh_occlusion_records(const pbrtx_bvh* bvh, uint32_t* n, uint32_t* depth, void* records);
 int pbrtx_bvh_free(pbrtx_bvh* bvh);
 
