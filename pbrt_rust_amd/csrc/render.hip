@@ -396,15 +396,39 @@ __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0
     for (uint32_t i = blockIdx.x * RB + threadIdx.x; i < n; i += gridDim.x * RB) {
         const float4 lb = lbase[i];
         f3 L = mk3(lb.x, lb.y, lb.z);
+        // with at most SH_CACHE slots, every slot's queue index, then its
+        // occlusion byte, then its contribution are loaded before any is
+        // used (three rounds of independent loads instead of a chain per
+        // slot); the sums run in the same order either way
+        float4 cv[SH_CACHE];
+        if (P.S <= SH_CACHE) {
+            uint32_t q[SH_CACHE];
+#pragma unroll
+            for (uint32_t k = 0; k < SH_CACHE; ++k) q[k] = k < P.S ? qidx[(size_t)k * n + i] : NONE;
+            bool lit[SH_CACHE];
+#pragma unroll
+            for (uint32_t k = 0; k < SH_CACHE; ++k) lit[k] = q[k] != NONE && !occ[q[k]];
+#pragma unroll
+            for (uint32_t k = 0; k < SH_CACHE; ++k) cv[k] = lit[k] ? contrib[q[k]] : make_float4(0, 0, 0, -1.0f);
+        }
         uint32_t k = 0;
         for (uint32_t li = 0; li < P.nlights; ++li) {
             const uint32_t ns = li < P.n_point ? 1u : P.light_samples;
             f3 Ld = mk3(0, 0, 0);
             for (uint32_t j = 0; j < ns; ++j, ++k) {
-                const uint32_t q = qidx[(size_t)k * n + i];
-                if (q == NONE) continue;
-                if (occ[q]) continue;  // occluded
-                const float4 c = contrib[q];
+                float4 c;
+                if (P.S <= SH_CACHE) {
+                    c = cv[0];
+#pragma unroll
+                    for (uint32_t kk = 1; kk < SH_CACHE; ++kk)
+                        if (kk == k) c = cv[kk];
+                    if (c.w < 0.0f) continue;  // no ray, or occluded
+                } else {
+                    const uint32_t q = qidx[(size_t)k * n + i];
+                    if (q == NONE) continue;
+                    if (occ[q]) continue;  // occluded
+                    c = contrib[q];
+                }
                 Ld = add3(Ld, mk3(c.x, c.y, c.z));
             }
             L = add3(L, divs3(Ld, (float)ns));
@@ -413,8 +437,7 @@ __global__ __launch_bounds__(RB) void k_accum_direct(RenderParams P, uint64_t g0
     }
 }
 
-// 64-bit running total (lo, hi words) += *add
-// hi:lo += *add; then (when given) *add and *clr are cleared for the next
+// hi:lo += *add (a 64-bit running total in two words); then (when given) *add and *clr are cleared for the next
 // chunk, which saves that chunk a clearing launch.
 __global__ void k_add_counter(uint32_t* lo, uint32_t* add, uint32_t* hi, bool clear_add, uint32_t* clr) {
     uint64_t v = ((uint64_t)*hi << 32) | (uint64_t)*lo;
