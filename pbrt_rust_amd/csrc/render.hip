@@ -1632,7 +1632,8 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
     }
     hipEvent_t a = sg.begin();
     if (fused || fused_seg) {
-        RCHK(fill_u32(cnt + 1, n, 1, st));
+        // fused: the camera kernel lists the hits and counts them here
+        RCHK(fill_u32(cnt + 1, fused ? 0u : n, 1, st));
     } else {
         hipLaunchKernelGGL(k_path_init, dim3(grid_for(n)), dim3(RB), 0, st, P, g0, n, W.rays.p, W.st.p,
                            batched ? W.pid.p : nullptr, batched ? cnt + 1 : nullptr);
@@ -1651,7 +1652,10 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
         a = sg.begin();
         const TraceTuning& tb = (bounce == 0 && !P.segs) ? tunCam : tunC;  // frames' bounce 0: camera rays
         if (fused && bounce == 0) {
-            RCHK(trace_camera(sv, cam_params(P, g0), n, W.hits.p, tb, scr, st));
+            // a missed camera ray's path ends in the traversal (zero
+            // radiance); the hits' path ids become the queue
+            const CamOut co{sample_l + g0, pin, cin};
+            RCHK(trace_camera(sv, cam_params(P, g0), n, W.hits.p, tb, scr, st, nullptr, &co));
         } else if (fused_seg && bounce == 0) {
             RCHK(trace_camera(sv, CamParams{}, n, W.hits.p, tunCam, scr, st, &ct));
         } else if (batched) {
@@ -1660,17 +1664,13 @@ int enqueue_path_batch(const RenderParams& P, const SceneView& sv, WorkBufs& W, 
             RCHK(trace_closest(sv, rin, n, W.hits.p, path, tb, scr, st));
         }
         RCHK(sg.end(a, &Timing::closest));
-        if ((fused || fused_seg) && bounce == 0) {
+        if (fused_seg && bounce == 0) {
             // the misses end here; the hits' path ids become the queue
             a = sg.begin();
             const uint32_t mb = (uint32_t)(((uint64_t)n + MISS_B * MISS_PER - 1) / (MISS_B * MISS_PER));
             RCHK(fill_u32(cin, 0u, 1, st));  // the list's count
-            if (fused)
-                hipLaunchKernelGGL(k_path_miss0<false>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
-                                   pin, cin, (Ray*)nullptr, (Hit*)nullptr, (PathState*)nullptr);
-            else
-                hipLaunchKernelGGL(k_path_miss0<true>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
-                                   pin, cin, rin, W.hits2.p, W.st.p);
+            hipLaunchKernelGGL(k_path_miss0<true>, dim3(mb), dim3(MISS_B), 0, st, P, g0, n, W.hits.p, sample_l,
+                               pin, cin, rin, W.hits2.p, W.st.p);
             RCHK(hipGetLastError());
             RCHK(sg.end(a, &Timing::other));
         }

@@ -951,10 +951,14 @@ __device__ __forceinline__ float4 cload4(const void* p, uint32_t k) {
 // its current tmax before the leaf's prims.  Interior records are the exact
 // 128-byte ones.  GEN: ray i is the camera ray of sample cam.g0 + i, made
 // here instead of read (no ray array; only the hits are written).
+// co.list (GEN, a frame's bounce 0 of path tracing): a camera ray that
+// misses ends its path here -- zero radiance written to co.miss[i], no hit
+// record -- and each packet's hits take a run of co.list (their sample
+// indices) with one atomic on co.count.
 template <bool GEN, bool SEGS = false>
 __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_closest_packet(SceneView sv, Ray* rays, uint32_t n_host,
                                                               const uint32_t* n_dev, Hit* hits, uint32_t* counter,
-                                                              uint32_t* err, CamParams cam, CamTable ct) {
+                                                              uint32_t* err, CamParams cam, CamTable ct, CamOut co) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     __shared__ int32_t s_ref[PWAVES][PSTK];
     __shared__ unsigned long long s_msk[PWAVES][PSTK];
@@ -1167,8 +1171,20 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
         }
         if (valid) {
             if (hprim != NO_HIT) hprim = sv.order[hprim];  // position -> prim id
-            hits[i] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
+            if (GEN && co.list && hprim == NO_HIT) co.miss[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            else hits[i] = Hit{hprim, hprim != NO_HIT ? r.tmax : 0.0f, hb1, hb2};
             if (!GEN && hprim != NO_HIT) rays[i].tmax = r.tmax;
+        }
+        if (GEN && co.list) {
+            const bool h = valid && hprim != NO_HIT;
+            const unsigned long long hm = __ballot(h);
+            if (hm) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(co.count, (uint32_t)__popcll(hm));
+                b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+                const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                if (h) co.list[b + (uint32_t)__popcll(hm & lt)] = i;
+            }
         }
     }
 }
@@ -1445,7 +1461,7 @@ static hipError_t launch(const SceneView& s, Ray* rays, uint32_t n, Hit* hits, u
         hipError_t e = fill_u32(scr->counter, 0u, 1, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_closest_packet<false>, dim3(wide_grid(k_closest_packet<false>, tun)), dim3(block), 0,
-                           st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{}, CamTable{});
+                           st, s, rays, n, n_dev, hits, scr->counter, err, CamParams{}, CamTable{}, CamOut{});
         return hipGetLastError();
     }
     if (!stack_supported(tun.stack)) return hipErrorInvalidValue;
@@ -1522,7 +1538,8 @@ int noalloc_scratch(TraceScratch* out, hipError_t* err) {
 }
 
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
-                        TraceScratch* scr, hipStream_t st, const CamTable* ct) {
+                        TraceScratch* scr, hipStream_t st, const CamTable* ct, const CamOut* co) {
+    if (co && ct) return hipErrorInvalidValue;  // the list form is for one camera
     if (n == 0) return hipSuccess;
     if (!camera_fusable(s, tun)) return hipErrorNotSupported;
     const int block = tun.block;
@@ -1554,11 +1571,11 @@ hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hi
     if (ct)
         hipLaunchKernelGGL((k_closest_packet<true, true>), dim3(wide_grid(k_closest_packet<true, true>, tun)),
                            dim3(block), 0, st, s, (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter,
-                           scr->counter + 1, c, *ct);
+                           scr->counter + 1, c, *ct, CamOut{});
     else
         hipLaunchKernelGGL((k_closest_packet<true, false>), dim3(wide_grid(k_closest_packet<true, false>, tun)),
                            dim3(block), 0, st, s, (Ray*)nullptr, n, (const uint32_t*)nullptr, hits, scr->counter,
-                           scr->counter + 1, c, CamTable{});
+                           scr->counter + 1, c, CamTable{}, co ? *co : CamOut{});
     return hipGetLastError();
 }
 

@@ -94,8 +94,16 @@ struct CamParams;
 // ct (served batches of several requests): the batch's segments and their
 // cameras instead of cam, whose samples [g0, g0 + n) are then unused.
 struct CamTable;
+// co (one camera): misses end in the kernel (zero radiance to co->miss[i], no
+// hit record) and the hits' sample indices are listed at co->list, their
+// number counted at co->count (cleared by the caller).
+struct CamOut {
+    float4* miss = nullptr;
+    uint32_t* list = nullptr;
+    uint32_t* count = nullptr;
+};
 hipError_t trace_camera(const SceneView& s, const CamParams& cam, uint32_t n, Hit* hits, const TraceTuning& tun,
-                        TraceScratch* scr, hipStream_t st, const CamTable* ct = nullptr);
+                        TraceScratch* scr, hipStream_t st, const CamTable* ct = nullptr, const CamOut* co = nullptr);
 bool camera_fusable(const SceneView& s, const TraceTuning& tun);
 
 // One request chunk of a served launch (see trace_segments).
