@@ -140,7 +140,7 @@ int pbrtx_intersect_p_noalloc(pbrtx_scene* scene, const pbrtx_ray* d_rays, uint3
 /* Launch shape of a path's closest-hit (kind 0), occlusion (kind 1) or
    renderer camera-ray (kind 2: closest hit of the coherent camera rays)
    kernel: block in {64,128,256}; blocks_per_cu 0 = occupancy maximum
-   (batched path); refill_pct = idle-lane refill threshold (batched path);
+   (batched path), else up to 64 (up to 1024 for kind 2); refill_pct = idle-lane refill threshold (batched path);
    scheme 0 = a traversal stack per ray over the compressed records (the
    exact ones when a scene has none), 1 = one stack per wave of 64
    consecutive rays in equal-sign groups (batched path, closest hit and

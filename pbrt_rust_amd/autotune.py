@@ -31,7 +31,9 @@ DEFAULT_SPACE = {
 # whole frame in one pass
 CHUNKS = [1 << 22, 1 << 24, 1 << 26, 0]
 # the packet scheme has no refill threshold
-PACKET_SPACE = {"block": [64, 128, 256], "blocks_per_cu": [0, 2, 4, 8, 16]}
+# camera packets run a fixed schedule over any grid: more blocks than fit at
+# once are started as others finish
+PACKET_SPACE = {"block": [64, 128, 256], "blocks_per_cu": [0, 8, 16, 32, 64, 128]}
 # on a direct-lighting frame the closest-hit stage traces the camera rays
 STAGE = {"camera": "ms_trace_closest", "any": "ms_trace_any"}
 KINDS = ("camera", "any")

@@ -48,15 +48,18 @@ bool g_camera_auto = true;
 
 // Camera rays of a frame with spp samples per pixel.  Unless set explicitly,
 // the batched path traces them with one stack per wave made in the
-// traversal (a wave's 64 camera rays are neighbouring samples of one row of
-// pixels), 256 threads and 16 blocks per CU: 1920x1080 at 64 spp, 19.4 ms a
-// frame; 1024x1024 at 16 spp, 7.09 ms against 7.36 with a stack per ray;
-// 512x512 at 1 spp, 0.464 ms against 0.469.  (With the packet kernel at
-// five waves per SIMD, before its register budget was cut to eight, a
-// stack per ray had won below 32 samples per pixel.)
+// traversal (a wave's 64 camera rays are neighbouring samples), 256 threads
+// and 64 blocks per CU: eight times the waves that fit at once, so the
+// fixed schedule hands each wave few packets and the hardware starts the
+// waves that wait as others finish.  Camera stage of the 1024x1024 16-spp
+// frame at 16 / 32 / 64 / 128 / 256 blocks per CU: 1.55 / 1.38 / 1.36 /
+// 1.40 / 1.40 ms; the 1920x1080 64-spp path-tracing frame 14.8 / 14.4 /
+// 14.1 / 14.1 / 14.1 ms; the 512x512 1-spp frame 0.195 / 0.196 / 0.196 /
+// 0.199 / 0.244 ms.  (Earlier: a stack per ray against one per wave, 7.36
+// against 7.09 ms on the 16-spp frame.)
 TraceTuning camera_tuning(int path, uint32_t spp) {
     (void)spp;
-    if (path == PATH_BATCHED && g_camera_auto) return TraceTuning{256, 16, 0, SCHEME_PACKET};
+    if (path == PATH_BATCHED && g_camera_auto) return TraceTuning{256, 64, 0, SCHEME_PACKET};
     return g_tuning[path][2];
 }
 
@@ -345,7 +348,11 @@ int pbrtx_set_tuning(int32_t path, int32_t kind, int32_t block, int32_t blocks_p
     if (path != 0 && path != 1) return fail(PBRTX_ERR_ARG, "unknown path");
     if (kind < 0 || kind > 2) return fail(PBRTX_ERR_ARG, "kind must be 0 (closest hit), 1 (occlusion) or 2 (camera)");
     if (block != 64 && block != 128 && block != 256) return fail(PBRTX_ERR_ARG, "block must be 64, 128 or 256");
-    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
+    // the camera kernel's fixed schedule takes any grid (more blocks than fit
+    // at once are run as others finish); the persistent kernels' spill
+    // stacks are sized by their grid
+    if (blocks_per_cu < 0 || blocks_per_cu > (kind == 2 ? 1024 : 64))
+        return fail(PBRTX_ERR_ARG, "blocks_per_cu out of range");
     if (refill_pct < 0 || refill_pct > 100) return fail(PBRTX_ERR_ARG, "refill_pct out of range");
     if (scheme != SCHEME_RAY && scheme != SCHEME_PACKET && scheme != SCHEME_RAY_EXACT)
         return fail(PBRTX_ERR_ARG, "unknown traversal scheme");
