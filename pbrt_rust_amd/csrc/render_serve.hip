@@ -43,7 +43,12 @@ namespace px {
 namespace {
 
 using Clock = std::chrono::steady_clock;
-constexpr int NSLOT = 3;               // batches in flight (one stream each)
+// Batches in flight, one stream each.  Saturated render serving (17 scenes,
+// 8M-sample batches) at 2 / 3 / 4 / 5 / 6 / 8 / 12 slots: 5.36 / 6.66-6.71 /
+// 7.52-7.86 / 7.48 / 7.93 / 8.07-8.43 / 8.08-8.18 Grays/s: a batch's later
+// bounces are small launches that leave most of the GPU to the others.
+// Each slot's work buffers are sized once for batch_samples.
+constexpr int NSLOT = 8;
 constexpr uint64_t SHARE_MIN = 65536;  // smallest per-request share of a batch (samples)
 
 struct RRequest {
