@@ -38,6 +38,9 @@
 namespace px {
 namespace {
 
+// Streams beside the compute stream for the launches of one step.
+constexpr int SIDE_STREAMS = 3;
+
 using Clock = std::chrono::steady_clock;
 constexpr uint32_t CHUNK_MIN = 65536;  // smallest per-request share of a step (rays)
 
@@ -88,6 +91,12 @@ struct Server {
     size_t spill_need = 0;                   // guarded by mu
     // compute, upload and download streams (all default priority)
     hipStream_t s_comp = nullptr, s_up = nullptr, s_down = nullptr;
+    // a step's launches (one per scene and kind) go to s_comp and these in
+    // turn, so small launches of different scenes run side by side; each
+    // waits for the step's segment upload (ev_up), and s_comp for each of
+    // them (ev_side) before the step's end
+    hipStream_t s_side[SIDE_STREAMS] = {};
+    hipEvent_t ev_up = nullptr, ev_side[SIDE_STREAMS] = {};
     TraceSeg* h_segs = nullptr;  // pinned staging of the segment tables, two halves
     TraceSeg* d_segs = nullptr;
     uint32_t seg_cap = 0;        // per half
@@ -244,6 +253,8 @@ int Server::launch_step() {
     if (e != hipSuccess) return hip_fail(e, "step start");
     uint32_t si = 0;
     std::vector<Request*> last;
+    struct Launch { const DeviceScene* sc; bool any; uint32_t s0, nseg, rays; };
+    std::vector<Launch> launches_v;
     for (auto& g : groups) {
         const DeviceScene* sc = scn[g.first.first];
         const bool any = g.first.second == 1;
@@ -263,16 +274,33 @@ int Server::launch_step() {
             if (r->t_first < 0) r->t_first = t_launch;
             if (r->launched == r->n) last.push_back(r);
         }
-        e = hipMemcpyAsync(ds + s0, hs + s0, (si - s0) * sizeof(TraceSeg), hipMemcpyHostToDevice, s_comp);
-        if (e != hipSuccess) return hip_fail(e, "segment upload");
-        const TraceTuning& tun = g_tuning[PATH_BATCHED][any ? 1 : 0];
+        launches_v.push_back({sc, any, s0, si - s0, start});
+    }
+    // one upload of the step's segment table
+    e = hipMemcpyAsync(ds, hs, si * sizeof(TraceSeg), hipMemcpyHostToDevice, s_comp);
+    if (e != hipSuccess) return hip_fail(e, "segment upload");
+    const int nside = (int)std::min<size_t>(SIDE_STREAMS, launches_v.size() - 1);
+    if (nside > 0) {
+        e = hipEventRecord(ev_up, s_comp);
+        for (int k = 0; k < nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(s_side[k], ev_up, 0);
+        if (e != hipSuccess) return hip_fail(e, "step fork");
+    }
+    for (size_t g = 0; g < launches_v.size(); ++g) {
+        const Launch& L = launches_v[g];
+        const hipStream_t stg = g % (size_t)(nside + 1) == 0 ? s_comp : s_side[g % (size_t)(nside + 1) - 1];
+        const TraceTuning& tun = g_tuning[PATH_BATCHED][L.any ? 1 : 0];
         TraceScratch* scr;
-        int rc = stream_scratch(s_comp, spill, &scr);
+        int rc = stream_scratch(stg, spill, &scr);
         if (rc) return rc;
-        e = trace_segments(sc->view, any, ds + s0, si - s0, start, tun, scr, s_comp);
+        e = trace_segments(L.sc->view, L.any, ds + L.s0, L.nseg, L.rays, tun, scr, stg);
         if (e != hipSuccess) return hip_fail(e, "trace_segments");
         ++launches;
     }
+    for (int k = 0; k < nside && e == hipSuccess; ++k) {
+        e = hipEventRecord(ev_side[k], s_side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s_comp, ev_side[k], 0);
+    }
+    if (e != hipSuccess) return hip_fail(e, "step join");
     e = hipEventRecord(st.end, s_comp);
     if (e != hipSuccess) return hip_fail(e, "step end");
     for (Request* r : last) start_copy_out(r, st.end);
@@ -398,6 +426,7 @@ void Server::run() {
             // a device error ends every admitted request with that code
             std::string msg = pbrtx_last_error();
             (void)hipStreamSynchronize(s_comp);
+            for (hipStream_t st : s_side) (void)hipStreamSynchronize(st);
             (void)hipStreamSynchronize(s_down);
             inflight.clear();
             for (Request* r : active)
@@ -410,6 +439,7 @@ void Server::run() {
     }
     publish();
     (void)hipStreamSynchronize(s_comp);
+    for (hipStream_t st : s_side) (void)hipStreamSynchronize(st);
     (void)hipStreamSynchronize(s_down);
 }
 
@@ -438,6 +468,11 @@ int pbrtx_server_create(const pbrtx_server_config* cfg, pbrtx_server** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.s_comp, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.s_up, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.s_down, hipStreamNonBlocking);
+    for (int k = 0; k < SIDE_STREAMS && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&s.s_side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ev_side[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ev_up, hipEventDisableTiming);
     // at most one chunk per admitted request per step (a step stops taking
     // chunks when its half of the staging is full)
     s.seg_cap = 4096;
@@ -551,15 +586,20 @@ int pbrtx_server_destroy(pbrtx_server* p) {
         s.cv_work.notify_all();
     }
     if (s.worker.joinable()) s.worker.join();
-    for (hipStream_t st : {s.s_comp, s.s_up, s.s_down})
+    std::vector<hipStream_t> streams = {s.s_comp, s.s_up, s.s_down};
+    for (hipStream_t st : s.s_side) streams.push_back(st);
+    for (hipStream_t st : streams)
         if (st) (void)hipStreamSynchronize(st);
     for (Step& st : s.spare) {
         (void)hipEventDestroy(st.start);
         (void)hipEventDestroy(st.end);
     }
+    if (s.ev_up) (void)hipEventDestroy(s.ev_up);
+    for (hipEvent_t ev : s.ev_side)
+        if (ev) (void)hipEventDestroy(ev);
     if (s.h_segs) (void)hipHostFree(s.h_segs);
     if (s.d_segs) (void)hipFree(s.d_segs);
-    for (hipStream_t st : {s.s_comp, s.s_up, s.s_down})
+    for (hipStream_t st : streams)
         if (st) {
             release_stream_scratch(st);
             (void)hipStreamDestroy(st);
