@@ -21,6 +21,11 @@
 //              its children's boxes, and the rounded slab distances are
 //              monotone in the box bounds, so whenever a grandchild passes,
 //              its parent passes too (at the same or a larger tmax).
+//              Occlusion walks its own records (the same leaves under the
+//              builder's own tree): by the same monotonicity a leaf whose
+//              exact box passes has every box above it pass, in any tree,
+//              so the walk reaches exactly the leaves the reference's does
+//              and its answer, which no visiting order changes, is the same.
 #include <mutex>
 
 #include "camera.hpp"
