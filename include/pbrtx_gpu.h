@@ -331,9 +331,11 @@ int pbrtx_server_destroy(pbrtx_server* srv);
    sample + 12 B per pixel) fits arena_bytes; each step gives every admitted
    request of a scene a share of up to batch_samples paths (progressive tiles:
    its next samples in pixel order) and runs them as one wavefront batch whose traversal
-   launches all requests share; batches of different scenes run side by side
-   on their own streams.  PBRTX_SERVE_PLAIN: one request at a time on the
-   plain path.  An image does not depend on the other requests in flight. */
+   launches all requests share; up to eight batches run side by side on
+   their own streams, each with work buffers sized at creation for
+   batch_samples paths (about 200 B a path).  PBRTX_SERVE_PLAIN: one request
+   at a time on the plain path.  An image does not depend on the other
+   requests in flight. */
 typedef struct pbrtx_render_server pbrtx_render_server;
 typedef struct pbrtx_render_server_config {
     uint64_t arena_bytes;    /* HBM for admitted requests (admission budget) */
