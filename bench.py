@@ -24,18 +24,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # HIP maps every stream of a process onto GPU_MAX_HW_QUEUES hardware queues
 # (4 by default).  The request servers overlap uploads, traversal and
-# downloads on three streams, and the renderer holds streams of its own: with
-# 4 queues the server's compute stream can share a queue with its copy
-# streams, whose waits then hold its kernels back (at 70% of the saturated
-# rate the ray server fell behind: 0.91 of the offered rate, p50 161 ms;
-# with 8 queues 1.00, p50 2.9 ms).  Read by the HIP runtime when it starts,
-# so raised to 8 before the library loads (a larger setting is kept).
+# downloads on several streams (the ray server six, the render server eight
+# batch streams), and the renderer holds streams of its own: with 4 queues
+# the ray server's compute stream can share a queue with its copy streams,
+# whose waits then hold its kernels back (at 70% of the saturated rate it
+# fell behind: 0.91 of the offered rate, p50 161 ms; with 8 queues 1.00, p50
+# 2.9 ms), and the render server gained a few per cent more from 16 queues
+# than from 8.  Read by the HIP runtime when it starts, so raised to 16
+# before the library loads (a larger setting is kept).
 try:
     _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
 except ValueError:
     _hwq = 4
-if _hwq < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+if _hwq < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector / matrix FP32

@@ -279,12 +279,14 @@ int pbrtx_renderer_destroy(pbrtx_renderer* r);
    and large requests progress in chunks.  PBRTX_SERVE_PLAIN: one request at a
    time, one plain-path kernel per request.  A ray's result does not depend on
    the other requests in flight.
-   The worker overlaps uploads, traversal and downloads on three streams, which
-   overlap only on different hardware queues: HIP maps all of a process's
-   streams onto GPU_MAX_HW_QUEUES queues (4 by default), so a process that also
-   holds other streams (a renderer, a second server) should set it to 8 or more
-   before HIP starts (measured: at 70% of the saturated rate, 0.91 of the
-   offered rate served with 4 queues beside a renderer, 1.00 with 8). */
+   The worker overlaps uploads, traversal and downloads on six streams
+   (compute, three side streams for a step's launches, upload, download),
+   which overlap only on different hardware queues: HIP maps all of a
+   process's streams onto GPU_MAX_HW_QUEUES queues (4 by default), so a
+   process that also holds other streams (a renderer, a second server) should
+   set it to 8 or more before HIP starts (measured: at 70% of the saturated
+   rate, 0.91 of the offered rate served with 4 queues beside a renderer, 1.00
+   with 8). */
 #define PBRTX_SERVE_BATCHED 0
 #define PBRTX_SERVE_PLAIN 1
 typedef struct pbrtx_server pbrtx_server;

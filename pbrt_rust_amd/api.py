@@ -348,9 +348,9 @@ class RayServer:
     continuous batching ("batched") or one plain kernel per request
     ("plain").  submit() returns a ticket; wait(ticket) blocks and returns the
     result (hit records or a bool array) and the request's timings (ms since
-    the server started).  Its three streams overlap only on separate hardware
+    the server started).  Its streams overlap only on separate hardware
     queues: in a process holding other streams too, set GPU_MAX_HW_QUEUES=8
-    (HIP's default is 4) before the library loads."""
+    or more (HIP's default is 4) before the library loads."""
 
     MODES = {"batched": 0, "plain": 1}
 
