@@ -333,7 +333,7 @@ int pbrtx_server_destroy(pbrtx_server* srv);
    sample + 12 B per pixel) fits arena_bytes; each step gives every admitted
    request of a scene a share of up to batch_samples paths (progressive tiles:
    its next samples in pixel order) and runs them as one wavefront batch whose traversal
-   launches all requests share; up to eight batches run side by side on
+   launches all requests share; up to sixteen batches run side by side on
    their own streams, each with work buffers sized at creation for
    batch_samples paths (about 200 B a path).  PBRTX_SERVE_PLAIN: one request
    at a time on the plain path.  An image does not depend on the other

@@ -44,11 +44,16 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 // Batches in flight, one stream each.  Saturated render serving (17 scenes,
-// 8M-sample batches) at 2 / 3 / 4 / 5 / 6 / 8 / 12 slots: 5.36 / 6.66-6.71 /
-// 7.52-7.86 / 7.48 / 7.93 / 8.07-8.43 / 8.08-8.18 Grays/s: a batch's later
-// bounces are small launches that leave most of the GPU to the others.
-// Each slot's work buffers are sized once for batch_samples.
-constexpr int NSLOT = 8;
+// 8M-sample batches) at 2 / 3 / 4 / 5 / 6 / 8 / 12 slots with 8 hardware
+// queues: 5.36 / 6.66-6.71 / 7.52-7.86 / 7.48 / 7.93 / 8.07-8.43 /
+// 8.08-8.18 Grays/s; at 8 / 12 / 16 / 24 / 32 slots with 16 queues:
+// 8.36-8.44 / 8.89-8.96 / 9.26-9.79 / 9.35 / 9.83 Grays/s (with 32 queues
+// everything ran at half speed); paired in the full benchmark, six passes
+// each: 16 slots 8.79-9.96 (mean 9.19), 8 slots 8.61-9.46 (mean 9.04).
+// A batch's later bounces are small launches
+// that leave most of the GPU to the others.  Each slot's work buffers are
+// sized once for batch_samples.
+constexpr int NSLOT = 16;
 constexpr uint64_t SHARE_MIN = 65536;  // smallest per-request share of a batch (samples)
 
 struct RRequest {
