@@ -585,6 +585,11 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) 
             // leaf code, so leaves are taken in bunches.  On the 1024x1024
             // direct-lighting frame, occlusion 3.87 -> 3.45 ms at 25 % (15 %:
             // 3.52, 35 %: 3.52, 50 %: 3.62); the path-tracing frame gains too.
+            // Tried and slower there: no waiting, with each leaf reached put
+            // in a per-wave queue in LDS and the whole wave taking 64 queued
+            // leaves at a time, one per lane, on rays read across lanes
+            // (exact, but 3.19 -> 3.58-3.78 ms: a round lasts as long as its
+            // largest leaf, and lanes whose rays wait on queued leaves idle).
             const unsigned long long at_rec = __ballot(has && cur >= 0);
             const unsigned long long at_leaf = __ballot(has && cur < 0);
             const bool leaf_now = at_rec == 0ull || (uint32_t)__popcll(at_leaf) * 100u >=
